@@ -1,0 +1,325 @@
+#!/usr/bin/env python3
+"""Benchmark of the select -> hash-join -> aggregate path on one MI355X.
+
+Headline (one JSON line on stdout): probe throughput of the C2 workload --
+SELECT COUNT(*), SUM(R.v + S.w) FROM R JOIN S ON R.k = S.k with |S| = 2^24
+(keys a seeded permutation of 0..2^24-1, w in [0,2^16)), |R| = 2^28 (k in
+[0,2^25), v in [0,2^16)), open addressing at load factor 0.5.  A step is one
+probe pass of R against the resident table (inputs resident in HBM), timed K
+times between device synchronisations; the build is timed separately.
+
+Extra legs (outside the timed steps) report the filtered-sum scan (C1) and
+later configurations as extra JSON keys.  Progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "multigpu-vector_amd"))
+
+import numpy as np  # noqa: E402
+
+import mgv  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def gb(x):
+    return x / 1e9
+
+
+class Timer:
+    """HIP events on one stream: per-launch kernel durations."""
+
+    def __init__(self, stream):
+        self.s = stream
+        self.ev = []
+
+    def around(self, fn):
+        a, b = mgv.Event(), mgv.Event()
+        a.record(self.s)
+        fn()
+        b.record(self.s)
+        self.ev.append((a, b))
+
+    def durations_ms(self):
+        out = [a.elapsed_ms(b) for a, b in self.ev]
+        for a, b in self.ev:
+            a.close()
+            b.close()
+        self.ev = []
+        return out
+
+
+def gen_bits(n, seed, stream, bits, s=None):
+    buf = mgv.DeviceBuffer(4 * n)
+    mgv.call("mgv_gen_bits", buf.vp, n, seed, stream, bits, s)
+    return buf
+
+
+def gen_perm(n, rng, seed, stream, s=None):
+    buf = mgv.DeviceBuffer(4 * n)
+    mgv.call("mgv_gen_perm", buf.vp, n, rng, seed, stream, s)
+    return buf
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/traffic.json)."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+# ------------------------------------------------------------------ C2 join
+
+def leg_c2(args, st):
+    ns, nr = 1 << 24, 1 << 28
+    log(f"C2: generating S ({ns} rows) and R ({nr} rows)")
+    sk = gen_perm(ns, ns, mgv.seed(2, 0), 0, st.vp)
+    sw = gen_bits(ns, mgv.seed(2, 1), 0, 16, st.vp)
+    rk = gen_bits(nr, mgv.seed(2, 2), 0, 25, st.vp)
+    rv = gen_bits(nr, mgv.seed(2, 3), 0, 16, st.vp)
+    ht = mgv.HashTable(ns, args.load_factor)
+    res = mgv.ResultBuffer()
+    st.sync()
+    L = mgv.launch(0, args.block, -1)
+
+    # build (timed: clear excluded)
+    build_ms = []
+    for rep in range(3):
+        ht.clear(st.vp)
+        res.clear(st.vp)
+        t = Timer(st.vp)
+        t.around(lambda: mgv.call("mgv_hashjoin_build", ht.buf.vp, ht.nbuckets, sk.vp, sw.vp,
+                                  mgv.MGV_PAYLOAD_COLUMN, None, 0, ns, 0, res.vp, L, st.vp))
+        st.sync()
+        build_ms.append(t.durations_ms()[0])
+    assert res.read().errors == 0
+    build_t = min(build_ms[1:])
+    log(f"C2: build {build_t:.3f} ms ({gb(ns) / (build_t / 1e3):.2f} Grows/s)")
+
+    def probe_step(timer=None):
+        res.clear(st.vp)
+        fn = lambda: mgv.call("mgv_hashjoin_probe", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp, nr, 0, 0,
+                              res.vp, L, st.vp)
+        if timer:
+            timer.around(fn)
+        else:
+            fn()
+
+    for _ in range(args.warmup):
+        probe_step()
+    st.sync()
+    # timed steps: wall clock around K steps, events around each probe launch
+    timer = Timer(st.vp)
+    mgv.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        probe_step(timer)
+    st.sync()
+    mgv.sync()
+    wall = time.perf_counter() - t0
+    kms = timer.durations_ms()
+    r = res.read()
+
+    # parity: COUNT has a closed form (#{R.k < 2^24}), counted by the select kernel
+    chk = mgv.ResultBuffer()
+    mgv.call("mgv_select_sum", rk.vp, rv.vp, nr, ns - 1, chk.vp, mgv.launch(0, 256, 0), st.vp)
+    st.sync()
+    c = chk.read()
+    ok_count = r.count == c.count
+    log(f"C2: probe count {r.count} (closed form {c.count}) sum {r.sum}")
+
+    ms_per_step = wall * 1e3 / args.steps
+    avg_kernel_ms = float(np.mean(kms))
+    algo_bytes = nr * 16
+    out = {
+        "ms_per_step": ms_per_step,
+        "value": nr / (ms_per_step / 1e3) / 1e9,
+        "kernel_ms_avg": avg_kernel_ms,
+        "kernel_ms_min": float(np.min(kms)),
+        "kernel_ms_median": float(np.median(kms)),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": algo_bytes / (avg_kernel_ms / 1e3) / 1e9,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": algo_bytes / (avg_kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBPS,
+            "traffic": traffic_from_profiles("probe_c2"),
+            "kernel": "k_pipeline<PProbeSum<false,true,false>,256>",
+            "algorithmic_bytes_per_launch": algo_bytes,
+        },
+        "build": {"ms": build_t, "grows_per_s": gb(ns) / (build_t / 1e3),
+                  "gbps_algorithmic": (ns * 8 + ht.nbuckets * 64) / (build_t / 1e3) / 1e9},
+        "result": {"count": r.count, "sum": r.sum, "count_closed_form_ok": ok_count},
+    }
+
+    if args.cpu:
+        out["cpu_baseline"], out["sample_parity"] = cpu_baseline_c2(args, st, sk, sw, rk, rv, ht, ns, nr, L)
+    for b in (sk, sw, rk, rv, ht.buf, res, chk):
+        b.close()
+    return out
+
+
+def cpu_baseline_c2(args, st, sk, sw, rk, rv, ht, ns, nr, L):
+    """Oracle chained-table join (the reference's algorithm, main8.cu:84-103)
+    on a bounded sample: full S build, probe of the first M rows of R.  The GPU
+    probes the same M rows and must give the same COUNT and SUM."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py
+    orc = oracle_py.Oracle()
+    st.sync()
+    hk, hw = sk.download(np.int32, ns), sw.download(np.int32, ns)
+    M = args.cpu_sample
+    hrk, hrv = rk.download(np.int32, M), rv.download(np.int32, M)
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    t = orc.table(hk, hw, log_buckets=25)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    want = orc.probe(t, hrk, hrv, threads=threads)
+    t_probe = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    want1 = orc.probe(t, hrk[: M // 8], hrv[: M // 8], threads=1)
+    t_probe1 = time.perf_counter() - t1
+    orc.free(t)
+    res = mgv.ResultBuffer()
+    mgv.call("mgv_hashjoin_probe", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp, M, 0, 0, res.vp, L, st.vp)
+    st.sync()
+    g = res.read()
+    res.close()
+    parity = {"rows": M, "gpu": [g.count, g.sum], "cpu": [want["count"], want["sum"]],
+              "equal": (g.count, g.sum) == (want["count"], want["sum"])}
+    log(f"C2 cpu: build {t_build:.2f}s probe {M} rows {t_probe:.2f}s on {threads} threads; parity {parity['equal']}")
+    cpu = {"value": M / t_probe / 1e9, "unit": "Grows/s", "cores": threads, "kind": "port",
+           "sample": f"oracle chained hash join: build all {ns} S rows ({t_build:.2f} s, 1 thread), "
+                     f"probe first {M} R rows",
+           "single_thread_value": (M // 8) / t_probe1 / 1e9, "build_s": t_build,
+           "cpu_model": cpu_model()}
+    return cpu, parity
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+# ------------------------------------------------------------------ C1 scan
+
+def leg_c1(args, st):
+    out = {}
+    bound = (1 << 30) - 1
+    for n, reps_in_graph in ((1 << 24, 100), (1 << 28, 1)):
+        a = gen_bits(n, mgv.seed(1, 0), 0, 20, st.vp)
+        b = gen_bits(n, mgv.seed(1, 1), 0, 31, st.vp)
+        res = mgv.ResultBuffer(reps_in_graph)
+        st.sync()
+        best = None
+        for compact in (0, 1):
+            L = mgv.launch(0, args.block, compact)
+            # capture reps_in_graph back-to-back scans in one graph
+            mgv.call("mgv_graph_begin_capture", st.vp)
+            for i in range(reps_in_graph):
+                mgv.call("mgv_select_sum", b.vp, a.vp, n, bound, res.slot(i), L, st.vp)
+            g = C.c_void_p()
+            mgv.call("mgv_graph_end_capture", st.vp, C.byref(g))
+            times = []
+            for rep in range(4):
+                res.clear(st.vp)
+                t = Timer(st.vp)
+                t.around(lambda: mgv.call("mgv_graph_launch", g, st.vp))
+                st.sync()
+                times.append(t.durations_ms()[0] / reps_in_graph)
+            mgv.call("mgv_graph_destroy", g)
+            r = res.read(0)
+            kms = min(times[1:])
+            gbps = n * 8 / (kms / 1e3) / 1e9
+            log(f"C1 n={n} compact={compact}: {kms:.4f} ms/scan, {gbps:.0f} GB/s, count {r.count} sum {r.sum}")
+            if best is None or kms < best["ms"]:
+                best = {"ms": kms, "gbps": gbps, "frac": gbps / HBM_PEAK_GBPS, "compact": compact,
+                        "count": r.count, "sum": r.sum, "grows_per_s": n / (kms / 1e3) / 1e9}
+        key = "c1_2^24_graph100" if n == 1 << 24 else "c1_2^28"
+        out[key] = best
+        for x in (a, b, res):
+            x.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--load-factor", type=float, default=0.5)
+    ap.add_argument("--legs", default="c1")
+    ap.add_argument("--cpu", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 25)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    mgv.load()
+    mgv.init(0)
+    p = mgv.props()
+    log(f"device {p.name.decode()} {p.arch.decode()} CUs={p.cus} HBM={p.hbm_bytes >> 30} GiB")
+    st = mgv.Stream()
+    c2 = leg_c2(args, st)
+    extra = {}
+    legs = [x for x in args.legs.split(",") if x]
+    if "c1" in legs:
+        extra["c1"] = leg_c1(args, st)
+
+    line = {
+        "metric": "probe Grows/s & %HBM peak; batched vs plain queries/s at bit-exact, p99 lat",
+        "value": c2["value"],
+        "unit": "Grows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": c2["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded Philox4x32-10 columns generated on the GPU)",
+        "config": {"workload": "C2 hash join + aggregate: probe 2^28 R rows against a resident open-addressing "
+                               "table of 2^24 S rows at load factor 0.5; COUNT(*), SUM(R.v+S.w) int64",
+                   "build_rows": 1 << 24, "probe_rows": 1 << 28, "load_factor": args.load_factor,
+                   "block": args.block, "parallelism": "1 GPU"},
+        "roofline": c2["roofline"],
+        "cpu_baseline": c2.get("cpu_baseline"),
+        "sample_parity": c2.get("sample_parity"),
+        "probe_kernel_ms": {k: c2[k] for k in ("kernel_ms_avg", "kernel_ms_min", "kernel_ms_median")},
+        "build": c2["build"],
+        "result": c2["result"],
+        "device": {"name": p.name.decode(), "arch": p.arch.decode(), "cus": p.cus},
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

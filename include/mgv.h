@@ -1,0 +1,230 @@
+/*
+ * mgv.h -- C ABI of libmgv.so, the MI355X (gfx950) implementation of the
+ * vector-at-a-time select -> hash-join -> aggregation operator path of
+ * chapeiro/multigpu-vector.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every function returns 0 on
+ * success, otherwise a HIP error code or one of the MGV_E* codes below, and
+ * records a message for mgv_last_error_string().  Nothing here calls exit().
+ * Device pointers come from mgv_malloc; every launch takes an explicit stream
+ * (NULL = the library's default stream of the current device).  Calls are
+ * thread-safe for distinct streams.
+ *
+ * Each entry point names the reference interface it replaces (path:line,
+ * relative to the reference tree).  The reference has no C ABI of its own:
+ * its boundary is the C++ operator concept dispatched through
+ * d_operator_t / h_operator_t (src/operators/operator.cuh:42-124) plus the
+ * buffer_manager / buffer_pool contract; the C++ drop-in layer in
+ * multigpu-vector_amd/host/ re-exposes that concept on top of this ABI.
+ */
+#ifndef MGV_H_
+#define MGV_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGV_ABI_VERSION 1
+
+/* Error codes beyond HIP's own (hipError_t values are < 10000). */
+#define MGV_OK 0
+#define MGV_E_INVALID 10001     /* bad argument / shape */
+#define MGV_E_NOT_INIT 10002    /* mgv_init not called on this thread's device */
+#define MGV_E_CAPACITY 10003    /* output / pool capacity exceeded */
+#define MGV_E_ALIGN 10004       /* pointer alignment requirement violated */
+#define MGV_E_RESERVED 10005    /* build row equal to the reserved empty slot */
+#define MGV_E_NOGPU 10006       /* no HIP device */
+
+typedef void *mgv_stream_t;   /* hipStream_t */
+typedef void *mgv_event_t;    /* hipEvent_t */
+typedef void *mgv_graph_t;    /* hipGraphExec_t (+ graph) */
+
+/* ------------------------------------------------------------ runtime */
+
+/* Select `device`, create the library's default stream and read device
+ * properties.  Replaces the device part of buffer_manager<int32_t>::init
+ * (src/buffer_manager.cu:38-211) and the gpu()/exit error convention
+ * (src/common.cuh:12-23). */
+int mgv_init(int device);
+int mgv_shutdown(void);
+int mgv_abi_version(void);
+const char *mgv_last_error_string(void);
+
+typedef struct mgv_device_props {
+    char name[64];
+    char arch[32];            /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+    int32_t cus;              /* compute units */
+    int32_t wave_size;        /* 64 */
+    int64_t lds_per_block;    /* bytes */
+    int64_t hbm_bytes;
+    int64_t l2_bytes;
+    int32_t clock_khz;
+    int32_t mem_clock_khz;
+    int32_t mem_bus_width;
+    int32_t max_threads_per_block;
+} mgv_device_props;
+int mgv_get_device_props(mgv_device_props *out);
+
+/* Memory: device, pinned host.  cudaMalloc / cudaMallocHost call sites of
+ * src/buffer.cuh:34-41, src/buffer_manager.cu:98-176. */
+int mgv_malloc(void **ptr, size_t bytes);
+int mgv_free(void *ptr);
+int mgv_host_malloc(void **ptr, size_t bytes);
+int mgv_host_free(void *ptr);
+int mgv_mem_get_info(size_t *free_bytes, size_t *total_bytes);
+
+/* Copies (direction inferred from the pointers).  Replaces
+ * buffer_manager::overwrite (src/buffer_manager.cu:214-218) and the
+ * buffer_inspector copies (src/buffer.cuh:183-217). */
+int mgv_memcpy(void *dst, const void *src, size_t bytes);
+int mgv_memcpy_async(void *dst, const void *src, size_t bytes, mgv_stream_t s);
+
+/* Fill with a 32-/64-bit pattern by kernel (graph-safe; no memset nodes). */
+int mgv_fill_u32(uint32_t *dst, uint64_t n, uint32_t value, mgv_stream_t s);
+int mgv_fill_u64(uint64_t *dst, uint64_t n, uint64_t value, mgv_stream_t s);
+
+/* Streams / events / sync (the reference's per-consumer streams,
+ * src/operators/exchange.cu:293-318).  Streams are created at default
+ * priority. */
+int mgv_stream_create(mgv_stream_t *s);
+int mgv_stream_destroy(mgv_stream_t s);
+int mgv_stream_synchronize(mgv_stream_t s);
+int mgv_device_synchronize(void);
+int mgv_event_create(mgv_event_t *e);
+int mgv_event_destroy(mgv_event_t e);
+int mgv_event_record(mgv_event_t e, mgv_stream_t s);
+int mgv_event_synchronize(mgv_event_t e);
+int mgv_event_query(mgv_event_t e);   /* 0 = complete, 1 = pending, else error */
+int mgv_event_elapsed_ms(float *ms, mgv_event_t start, mgv_event_t stop);
+int mgv_stream_wait_event(mgv_stream_t s, mgv_event_t e);
+
+/* Graphs: capture a stream into an executable graph and replay it. */
+int mgv_graph_begin_capture(mgv_stream_t s);
+int mgv_graph_end_capture(mgv_stream_t s, mgv_graph_t *g);
+int mgv_graph_launch(mgv_graph_t g, mgv_stream_t s);
+int mgv_graph_destroy(mgv_graph_t g);
+
+/* ------------------------------------------------------------ launch shape */
+
+/* Launch shape of a pipeline.  Mirrors launch_conf{gridDim, blockDim,
+ * shared_mem, device} (src/common.cuh:183-209): grid 0 = persistent grid
+ * sized from the CU count; block in {64,128,256,512,1024}; compact selects
+ * LDS stream compaction (1) or predication (0) between select and its parent
+ * (-1 = the operator's default). */
+typedef struct mgv_launch {
+    uint32_t grid;
+    uint32_t block;
+    int32_t compact;
+    uint32_t reserved;
+} mgv_launch;
+
+/* ------------------------------------------------------------ datagen */
+
+/* Counter-based Philox4x32-10 columns; identical to the oracle's generator. */
+int mgv_gen_bits(int32_t *dst, uint64_t n, uint64_t seed, uint32_t stream, uint32_t bits, mgv_stream_t s);
+int mgv_gen_range(int32_t *dst, uint64_t n, uint64_t seed, uint32_t stream, uint32_t range, mgv_stream_t s);
+int mgv_gen_perm(int32_t *dst, uint64_t n, uint32_t range, uint64_t seed, uint32_t pstream, mgv_stream_t s);
+int mgv_gen_iota(int32_t *dst, uint64_t n, int32_t base, mgv_stream_t s);
+/* Zipf(s) thresholds (host-computed, uploaded to *table, nranks+... entries). */
+int mgv_zipf_table_create(uint64_t **table, uint32_t nranks, double s_exp);
+int mgv_gen_zipf(int32_t *dst, uint64_t n, uint64_t seed, uint32_t stream, const uint64_t *table,
+                 uint32_t nranks, uint64_t pseed, uint32_t pstream, mgv_stream_t s);
+
+/* ------------------------------------------------------------ results */
+
+/* Scalar result of an aggregating pipeline, device resident, accumulated with
+ * atomics (clear before use).  count/sum replace the int32 `res` of
+ * aggregation (src/operators/aggregation.cuh:12) and hashjoin
+ * (src/operators/hashjoin.cuh:41) with exact 64-bit values. */
+typedef struct mgv_result {
+    uint64_t count;
+    int64_t sum;
+    uint64_t digest_sum;      /* sum of mix64(r_row << 32 | s_payload) over join matches */
+    uint64_t digest_xor;      /* xor of the same */
+    uint64_t errors;          /* rejected rows (reserved build pattern) */
+    uint64_t overflow;        /* rows that did not fit an output buffer */
+    uint64_t reserved[2];
+} mgv_result;
+int mgv_result_clear(mgv_result *r, uint32_t n, mgv_stream_t s);
+
+/* ------------------------------------------------------------ hash table */
+
+/* Open-addressing hash table: nbuckets buckets of 8 slots of 8 bytes
+ * {key:int32, payload:int32} (64-byte buckets), bucket = (fmix32(key) *
+ * nbuckets) >> 32, bucketised linear probing.  The slot pattern
+ * {key=-1, payload=-1} marks an empty slot and is rejected by the build.
+ * Replaces the chained first[]/next[] table of hashjoin::hashjoin
+ * (src/operators/hashjoin.cu:63-75). */
+#define MGV_SLOTS_PER_BUCKET 8
+int mgv_ht_plan(uint64_t nkeys, double load_factor, uint32_t *nbuckets, uint64_t *bytes);
+int mgv_ht_clear(uint64_t *slots, uint32_t nbuckets, mgv_stream_t s);
+
+/* payload modes of the build */
+#define MGV_PAYLOAD_NONE 0     /* payload 0 (the reference's keys-only table) */
+#define MGV_PAYLOAD_COLUMN 1   /* payload[i] */
+#define MGV_PAYLOAD_ROWID 2    /* row_base + i (for join-pair digests) */
+
+/* Build: insert every row i (optionally only rows with pred[i] <= bound).
+ * Replaces hashjoin_builder::consume_warp (src/operators/hashjoin.cu:41-60).
+ * Rejected rows are counted in res->errors when res != NULL. */
+int mgv_hashjoin_build(uint64_t *slots, uint32_t nbuckets, const int32_t *keys, const int32_t *payload,
+                       int payload_mode, const int32_t *pred, int32_t bound, uint64_t n, int64_t row_base,
+                       mgv_result *res, mgv_launch launch, mgv_stream_t s);
+
+/* Probe flags */
+#define MGV_PROBE_DIGEST 1     /* also accumulate the match-pair digest (payload = s row id) */
+
+/* Probe + aggregate: for every probe row r (optionally pred[r] <= bound),
+ * for every slot with key == keys[r]: count += 1, sum += vals[r] + payload
+ * (vals may be NULL).  Replaces hashjoin::consume_warp + at_close
+ * (src/operators/hashjoin.cu:88-96,135-182) fused with aggregation
+ * (src/operators/aggregation.cu:27-66). */
+int mgv_hashjoin_probe(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
+                       const int32_t *keys, const int32_t *vals, uint64_t n, int64_t row_base, uint32_t flags,
+                       mgv_result *res, mgv_launch launch, mgv_stream_t s);
+
+/* Probe + group-by: group g = payload of the matching slot (g < ngroups),
+ * cnt[g] += 1, sum[g] += vals[r].  No reference counterpart for group-by;
+ * the probe part replaces src/operators/hashjoin.cu:135-182. */
+int mgv_hashjoin_probe_groupby(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
+                               const int32_t *keys, const int32_t *vals, uint64_t n, uint64_t *gcnt,
+                               int64_t *gsum, uint32_t ngroups, mgv_launch launch, mgv_stream_t s);
+
+/* Probe + materialise the match multiset as (r_row, payload) pairs appended
+ * to out_r/out_p (capacity cap); res->count = total matches, res->overflow =
+ * pairs beyond cap.  Replaces the hashjoin -> gpu_to_cpu chain of
+ * src/main9.cu:326-367. */
+int mgv_hashjoin_probe_materialize(const uint64_t *slots, uint32_t nbuckets, const int32_t *keys, uint64_t n,
+                                   int64_t row_base, int32_t *out_r, int32_t *out_p, uint64_t cap,
+                                   mgv_result *res, mgv_launch launch, mgv_stream_t s);
+
+/* ------------------------------------------------------------ select / aggregate */
+
+/* SELECT COUNT(*), SUM(val) WHERE pred <= bound.  Replaces the
+ * unstable_select -> aggregation chain (src/operators/select3.cu:19-85,
+ * src/operators/aggregation.cu:19-66, plan src/main9.cu:202-226).  val may
+ * equal pred (the reference's single-column form). */
+int mgv_select_sum(const int32_t *pred, const int32_t *val, uint64_t n, int32_t bound, mgv_result *res,
+                   mgv_launch launch, mgv_stream_t s);
+
+/* Stream compaction: out = {val[i] : pred[i] <= bound} in unspecified order
+ * (res->count = selected rows, res->overflow = rows beyond cap).  Replaces
+ * unstable_select with its gpu_to_cpu parent (src/operators/select3.cu:19-270,
+ * src/buffer.cuh:92-134, plan src/main9.cu:150-200). */
+int mgv_select_materialize(const int32_t *pred, const int32_t *val, uint64_t n, int32_t bound, int32_t *out,
+                           uint64_t cap, mgv_result *res, mgv_launch launch, mgv_stream_t s);
+
+/* GROUP BY key: cnt[key] += 1, sum[key] += val, keys in [0, ngroups)
+ * (other keys counted in res->errors when res != NULL).  No reference
+ * counterpart (aggregation.cu is scalar only). */
+int mgv_groupby(const int32_t *keys, const int32_t *vals, uint64_t n, uint64_t *gcnt, int64_t *gsum,
+                uint32_t ngroups, mgv_result *res, mgv_launch launch, mgv_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MGV_H_ */

@@ -1,0 +1,56 @@
+// mgv_internal.hpp -- host-side helpers shared by the libmgv translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/mgv.h"
+
+namespace mgv {
+
+// Per-thread last error message (mgv_last_error_string).
+void set_error(const char *fmt, ...);
+int fail(int code, const char *fmt, ...);
+
+struct DeviceState {
+    bool init = false;
+    int device = -1;
+    int cus = 0;
+    size_t lds_per_block = 0;
+    hipStream_t default_stream = nullptr;
+};
+
+// State of the calling thread's current device (mgv_init must have run).
+DeviceState *current_state();
+
+inline hipStream_t stream_of(mgv_stream_t s) {
+    if (s) return (hipStream_t)s;
+    DeviceState *st = current_state();
+    return st ? st->default_stream : nullptr;
+}
+
+inline int hip_status(hipError_t e, const char *what) {
+    if (e == hipSuccess) return MGV_OK;
+    return fail((int)e, "%s: %s", what, hipGetErrorString(e));
+}
+
+// Check the launch that was just issued.
+inline int check_launch(const char *what) { return hip_status(hipGetLastError(), what); }
+
+inline bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+// Persistent grid for a kernel: CUs x resident blocks per CU (cached by caller).
+template <typename K>
+int persistent_blocks(K kernel, int block) {
+    DeviceState *st = current_state();
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    int cus = (st && st->cus > 0) ? st->cus : 256;
+    return cus * per_cu;
+}
+
+}  // namespace mgv
