@@ -187,7 +187,12 @@ class DeviceBuffer:
 
     @property
     def vp(self) -> VP:
-        return VP(self.ptr)
+        # The pointer object keeps this buffer alive while it is referenced
+        # (e.g. inside an argument list), so `device_array(x).vp` cannot be
+        # freed before the call that uses it.
+        v = VP(self.ptr)
+        v._owner = self
+        return v
 
     def close(self):
         if self.ptr:
@@ -214,7 +219,9 @@ class DeviceBuffer:
         return out
 
     def at(self, byte_offset: int) -> VP:
-        return VP(self.ptr + byte_offset)
+        v = VP(self.ptr + byte_offset)
+        v._owner = self
+        return v
 
 
 def device_array(arr: np.ndarray) -> DeviceBuffer:
@@ -239,7 +246,9 @@ class ResultBuffer(DeviceBuffer):
         return r
 
     def slot(self, i: int) -> VP:
-        return VP(self.ptr + i * C.sizeof(Result))
+        v = VP(self.ptr + i * C.sizeof(Result))
+        v._owner = self
+        return v
 
 
 def ht_plan(nkeys: int, load_factor: float = 0.5):

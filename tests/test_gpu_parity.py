@@ -305,7 +305,8 @@ def test_groupby_matches_oracle(gpu, orc, ngroups):
     gpu.call("mgv_fill_u64", gc.vp, ngroups, 0, None)
     gpu.call("mgv_fill_u64", gs.vp, ngroups, 0, None)
     res = mgv.ResultBuffer()
-    gpu.call("mgv_groupby", dev(k).vp, dev(v).vp, n, gc.vp, gs.vp, ngroups, res.vp, mgv.launch(0, 256), None)
+    dk, dv = dev(k), dev(v)
+    gpu.call("mgv_groupby", dk.vp, dv.vp, n, gc.vp, gs.vp, ngroups, res.vp, mgv.launch(0, 256), None)
     gpu_sync()
     wc, ws = orc.groupby(k, v, ngroups)
     assert np.array_equal(gc.download(np.uint64, ngroups), wc)
