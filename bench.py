@@ -111,19 +111,48 @@ def leg_c2(args, st):
     build_t = min(build_ms[1:])
     log(f"C2: build {build_t:.3f} ms ({gb(ns) / (build_t / 1e3):.2f} Grows/s)")
 
+    ws = None
+
+    def make_probe(fanout):
+        nonlocal ws
+        if fanout > 1:
+            need = C.c_uint64()
+            mgv.call("mgv_partition_workspace_size", nr, fanout, 2, C.byref(need))
+            if ws is None or ws.nbytes < need.value:
+                if ws is not None:
+                    ws.close()
+                ws = mgv.DeviceBuffer(need.value)
+            return lambda: mgv.call("mgv_hashjoin_probe_partitioned", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp,
+                                    nr, 0, 0, fanout, ws.vp, ws.nbytes, res.vp, L, st.vp)
+        return lambda: mgv.call("mgv_hashjoin_probe", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp, nr, 0, 0,
+                                res.vp, L, st.vp)
+
+    # choose the probe path: direct (fan-out 1) or partitioned
+    candidates = [args.fanout] if args.fanout >= 0 else [1, 64, 128, 256]
+    sweep = {}
+    for f in candidates:
+        fn = make_probe(f)
+        t = Timer(st.vp)
+        for rep in range(3):
+            res.clear(st.vp)
+            t.around(fn)
+        st.sync()
+        sweep[f] = min(t.durations_ms()[1:] or t.durations_ms())
+        log(f"C2: probe fan-out {f}: {sweep[f]:.3f} ms")
+    fanout = min(sweep, key=sweep.get)
+    probe_fn = make_probe(fanout)
+
     def probe_step(timer=None):
         res.clear(st.vp)
-        fn = lambda: mgv.call("mgv_hashjoin_probe", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp, nr, 0, 0,
-                              res.vp, L, st.vp)
         if timer:
-            timer.around(fn)
+            timer.around(probe_fn)
         else:
-            fn()
+            probe_fn()
 
     for _ in range(args.warmup):
         probe_step()
     st.sync()
-    # timed steps: wall clock around K steps, events around each probe launch
+    # timed steps: wall clock around K steps, events around each probe call
     timer = Timer(st.vp)
     mgv.sync()
     t0 = time.perf_counter()
@@ -158,18 +187,22 @@ def leg_c2(args, st):
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": algo_bytes / (avg_kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBPS,
-            "traffic": traffic_from_profiles("probe_c2"),
-            "kernel": "k_pipeline<PProbeSum<false,true,false>,256>",
+            "traffic": traffic_from_profiles(f"probe_c2_f{fanout}"),
+            "kernel": ("k_pipeline<PProbeSum<false,true,false>,256>" if fanout == 1 else
+                       "mgv_hashjoin_probe_partitioned: k_part_prep+k_part_count+k_part_scan+k_part_scatter+"
+                       "k_part_probe<PProbeSum<false,true,false>,256> (events around the whole call)"),
+            "fanout": fanout,
             "algorithmic_bytes_per_launch": algo_bytes,
         },
         "build": {"ms": build_t, "grows_per_s": gb(ns) / (build_t / 1e3),
                   "gbps_algorithmic": (ns * 8 + ht.nbuckets * 64) / (build_t / 1e3) / 1e9},
         "result": {"count": r.count, "sum": r.sum, "count_closed_form_ok": ok_count},
+        "fanout_sweep_ms": {str(k): v for k, v in sweep.items()},
     }
 
     if args.cpu:
         out["cpu_baseline"], out["sample_parity"] = cpu_baseline_c2(args, st, sk, sw, rk, rv, ht, ns, nr, L)
-    for b in (sk, sw, rk, rv, ht.buf, res, chk):
+    for b in (sk, sw, rk, rv, ht.buf, res, chk) + ((ws,) if ws is not None else ()):
         b.close()
     return out
 
@@ -270,6 +303,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--load-factor", type=float, default=0.5)
+    ap.add_argument("--fanout", type=int, default=-1, help="1 = direct probe, >1 = partitioned, -1 = sweep")
     ap.add_argument("--legs", default="c1")
     ap.add_argument("--cpu", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=1 << 25)
@@ -308,11 +342,12 @@ def main():
         "config": {"workload": "C2 hash join + aggregate: probe 2^28 R rows against a resident open-addressing "
                                "table of 2^24 S rows at load factor 0.5; COUNT(*), SUM(R.v+S.w) int64",
                    "build_rows": 1 << 24, "probe_rows": 1 << 28, "load_factor": args.load_factor,
-                   "block": args.block, "parallelism": "1 GPU"},
+                   "block": args.block, "fanout": c2["roofline"]["fanout"], "parallelism": "1 GPU"},
         "roofline": c2["roofline"],
         "cpu_baseline": c2.get("cpu_baseline"),
         "sample_parity": c2.get("sample_parity"),
         "probe_kernel_ms": {k: c2[k] for k in ("kernel_ms_avg", "kernel_ms_min", "kernel_ms_median")},
+        "fanout_sweep_ms": c2["fanout_sweep_ms"],
         "build": c2["build"],
         "result": c2["result"],
         "device": {"name": p.name.decode(), "arch": p.arch.decode(), "cus": p.cus},

@@ -201,6 +201,36 @@ int mgv_hashjoin_probe_materialize(const uint64_t *slots, uint32_t nbuckets, con
                                    int64_t row_base, int32_t *out_r, int32_t *out_p, uint64_t cap,
                                    mgv_result *res, mgv_launch launch, mgv_stream_t s);
 
+/* ------------------------------------------------------------ partitioned probe */
+
+/* Workspace bytes of a partitioned probe of n rows with the given fan-out
+ * (power of two in [2,512]) and partitioned columns (2 = key+val, 3 = with
+ * row ids for the digest). */
+int mgv_partition_workspace_size(uint64_t n, uint32_t fanout, uint32_t ncols, uint64_t *bytes);
+
+/* Same results as mgv_hashjoin_probe, computed by first scattering the
+ * (selected) probe rows into `fanout` partitions by the hash bits that pick
+ * the bucket, then probing partition by partition with each XCD draining its
+ * own partitions, so the table slice being probed stays in that XCD's L2.
+ * The partitioning step is the counterpart of split::consume_warp
+ * (src/operators/split.cu:54-118); the probe replaces
+ * src/operators/hashjoin.cu:135-182.  Needs n < 2^31.  The workspace comes
+ * from the caller; the call allocates nothing. */
+int mgv_hashjoin_probe_partitioned(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
+                                   const int32_t *keys, const int32_t *vals, uint64_t n, int64_t row_base,
+                                   uint32_t flags, uint32_t fanout, void *workspace, uint64_t workspace_bytes,
+                                   mgv_result *res, mgv_launch launch, mgv_stream_t s);
+
+/* Partitioned form of mgv_hashjoin_probe_groupby. */
+int mgv_hashjoin_probe_groupby_partitioned(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred,
+                                           int32_t bound, const int32_t *keys, const int32_t *vals, uint64_t n,
+                                           uint64_t *gcnt, int64_t *gsum, uint32_t ngroups, uint32_t fanout,
+                                           void *workspace, uint64_t workspace_bytes, mgv_launch launch,
+                                           mgv_stream_t s);
+
+/* Diagnostic: out8[x] += number of `grid` one-wave blocks that ran on XCD x. */
+int mgv_debug_xcc_histogram(uint32_t *out8, uint32_t grid, mgv_stream_t s);
+
 /* ------------------------------------------------------------ select / aggregate */
 
 /* SELECT COUNT(*), SUM(val) WHERE pred <= bound.  Replaces the

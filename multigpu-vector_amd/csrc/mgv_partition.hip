@@ -1,0 +1,474 @@
+// mgv_partition.hip -- radix-partitioned, XCD-aware probe.
+//
+// A table larger than the L2 makes every probe a 128-byte line fetch from
+// HBM.  This path first scatters the probe rows into F partitions by the top
+// bits of fmix32(key) -- the same bits that pick the bucket, so partition p
+// only touches the p-th 1/F slice of the bucket array -- and then probes
+// partition by partition, each XCD draining the partitions p = xcd (mod 8) so
+// that an XCD's active table slices stay in its own L2.  The optional select
+// (pred <= bound) is applied while partitioning, so only survivors move.
+//
+// Kernels (one stream, in order): k_part_prep (zero counters),
+// k_part_count (per-block histograms), k_part_scan (aligned partition
+// starts), k_part_scatter (LDS-staged scatter), k_part_probe (persistent
+// work-queue probe).  The reference's counterpart of the partitioning step is
+// split's ballot-compacted per-destination staging (src/operators/split.cu:
+// 54-118), there used for 2-GPU repartitioning with hash3(x) = x & 1.
+#include <hip/hip_runtime.h>
+
+#include "mgv_internal.hpp"
+#include "mgv_pipes.hpp"
+
+namespace mgv {
+
+constexpr int kPartBlock = 256;
+constexpr int kTile = kPartBlock * 8;      // rows staged in LDS per scatter step
+constexpr uint32_t kMaxFanout = 512;
+constexpr uint32_t kPartAlign = 64;        // partition starts aligned to 64 rows
+constexpr uint32_t kItemRows = 16384;      // probe work item
+constexpr uint32_t kMaxPartBlocks = 4096;
+
+// XCC (XCD) id of the executing wave: HW_REG_XCC_ID (hwreg 20), bits [3:0].
+// Used only to choose which partitions a block drains first.
+__device__ __forceinline__ uint32_t xcc_id() {
+    return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | 20) & 7u;
+}
+
+__device__ __forceinline__ uint32_t part_of(int32_t key, uint32_t F) {
+    return (uint32_t)(((uint64_t)fmix32((uint32_t)key) * F) >> 32);
+}
+
+struct PartLayout {
+    uint32_t F, nblk, ncols;
+    uint64_t rows_per_blk, n, cap;   // cap = rows per partitioned column
+    uint32_t *hist;                  // [nblk][F]
+    unsigned long long *ptotal;      // [F]
+    unsigned long long *pstart;      // [F + 1]
+    unsigned long long *pcursor;     // [F]
+    uint32_t *xhead;                 // [8]
+    int32_t *col[3];                 // partitioned k, v, rowid
+};
+
+__host__ __device__ static inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+static int plan_layout(uint64_t n, uint32_t F, uint32_t ncols, void *ws, uint64_t ws_bytes, PartLayout *L,
+                       uint64_t *need) {
+    if (F < 2 || F > kMaxFanout || (F & (F - 1))) return fail(MGV_E_INVALID, "fan-out %u not a power of two in [2,512]", F);
+    if (ncols < 1 || ncols > 3) return fail(MGV_E_INVALID, "partitioned columns %u not in [1,3]", ncols);
+    uint64_t nblk = (n + 16383) / 16384;
+    if (nblk < 1) nblk = 1;
+    if (nblk > kMaxPartBlocks) nblk = kMaxPartBlocks;
+    uint64_t rpb = align_up((n + nblk - 1) / nblk, kTile);
+    nblk = (n + rpb - 1) / rpb;
+    if (nblk < 1) nblk = 1;
+    const uint64_t cap = align_up(n + (uint64_t)F * kPartAlign, 64);
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) {
+        uint64_t o = off;
+        off = align_up(off + bytes, 256);
+        return o;
+    };
+    const uint64_t o_hist = take(nblk * F * 4);
+    const uint64_t o_ptot = take(F * 8);
+    const uint64_t o_pst = take((F + 1) * 8);
+    const uint64_t o_pcur = take(F * 8);
+    const uint64_t o_xh = take(64);
+    uint64_t o_col[3] = {0, 0, 0};
+    for (uint32_t c = 0; c < ncols; ++c) o_col[c] = take(cap * 4);
+    if (need) *need = off;
+    if (!L) return MGV_OK;
+    if (!ws || ws_bytes < off)
+        return fail(MGV_E_CAPACITY, "partition workspace %llu bytes < %llu needed", (unsigned long long)ws_bytes,
+                    (unsigned long long)off);
+    char *b = (char *)ws;
+    L->F = F;
+    L->nblk = (uint32_t)nblk;
+    L->ncols = ncols;
+    L->rows_per_blk = rpb;
+    L->n = n;
+    L->cap = cap;
+    L->hist = (uint32_t *)(b + o_hist);
+    L->ptotal = (unsigned long long *)(b + o_ptot);
+    L->pstart = (unsigned long long *)(b + o_pst);
+    L->pcursor = (unsigned long long *)(b + o_pcur);
+    L->xhead = (uint32_t *)(b + o_xh);
+    for (int c = 0; c < 3; ++c) L->col[c] = c < (int)ncols ? (int32_t *)(b + o_col[c]) : nullptr;
+    return MGV_OK;
+}
+
+// Input of the partitioning kernels.
+struct PartIn {
+    const int32_t *pred;
+    int32_t bound;
+    const int32_t *key;
+    const int32_t *val;   // may be null (partitioned as zeros)
+    int64_t row_base;
+    uint32_t want_rowid;  // third partitioned column = row ids
+};
+
+__global__ void k_part_prep(PartLayout L) {
+    for (uint32_t p = threadIdx.x; p < L.F; p += blockDim.x) L.ptotal[p] = 0;
+    if (threadIdx.x < 8) L.xhead[threadIdx.x] = 0;
+}
+
+// Loads the 4 rows [i0, i0+4) of a column (vector load when all are in range).
+__device__ __forceinline__ v4i load_quad(const int32_t *p, uint64_t i0, uint64_t hi) {
+    if (i0 + 3 < hi) return load_stream4(p + i0);
+    v4i q = {0, 0, 0, 0};
+    if (i0 < hi) q.x = load_stream1(p + i0);
+    if (i0 + 1 < hi) q.y = load_stream1(p + i0 + 1);
+    if (i0 + 2 < hi) q.z = load_stream1(p + i0 + 2);
+    return q;
+}
+
+__device__ __forceinline__ uint32_t quad_mask(const PartIn &in, uint64_t i0, uint64_t hi, v4i *predq) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) m |= (uint32_t)(i0 + j < hi) << j;
+    if (in.pred) {
+        v4i c = load_quad(in.pred, i0, hi);
+        *predq = c;
+        uint32_t s = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s |= (uint32_t)(q_get(c, j) <= in.bound) << j;
+        m &= s;
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(kPartBlock) void k_part_count(PartLayout L, PartIn in) {
+    __shared__ uint32_t h[kMaxFanout];
+    for (uint32_t p = threadIdx.x; p < L.F; p += kPartBlock) h[p] = 0;
+    __syncthreads();
+    const uint64_t lo = (uint64_t)blockIdx.x * L.rows_per_blk;
+    const uint64_t hi = min(L.n, lo + L.rows_per_blk);
+    for (uint64_t i0 = lo + 4ull * threadIdx.x; i0 < hi; i0 += 4ull * kPartBlock) {
+        v4i c;
+        uint32_t m = quad_mask(in, i0, hi, &c);
+        if (m) {
+            v4i k = load_quad(in.key, i0, hi);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if ((m >> j) & 1u) atomicAdd(&h[part_of(q_get(k, j), L.F)], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < L.F; p += kPartBlock) {
+        uint32_t c = h[p];
+        L.hist[(uint64_t)blockIdx.x * L.F + p] = c;
+        if (c) atomicAdd(&L.ptotal[p], (unsigned long long)c);
+    }
+}
+
+// One block: aligned exclusive scan of the partition totals.
+__global__ __launch_bounds__(kMaxFanout) void k_part_scan(PartLayout L) {
+    __shared__ unsigned long long s[kMaxFanout];
+    const uint32_t t = threadIdx.x;
+    unsigned long long v = (t < L.F) ? align_up(L.ptotal[t], kPartAlign) : 0;
+    s[t] = v;
+    __syncthreads();
+    for (uint32_t d = 1; d < kMaxFanout; d <<= 1) {
+        unsigned long long a = (t >= d) ? s[t - d] : 0;
+        __syncthreads();
+        s[t] += a;
+        __syncthreads();
+    }
+    if (t < L.F) {
+        unsigned long long start = s[t] - v;   // exclusive
+        L.pstart[t] = start;
+        L.pcursor[t] = start;
+    }
+    if (t == 0) L.pstart[L.F] = s[L.F - 1];
+}
+
+__device__ __forceinline__ uint32_t scan_block_excl(uint32_t v, uint32_t *tmp) {
+    // exclusive scan over kPartBlock threads through LDS (Hillis-Steele)
+    const uint32_t t = threadIdx.x;
+    tmp[t] = v;
+    __syncthreads();
+    for (uint32_t d = 1; d < kPartBlock; d <<= 1) {
+        uint32_t a = (t >= d) ? tmp[t - d] : 0;
+        __syncthreads();
+        tmp[t] += a;
+        __syncthreads();
+    }
+    uint32_t incl = tmp[t];
+    __syncthreads();
+    return incl - v;
+}
+
+// LDS-staged scatter: each tile of kTile rows is counting-sorted by partition
+// in LDS, then written out so that consecutive lanes write consecutive slots
+// of a partition's run.
+__global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartIn in) {
+    __shared__ unsigned long long cursor[kMaxFanout];
+    __shared__ uint32_t tcnt[kMaxFanout];
+    __shared__ uint32_t toff[kMaxFanout];
+    __shared__ uint32_t tmp[kPartBlock];
+    __shared__ int32_t sk[kTile];
+    __shared__ int32_t sv[kTile];
+    __shared__ int32_t sr[kTile];
+    __shared__ uint16_t sp[kTile];
+    const uint32_t F = L.F;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t p = t; p < F; p += kPartBlock) {
+        uint32_t c = L.hist[(uint64_t)blockIdx.x * F + p];
+        cursor[p] = c ? atomicAdd(&L.pcursor[p], (unsigned long long)c) : 0;
+        tcnt[p] = 0;
+    }
+    __syncthreads();
+    const uint64_t lo = (uint64_t)blockIdx.x * L.rows_per_blk;
+    const uint64_t hi = min(L.n, lo + L.rows_per_blk);
+    for (uint64_t tb = lo; tb < hi; tb += kTile) {
+        // 1. load 8 rows per thread (2 quads), rank each row within its partition
+        uint32_t rank[8], part[8];
+        int32_t kk[8], vv[8];
+        uint32_t mm = 0;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const uint64_t i0 = tb + 4ull * t + (uint64_t)q * 4 * kPartBlock;
+            v4i c;
+            uint32_t m = (i0 < hi) ? quad_mask(in, i0, hi, &c) : 0u;
+            v4i k = {0, 0, 0, 0}, v = {0, 0, 0, 0};
+            if (m) {
+                k = load_quad(in.key, i0, hi);
+                if (in.val) v = load_quad(in.val, i0, hi);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int r = q * 4 + j;
+                kk[r] = q_get(k, j);
+                vv[r] = q_get(v, j);
+                if ((m >> j) & 1u) {
+                    part[r] = part_of(kk[r], F);
+                    rank[r] = atomicAdd(&tcnt[part[r]], 1u);
+                    mm |= 1u << r;
+                }
+            }
+        }
+        __syncthreads();
+        // 2. exclusive scan of the tile counts
+        uint32_t mine = 0;
+        const uint32_t per = (F + kPartBlock - 1) / kPartBlock;   // partitions per thread (<= 2)
+        for (uint32_t e = 0; e < per; ++e) {
+            uint32_t p = t * per + e;
+            mine += (p < F) ? tcnt[p] : 0;
+        }
+        uint32_t base = scan_block_excl(mine, tmp);
+        for (uint32_t e = 0; e < per; ++e) {
+            uint32_t p = t * per + e;
+            if (p < F) {
+                toff[p] = base;
+                base += tcnt[p];
+            }
+        }
+        __syncthreads();
+        const uint32_t total = toff[F - 1] + tcnt[F - 1];
+        // 3. stage rows in partition order
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            if ((mm >> r) & 1u) {
+                const uint32_t pos = toff[part[r]] + rank[r];
+                sk[pos] = kk[r];
+                sv[pos] = vv[r];
+                sp[pos] = (uint16_t)part[r];
+                if (in.want_rowid) {
+                    const int q = r / 4, j = r % 4;
+                    sr[pos] = (int32_t)(in.row_base + (int64_t)(tb + 4ull * t + (uint64_t)q * 4 * kPartBlock + j));
+                }
+            }
+        }
+        __syncthreads();
+        // 4. write runs
+        for (uint32_t i = t; i < total; i += kPartBlock) {
+            const uint32_t p = sp[i];
+            const unsigned long long dst = cursor[p] + (i - toff[p]);
+            L.col[0][dst] = sk[i];
+            if (L.ncols > 1) L.col[1][dst] = sv[i];
+            if (L.ncols > 2) L.col[2][dst] = sr[i];
+        }
+        __syncthreads();
+        for (uint32_t p = t; p < F; p += kPartBlock) {
+            cursor[p] += tcnt[p];
+            tcnt[p] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// Work item i of XCD x -> (partition, first row, last row).  Partitions of XCD
+// x are p = x, x + 8, ... (fewer than 8 partitions: all on XCD 0..F-1).
+struct ItemMap {
+    __device__ static uint32_t nparts_of(uint32_t F, uint32_t x) {
+        return F > x ? (F - x + 7) / 8 : 0;
+    }
+};
+
+template <class Pipe, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_part_probe(typename Pipe::Params prm, PartLayout L) {
+    constexpr int WAVES = BLOCK / kWave;
+    typedef typename Pipe::template Op<WAVES> Op;
+    __shared__ __align__(16) typename Op::Smem sm;
+    __shared__ unsigned long long item_lo, item_hi;
+    __shared__ uint32_t item_ok;
+    Op op;
+    Pipe::setup(op, prm, &sm);
+    op.consume_open();
+    const uint32_t home = xcc_id();
+    Cols cols = {{L.col[0], L.ncols > 1 ? L.col[1] : nullptr, L.ncols > 2 ? L.col[2] : nullptr, nullptr}, -1};
+    for (uint32_t k = 0; k < 8; ++k) {
+        const uint32_t x = (home + k) & 7u;
+        for (;;) {
+            if (threadIdx.x == 0) {
+                item_ok = 0;
+                const uint32_t i = atomicAdd(&L.xhead[x], 1u);
+                // find the partition of item i among p = x, x+8, ...
+                uint32_t acc = 0;
+                for (uint32_t p = x; p < L.F; p += 8) {
+                    const unsigned long long s0 = L.pstart[p];
+                    const unsigned long long sz = L.ptotal[p];
+                    const uint32_t ni = (uint32_t)((sz + kItemRows - 1) / kItemRows);
+                    if (i < acc + ni) {
+                        const unsigned long long a = s0 + (unsigned long long)(i - acc) * kItemRows;
+                        const unsigned long long e = s0 + sz;
+                        item_lo = a;
+                        item_hi = (a + kItemRows < e) ? a + kItemRows : e;
+                        item_ok = 1;
+                        break;
+                    }
+                    acc += ni;
+                }
+            }
+            __syncthreads();
+            const uint32_t ok = item_ok;
+            const uint64_t lo = item_lo, hi = item_hi;
+            __syncthreads();
+            if (!ok) break;
+            for (uint64_t base = lo + (uint64_t)wave_id() * kChunk; base < hi; base += 2ull * WAVES * kChunk) {
+                Rows r0, r1;
+                load_rows(r0, cols, base, hi, 0);
+                const uint64_t b1 = base + (uint64_t)WAVES * kChunk;
+                if (b1 < hi) load_rows(r1, cols, b1, hi, 0);
+                op.consume(r0);
+                if (b1 < hi) op.consume(r1);
+            }
+        }
+    }
+    op.consume_close();
+}
+
+}  // namespace mgv
+
+// ------------------------------------------------------------------ host side
+
+namespace mgv {
+
+template <class Pipe, int B>
+static int part_probe_b(const typename Pipe::Params &prm, const PartLayout &L, hipStream_t s) {
+    auto kern = k_part_probe<Pipe, B>;
+    static thread_local int cached = 0;
+    if (!cached) cached = persistent_blocks(kern, B);
+    hipLaunchKernelGGL(kern, dim3(cached), dim3(B), 0, s, prm, L);
+    return check_launch("k_part_probe");
+}
+
+template <class Pipe>
+static int part_probe(const typename Pipe::Params &prm, const PartLayout &L, const mgv_launch &LC, hipStream_t s) {
+    switch (LC.block) {
+        case 64: return part_probe_b<Pipe, 64>(prm, L, s);
+        case 128: return part_probe_b<Pipe, 128>(prm, L, s);
+        case 0:
+        case 256: return part_probe_b<Pipe, 256>(prm, L, s);
+        case 512: return part_probe_b<Pipe, 512>(prm, L, s);
+        case 1024: return part_probe_b<Pipe, 1024>(prm, L, s);
+        default: return fail(MGV_E_INVALID, "block %u not in {64,128,256,512,1024}", LC.block);
+    }
+}
+
+static int partition_rows(const PartLayout &L, const PartIn &in, hipStream_t s) {
+    for (const int32_t *p : {in.pred, in.key, in.val})
+        if (p && !aligned16(p)) return fail(MGV_E_ALIGN, "partitioned column is not 16-byte aligned");
+    hipLaunchKernelGGL(k_part_prep, dim3(1), dim3(kMaxFanout), 0, s, L);
+    int rc = check_launch("k_part_prep");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_part_count, dim3(L.nblk), dim3(kPartBlock), 0, s, L, in);
+    if ((rc = check_launch("k_part_count"))) return rc;
+    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kMaxFanout), 0, s, L);
+    if ((rc = check_launch("k_part_scan"))) return rc;
+    hipLaunchKernelGGL(k_part_scatter, dim3(L.nblk), dim3(kPartBlock), 0, s, L, in);
+    return check_launch("k_part_scatter");
+}
+
+__global__ void k_xcc_histogram(uint32_t *out) {
+    if (threadIdx.x == 0) atomicAdd(&out[xcc_id()], 1u);
+}
+
+}  // namespace mgv
+
+using namespace mgv;
+
+extern "C" {
+
+int mgv_partition_workspace_size(uint64_t n, uint32_t fanout, uint32_t ncols, uint64_t *bytes) {
+    if (!bytes) return fail(MGV_E_INVALID, "null bytes");
+    return plan_layout(n, fanout, ncols, nullptr, 0, nullptr, bytes);
+}
+
+int mgv_hashjoin_probe_partitioned(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
+                                   const int32_t *keys, const int32_t *vals, uint64_t n, int64_t row_base,
+                                   uint32_t flags, uint32_t fanout, void *ws, uint64_t ws_bytes, mgv_result *res,
+                                   mgv_launch LC, mgv_stream_t st) {
+    if (!current_state()) return fail(MGV_E_NOT_INIT, "mgv_init was not called");
+    if (!slots || !keys || !res || nbuckets == 0) return fail(MGV_E_INVALID, "probe_partitioned: null argument");
+    if (n == 0) return MGV_OK;
+    if (n >= (1ull << 31)) return fail(MGV_E_INVALID, "probe_partitioned: n must be < 2^31");
+    const bool digest = (flags & MGV_PROBE_DIGEST) != 0;
+    PartLayout L;
+    int rc = plan_layout(n, fanout, digest ? 3 : 2, ws, ws_bytes, &L, nullptr);
+    if (rc) return rc;
+    hipStream_t s = stream_of(st);
+    PartIn in = {pred, bound, keys, vals, row_base, digest ? 1u : 0u};
+    if ((rc = partition_rows(L, in, s))) return rc;
+    if (digest) {
+        PProbeSum<false, true, true>::Params p = {0, slots, nbuckets, res};
+        return part_probe<PProbeSum<false, true, true>>(p, L, LC, s);
+    }
+    PProbeSum<false, true, false>::Params p = {0, slots, nbuckets, res};
+    return part_probe<PProbeSum<false, true, false>>(p, L, LC, s);
+}
+
+int mgv_hashjoin_probe_groupby_partitioned(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred,
+                                           int32_t bound, const int32_t *keys, const int32_t *vals, uint64_t n,
+                                           uint64_t *gcnt, int64_t *gsum, uint32_t ngroups, uint32_t fanout,
+                                           void *ws, uint64_t ws_bytes, mgv_launch LC, mgv_stream_t st) {
+    if (!current_state()) return fail(MGV_E_NOT_INIT, "mgv_init was not called");
+    if (!slots || !keys || !vals || !gcnt || !gsum || nbuckets == 0 || ngroups == 0)
+        return fail(MGV_E_INVALID, "probe_groupby_partitioned: bad argument");
+    if (n == 0) return MGV_OK;
+    if (n >= (1ull << 31)) return fail(MGV_E_INVALID, "probe_groupby_partitioned: n must be < 2^31");
+    PartLayout L;
+    int rc = plan_layout(n, fanout, 2, ws, ws_bytes, &L, nullptr);
+    if (rc) return rc;
+    hipStream_t s = stream_of(st);
+    PartIn in = {pred, bound, keys, vals, 0, 0u};
+    if ((rc = partition_rows(L, in, s))) return rc;
+    if (ngroups <= 1024) {
+        PProbeGroup<false, true, 1024>::Params p = {0, slots, nbuckets, gcnt, gsum, ngroups};
+        return part_probe<PProbeGroup<false, true, 1024>>(p, L, LC, s);
+    }
+    if (ngroups <= 4096) {
+        PProbeGroup<false, true, 4096>::Params p = {0, slots, nbuckets, gcnt, gsum, ngroups};
+        return part_probe<PProbeGroup<false, true, 4096>>(p, L, LC, s);
+    }
+    PProbeGroup<false, true, 0>::Params p = {0, slots, nbuckets, gcnt, gsum, ngroups};
+    return part_probe<PProbeGroup<false, true, 0>>(p, L, LC, s);
+}
+
+int mgv_debug_xcc_histogram(uint32_t *out8, uint32_t grid, mgv_stream_t st) {
+    if (!current_state()) return fail(MGV_E_NOT_INIT, "mgv_init was not called");
+    if (!out8 || grid == 0) return fail(MGV_E_INVALID, "bad argument");
+    hipLaunchKernelGGL(k_xcc_histogram, dim3(grid), dim3(64), 0, stream_of(st), out8);
+    return check_launch("k_xcc_histogram");
+}
+
+}  // extern "C"

@@ -10,273 +10,9 @@
 #include <hip/hip_runtime.h>
 
 #include "mgv_internal.hpp"
-#include "mgv_ops.hpp"
+#include "mgv_pipes.hpp"
 
 namespace mgv {
-
-// Input columns of a pipeline: up to 4 int32 columns; column `rowid_col`
-// (if >= 0) is synthesised as row_base + row index instead of being loaded.
-struct Cols {
-    const int32_t *p[4];
-    int32_t rowid_col;
-};
-
-__device__ __forceinline__ void load_rows(Rows &r, const Cols &cols, uint64_t base, uint64_t n, int64_t row_base) {
-    const uint64_t i0 = base + 4ull * lane_id();
-    if (i0 + 3 < n) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            if (cols.p[c]) {
-                r.c[c] = load_stream4(cols.p[c] + i0);
-            } else if (c == cols.rowid_col) {
-                int32_t b = (int32_t)(row_base + (int64_t)i0);
-                r.c[c] = v4i{b, b + 1, b + 2, b + 3};
-            } else {
-                r.c[c] = v4i{0, 0, 0, 0};
-            }
-        }
-        r.m = 0xFu;
-    } else {
-        int32_t v[4][4];
-        uint32_t m = 0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint64_t i = i0 + j;
-            const bool ok = i < n;
-            m |= (uint32_t)ok << j;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                int32_t x = 0;
-                if (ok) {
-                    if (cols.p[c]) x = load_stream1(cols.p[c] + i);
-                    else if (c == cols.rowid_col) x = (int32_t)(row_base + (int64_t)i);
-                }
-                v[c][j] = x;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) r.c[c] = v4i{v[c][0], v[c][1], v[c][2], v[c][3]};
-        r.m = m;
-    }
-}
-
-// Two wave chunks are loaded before either is consumed so that each lane keeps
-// 2 x 16 B per column in flight.
-template <class Pipe, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_pipeline(typename Pipe::Params prm, Cols cols, uint64_t n,
-                                                    int64_t row_base) {
-    constexpr int WAVES = BLOCK / kWave;
-    typedef typename Pipe::template Op<WAVES> Op;
-    __shared__ __align__(16) typename Op::Smem sm;
-    Op op;
-    Pipe::setup(op, prm, &sm);
-    op.consume_open();
-    const uint64_t w = (uint64_t)blockIdx.x * WAVES + wave_id();
-    const uint64_t stride = (uint64_t)gridDim.x * WAVES * kChunk;
-    for (uint64_t base = w * kChunk; base < n; base += 2 * stride) {
-        Rows r0, r1;
-        load_rows(r0, cols, base, n, row_base);
-        const uint64_t b1 = base + stride;
-        if (b1 < n) {
-            load_rows(r1, cols, b1, n, row_base);
-        } else {
-            r1.m = 0;
-        }
-        op.consume(r0);
-        if (b1 < n) op.consume(r1);
-    }
-    op.consume_close();
-}
-
-// ------------------------------------------------------------------ pipelines
-
-// SELECT COUNT(*), SUM(c1) WHERE c0 <= bound.
-template <bool COMPACT>
-struct PSelectSum {
-    struct Params {
-        int32_t bound;
-        mgv_result *res;
-    };
-    template <int W>
-    using Op = OpSelect<OpAggregate<1, false, W>, COMPACT, 1, W>;
-    template <class O>
-    __device__ static void setup(O &op, const Params &p, typename O::Smem *sm) {
-        op.sm = sm;
-        op.bound = p.bound;
-        op.parent.sm = &sm->parent;
-        op.parent.res = p.res;
-    }
-};
-
-// SELECT c1 WHERE c0 <= bound, materialised.
-struct PSelectMat {
-    struct Params {
-        int32_t bound;
-        int32_t *out;
-        uint64_t cap;
-        mgv_result *res;
-    };
-    template <int W>
-    using Op = OpSelect<OpMaterialize<0, 0, false>, true, 1, W>;
-    template <class O>
-    __device__ static void setup(O &op, const Params &p, typename O::Smem *sm) {
-        op.sm = sm;
-        op.bound = p.bound;
-        op.parent.sm = &sm->parent;
-        op.parent.out_a = p.out;
-        op.parent.out_b = nullptr;
-        op.parent.cap = p.cap;
-        op.parent.res = p.res;
-    }
-};
-
-// Build: (c0 key, c1 payload) or, with SEL, (c0 pred, c1 key, c2 payload).
-template <bool SEL>
-struct PBuild {
-    struct Params {
-        int32_t bound;
-        uint64_t *slots;
-        uint32_t nbuckets;
-        mgv_result *res;
-    };
-    template <int W>
-    using Op = typename std::conditional<SEL, OpSelect<OpBuild<W>, true, 2, W>, OpBuild<W>>::type;
-    template <class O>
-    __device__ static void setup(O &op, const Params &p, typename O::Smem *sm) {
-        if constexpr (SEL) {
-            op.sm = sm;
-            op.bound = p.bound;
-            op.parent.sm = &sm->parent;
-            op.parent.slots = p.slots;
-            op.parent.nbuckets = p.nbuckets;
-            op.parent.res = p.res;
-        } else {
-            op.sm = sm;
-            op.slots = p.slots;
-            op.nbuckets = p.nbuckets;
-            op.res = p.res;
-        }
-    }
-};
-
-// Probe + COUNT/SUM(val + payload) [+ digest]: (c0 key, c1 val, c2 rowid) or,
-// with SEL, (c0 pred, c1 key, c2 val, c3 rowid).
-template <bool SEL, bool COMPACT, bool DIGEST>
-struct PProbeSum {
-    struct Params {
-        int32_t bound;
-        const uint64_t *slots;
-        uint32_t nbuckets;
-        mgv_result *res;
-    };
-    template <int W>
-    using Agg = OpAggregate<2, DIGEST, W>;
-    template <int W>
-    using Op = typename std::conditional<SEL, OpSelect<OpProbe<Agg<W>>, COMPACT, DIGEST ? 3 : 2, W>,
-                                         OpProbe<Agg<W>>>::type;
-    template <class O>
-    __device__ static void setup(O &op, const Params &p, typename O::Smem *sm) {
-        if constexpr (SEL) {
-            op.sm = sm;
-            op.bound = p.bound;
-            op.parent.slots = p.slots;
-            op.parent.nbuckets = p.nbuckets;
-            op.parent.parent.sm = &sm->parent;
-            op.parent.parent.res = p.res;
-        } else {
-            op.slots = p.slots;
-            op.nbuckets = p.nbuckets;
-            op.parent.sm = sm;
-            op.parent.res = p.res;
-        }
-    }
-};
-
-// Probe + GROUP BY payload: (c0 key, c1 val) or (c0 pred, c1 key, c2 val).
-template <bool SEL, bool COMPACT, int LDSG>
-struct PProbeGroup {
-    struct Params {
-        int32_t bound;
-        const uint64_t *slots;
-        uint32_t nbuckets;
-        uint64_t *gcnt;
-        int64_t *gsum;
-        uint32_t ngroups;
-    };
-    template <int W>
-    using G = OpGroupBy<LDSG>;
-    template <int W>
-    using Op = typename std::conditional<SEL, OpSelect<OpProbe<G<W>>, COMPACT, 2, W>, OpProbe<G<W>>>::type;
-    template <class O>
-    __device__ static void setup(O &op, const Params &p, typename O::Smem *sm) {
-        if constexpr (SEL) {
-            op.sm = sm;
-            op.bound = p.bound;
-            op.parent.slots = p.slots;
-            op.parent.nbuckets = p.nbuckets;
-            auto &g = op.parent.parent;
-            g.sm = &sm->parent;
-            g.gcnt = p.gcnt;
-            g.gsum = p.gsum;
-            g.ngroups = p.ngroups;
-            g.res = nullptr;
-        } else {
-            op.slots = p.slots;
-            op.nbuckets = p.nbuckets;
-            auto &g = op.parent;
-            g.sm = sm;
-            g.gcnt = p.gcnt;
-            g.gsum = p.gsum;
-            g.ngroups = p.ngroups;
-            g.res = nullptr;
-        }
-    }
-};
-
-// Probe + materialise (r_row, payload): (c0 key, c1 rowid).
-struct PProbeMat {
-    struct Params {
-        const uint64_t *slots;
-        uint32_t nbuckets;
-        int32_t *out_r;
-        int32_t *out_p;
-        uint64_t cap;
-        mgv_result *res;
-    };
-    template <int W>
-    using Op = OpProbe<OpMaterialize<1, 0, true>>;
-    template <class O>
-    __device__ static void setup(O &op, const Params &p, typename O::Smem *sm) {
-        op.slots = p.slots;
-        op.nbuckets = p.nbuckets;
-        op.parent.sm = sm;
-        op.parent.out_a = p.out_r;
-        op.parent.out_b = p.out_p;
-        op.parent.cap = p.cap;
-        op.parent.res = p.res;
-    }
-};
-
-// GROUP BY c0: COUNT, SUM(c1).
-template <int LDSG>
-struct PGroupBy {
-    struct Params {
-        uint64_t *gcnt;
-        int64_t *gsum;
-        uint32_t ngroups;
-        mgv_result *res;
-    };
-    template <int W>
-    using Op = OpGroupBy<LDSG>;
-    template <class O>
-    __device__ static void setup(O &op, const Params &p, typename O::Smem *sm) {
-        op.sm = sm;
-        op.gcnt = p.gcnt;
-        op.gsum = p.gsum;
-        op.ngroups = p.ngroups;
-        op.res = p.res;
-    }
-};
 
 // ------------------------------------------------------------------ launch
 
@@ -342,6 +78,9 @@ int mgv_ht_plan(uint64_t nkeys, double load_factor, uint32_t *nbuckets, uint64_t
     uint64_t nb = (uint64_t)((slots + MGV_SLOTS_PER_BUCKET - 1) / MGV_SLOTS_PER_BUCKET);
     if (nb < 1) nb = 1;
     while (nb * MGV_SLOTS_PER_BUCKET <= nkeys) ++nb;   // keep at least one empty slot
+    // Large tables get a multiple of 256 buckets so that every partition of a
+    // partitioned probe (fan-out <= 256) maps to a whole range of buckets.
+    if (nb >= 4096) nb = (nb + 255) / 256 * 256;
     if (nb > 0xffffffffull) return fail(MGV_E_INVALID, "table too large");
     if (nbuckets) *nbuckets = (uint32_t)nb;
     if (bytes) *bytes = nb * MGV_SLOTS_PER_BUCKET * sizeof(uint64_t);

@@ -123,6 +123,11 @@ SIGNATURES = {
     "mgv_select_sum": [VP, VP, U64, I32, VP, Launch, VP],
     "mgv_select_materialize": [VP, VP, U64, I32, VP, U64, VP, Launch, VP],
     "mgv_groupby": [VP, VP, U64, VP, VP, U32, VP, Launch, VP],
+    "mgv_partition_workspace_size": [U64, U32, U32, C.POINTER(U64)],
+    "mgv_hashjoin_probe_partitioned": [VP, U32, VP, I32, VP, VP, U64, I64, U32, U32, VP, U64, VP, Launch, VP],
+    "mgv_hashjoin_probe_groupby_partitioned": [VP, U32, VP, I32, VP, VP, U64, VP, VP, U32, U32, VP, U64, Launch,
+                                               VP],
+    "mgv_debug_xcc_histogram": [VP, U32, VP],
 }
 _RESTYPE = {"mgv_last_error_string": C.c_char_p}
 
@@ -269,6 +274,12 @@ class HashTable:
 
     def slots(self) -> np.ndarray:
         return self.buf.download(np.uint64, self.nbuckets * SLOTS_PER_BUCKET)
+
+
+def partition_workspace(n: int, fanout: int, ncols: int = 2) -> "DeviceBuffer":
+    b = U64()
+    call("mgv_partition_workspace_size", n, fanout, ncols, C.byref(b))
+    return DeviceBuffer(b.value)
 
 
 class Stream:

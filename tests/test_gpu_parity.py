@@ -351,3 +351,119 @@ def test_join_mid_size_exact(gpu, orc):
     want = orc.probe(t, hrk, hrv, threads=8)
     orc.free(t)
     assert (r.count, r.sum) == (want["count"], want["sum"])
+
+
+# ----------------------------------------------------------------- partitioned probe
+
+def probe_part(gpu, ht, keys, vals, n, fanout, pred=None, bound=0, flags=0, block=256):
+    ncols = 3 if flags & mgv.MGV_PROBE_DIGEST else 2
+    ws = mgv.partition_workspace(n, fanout, ncols)
+    res = mgv.ResultBuffer()
+    gpu.call("mgv_hashjoin_probe_partitioned", ht.buf.vp, ht.nbuckets, pred.vp if pred is not None else None, bound,
+             keys.vp, vals.vp if vals is not None else None, n, 0, flags, fanout, ws.vp, ws.nbytes, res.vp,
+             mgv.launch(0, block), None)
+    gpu_sync()
+    return res.read()
+
+
+@pytest.mark.parametrize("fanout", [2, 8, 64, 256, 512])
+def test_partitioned_probe_c2_small(gpu, orc, golden, fanout):
+    g = golden["c2_small"]
+    sk, sw, rk, rv = c2_small_cols(gpu, orc, g)
+    ht, _ = build(gpu, sk, sw, g["ns"])
+    r = probe_part(gpu, ht, rk, rv, g["nr"], fanout)
+    assert (r.count, r.sum) == (g["count"], g["sum"])
+    ht2, _ = build(gpu, sk, None, g["ns"], 0.5, mgv.MGV_PAYLOAD_ROWID)
+    d = probe_part(gpu, ht2, rk, None, g["nr"], fanout, flags=mgv.MGV_PROBE_DIGEST)
+    assert (d.count, d.digest_sum, d.digest_xor) == (g["count"], g["digest_sum"], g["digest_xor"])
+
+
+@pytest.mark.parametrize("fanout", [16, 128])
+def test_partitioned_select_probe_matches_oracle(gpu, orc, fanout):
+    ns, nr = 30000, 500009
+    sk = orc.gen_perm(ns, ns, orc.seed(4, 3), 1)
+    sw = orc.gen_bits(ns, orc.seed(4, 4), 1, 16)
+    rk = orc.gen_range(nr, orc.seed(4, 1), 1, 2 * ns)
+    rv = orc.gen_bits(nr, orc.seed(4, 2), 1, 16)
+    c = orc.gen_bits(nr, orc.seed(4, 0), 1, 31)
+    ht, _ = build(gpu, dev(sk), dev(sw), ns)
+    t = orc.table(sk, sw)
+    dk, dv, dc = dev(rk), dev(rv), dev(c)
+    for bound in (INT_MIN, 2**31 // 100, 2**30, INT_MAX):
+        want = orc.probe(t, rk, rv, pred=c, bound=bound)
+        r = probe_part(gpu, ht, dk, dv, nr, fanout, pred=dc, bound=bound)
+        assert (r.count, r.sum) == (want["count"], want["sum"]), bound
+    orc.free(t)
+
+
+def test_partitioned_probe_duplicates_and_skew(gpu, orc, golden):
+    g = golden["dup_join"]
+    sk = dev(orc.gen_range(g["ns"], orc.seed(2, 10), 0, 64))
+    sw = dev(orc.gen_bits(g["ns"], orc.seed(2, 11), 0, 16))
+    rk = dev(orc.gen_range(g["nr"], orc.seed(2, 12), 0, 80))
+    rv = dev(orc.gen_bits(g["nr"], orc.seed(2, 13), 0, 16))
+    ht, _ = build(gpu, sk, sw, g["ns"])
+    for fanout in (4, 32):
+        r = probe_part(gpu, ht, rk, rv, g["nr"], fanout)
+        assert (r.count, r.sum) == (g["count"], g["sum"])
+
+
+def test_partitioned_groupby_c3_small(gpu, orc, golden):
+    g = golden["c3_small"]
+    T = orc.zipf_cdf(g["nranks"], 1.0)
+    c = dev(orc.gen_bits(g["nr"], orc.seed(3, 0), 0, 31))
+    rk = dev(orc.gen_zipf(g["nr"], orc.seed(3, 1), 0, T, orc.seed(3, 5), 0))
+    rv = dev(orc.gen_bits(g["nr"], orc.seed(3, 2), 0, 16))
+    sk = dev(orc.gen_perm(g["nranks"], g["nranks"], orc.seed(3, 3), 0))
+    sg = dev(orc.gen_bits(g["nranks"], orc.seed(3, 4), 0, 6))
+    ht, _ = build(gpu, sk, sg, g["nranks"])
+    G = g["ngroups"]
+    ws = mgv.partition_workspace(g["nr"], 16, 2)
+    for case in g["cases"]:
+        gc, gs = mgv.DeviceBuffer(8 * G), mgv.DeviceBuffer(8 * G)
+        gpu.call("mgv_fill_u64", gc.vp, G, 0, None)
+        gpu.call("mgv_fill_u64", gs.vp, G, 0, None)
+        gpu.call("mgv_hashjoin_probe_groupby_partitioned", ht.buf.vp, ht.nbuckets, c.vp, case["bound"], rk.vp, rv.vp,
+                 g["nr"], gc.vp, gs.vp, G, 16, ws.vp, ws.nbytes, mgv.launch(0, 256), None)
+        gpu_sync()
+        assert gc.download(np.uint64, G).tolist() == case["gcnt"]
+        assert gs.download(np.int64, G).tolist() == case["gsum"]
+
+
+def test_partitioned_workspace_too_small_is_an_error(gpu):
+    ht = mgv.HashTable(100)
+    k = dev(np.arange(1000, dtype=np.int32))
+    res = mgv.ResultBuffer()
+    small = mgv.DeviceBuffer(1024)
+    with pytest.raises(mgv.MgvError) as e:
+        gpu.call("mgv_hashjoin_probe_partitioned", ht.buf.vp, ht.nbuckets, None, 0, k.vp, None, 1000, 0, 0, 16,
+                 small.vp, small.nbytes, res.vp, mgv.launch(0, 256), None)
+    assert e.value.code == mgv.E_CAPACITY
+    with pytest.raises(mgv.MgvError) as e:
+        gpu.call("mgv_hashjoin_probe_partitioned", ht.buf.vp, ht.nbuckets, None, 0, k.vp, None, 1000, 0, 0, 12,
+                 small.vp, small.nbytes, res.vp, mgv.launch(0, 256), None)
+    assert e.value.code == mgv.E_INVALID
+
+
+def test_xcc_ids_cover_eight_xcds(gpu):
+    out = mgv.DeviceBuffer(32)
+    gpu.call("mgv_fill_u32", out.vp, 8, 0, None)
+    gpu.call("mgv_debug_xcc_histogram", out.vp, 4096, None)
+    gpu_sync()
+    h = out.download(np.uint32, 8)
+    assert int(h.sum()) == 4096
+    print("blocks per XCD:", h.tolist())
+
+
+def test_join_mid_size_partitioned_exact(gpu, orc):
+    ns, nr = 1 << 20, 1 << 23
+    sk = gen(gpu, "perm", ns, ns, orc.seed(2, 0), 0)
+    sw = gen(gpu, "bits", ns, orc.seed(2, 1), 0, 16)
+    rk = gen(gpu, "bits", nr, orc.seed(2, 2), 0, 21)
+    rv = gen(gpu, "bits", nr, orc.seed(2, 3), 0, 16)
+    ht, _ = build(gpu, sk, sw, ns)
+    direct = probe(gpu, ht, rk, rv, nr)
+    for fanout in (64, 256):
+        r = probe_part(gpu, ht, rk, rv, nr, fanout)
+        assert (r.count, r.sum) == (direct.count, direct.sum)
+    assert direct.count == int((down(rk, nr) < ns).sum())
