@@ -118,8 +118,12 @@ typedef struct mgv_launch {
     uint32_t grid;
     uint32_t block;
     int32_t compact;
-    uint32_t reserved;
+    uint32_t reserved;        /* flags: MGV_LAUNCH_* */
 } mgv_launch;
+
+/* Launch flag: the probe-sum pipelines use the wave-cooperative bucket scan
+ * (4 lanes per row) instead of one row per lane. */
+#define MGV_LAUNCH_COOP_PROBE 1u
 
 /* ------------------------------------------------------------ datagen */
 
@@ -200,6 +204,113 @@ int mgv_hashjoin_probe_groupby(const uint64_t *slots, uint32_t nbuckets, const i
 int mgv_hashjoin_probe_materialize(const uint64_t *slots, uint32_t nbuckets, const int32_t *keys, uint64_t n,
                                    int64_t row_base, int32_t *out_r, int32_t *out_p, uint64_t cap,
                                    mgv_result *res, mgv_launch launch, mgv_stream_t s);
+
+/* ------------------------------------------------------------ buffer pool */
+
+/* Device buffer pool: nbuffers buffers of `rows` int32 (rows a multiple of
+ * 256) in one device arena, each with a device-visible count, and a
+ * lock-free stack of free buffers (64-bit {index, ABA tag} head, one-lane
+ * operations) with the reference's producer / available counters.  Replaces
+ * buffer_pool<T> (src/buffer_pool.cuh:35-170), the device half of
+ * buffer_manager<int32_t> (src/buffer_manager.cuh:40-181) and the device
+ * stacks (src/lockfree_stack.cuh:19-208, src/threadsafe_device_stack.cuh:
+ * 19-87). */
+typedef struct mgv_pool mgv_pool;
+#define MGV_POOL_END (-1)     /* no producer registered and no buffer available (end of stream) */
+#define MGV_POOL_RETRY (-2)   /* 100 attempts failed; the reference's (buffer_t *)1 sentinel */
+int mgv_pool_create(uint32_t nbuffers, uint32_t rows, mgv_pool **pool);
+int mgv_pool_destroy(mgv_pool *pool);
+/* acquire_buffer_blocked_unsafe (src/buffer_pool.cuh:83-96) run by one lane
+ * on the device; *buffer = index, MGV_POOL_END or MGV_POOL_RETRY. */
+int mgv_pool_acquire(mgv_pool *pool, int32_t *buffer, mgv_stream_t s);
+/* release_buffer (src/buffer_pool.cuh:125-130); the buffer count is reset
+ * when it is next acquired. */
+int mgv_pool_release(mgv_pool *pool, int32_t buffer, mgv_stream_t s);
+/* register_producer / unregister_producer (src/buffer_pool.cuh:136-165). */
+int mgv_pool_register_producer(mgv_pool *pool, mgv_stream_t s);
+int mgv_pool_unregister_producer(mgv_pool *pool, mgv_stream_t s);
+/* Device pointers of buffer `buffer`: its rows and its row count. */
+int mgv_pool_buffer(mgv_pool *pool, int32_t buffer, int32_t **data, uint32_t **count);
+int mgv_pool_info(mgv_pool *pool, uint32_t *nbuffers, uint32_t *rows);
+/* available / producers counters and the number of buffers on the stack
+ * (the last only meaningful when no kernel uses the pool). */
+int mgv_pool_stats(mgv_pool *pool, uint32_t *available, uint32_t *producers, uint32_t *stacked, mgv_stream_t s);
+/* Concurrency check of the stack: `blocks` x 4 waves each acquire / verify
+ * exclusive ownership / release `iters` times; ops4 (device, 4 x uint64,
+ * accumulated): pairs done, ownership violations, RETRY results, END results. */
+int mgv_pool_stress(mgv_pool *pool, uint32_t blocks, uint32_t iters, uint64_t *ops4, mgv_stream_t s);
+/* Append {val[i] : pred[i] <= bound} to pool buffer `buffer` with one
+ * atomic reservation per wave (buffer::try_write / try_partial_final_write,
+ * src/buffer.cuh:92-134); rows past the buffer capacity are counted but not
+ * written. */
+int mgv_select_to_buffer(const int32_t *pred, const int32_t *val, uint64_t n, int32_t bound, mgv_pool *pool,
+                         int32_t buffer, mgv_stream_t s);
+
+/* ------------------------------------------------------------ result channel */
+
+/* Device -> host result channel: pinned, coherent slots {flag, mgv_result}.
+ * emit copies a device result into a slot and publishes it with a
+ * system-scope release; poll returns 0 and the result once the slot is set
+ * (and clears it), 1 while it is not.  The reduced gpu_to_cpu ring
+ * (src/operators/gpu_to_cpu.cu:42-58,108-124,149-165). */
+typedef struct mgv_channel mgv_channel;
+int mgv_channel_create(uint32_t nslots, mgv_channel **channel);
+int mgv_channel_destroy(mgv_channel *channel);
+int mgv_channel_emit(mgv_channel *channel, uint32_t slot, const mgv_result *res, uint32_t seq, mgv_stream_t s);
+int mgv_channel_poll(mgv_channel *channel, uint32_t slot, mgv_result *out, uint32_t *seq);
+
+/* ------------------------------------------------------------ fused query batches */
+
+/* Query kinds of a fused batch. */
+#define MGV_Q_SELECT_SUM 1      /* COUNT, SUM(val) WHERE pred <= bound (pred NULL = all rows) */
+#define MGV_Q_PROBE_SUM 2       /* [select] -> probe -> COUNT, SUM(val + payload) [+ digest] */
+#define MGV_Q_PROBE_GROUPBY 3   /* [select] -> probe -> GROUP BY payload: COUNT, SUM(val) */
+#define MGV_Q_GROUPBY 4         /* GROUP BY key: COUNT, SUM(val) */
+#define MGV_Q_BUILD 5           /* [select] -> build (payload_mode) */
+
+/* One query of a batch, device resident.  Row ranges of a query are its
+ * vectors; all pointers are device pointers with 16-byte aligned columns. */
+typedef struct mgv_query {
+    uint32_t kind;
+    int32_t bound;            /* select: pred <= bound; ignored when pred is NULL */
+    const int32_t *pred;
+    const int32_t *key;
+    const int32_t *val;       /* summed / carried column; BUILD payload column */
+    uint64_t n;               /* rows */
+    uint64_t *slots;          /* hash table (PROBE_*, BUILD) */
+    uint32_t nbuckets;
+    uint32_t ngroups;
+    uint64_t *gcnt;           /* group-by outputs */
+    int64_t *gsum;
+    mgv_result *res;          /* COUNT/SUM; errors of builds and group-bys */
+    int32_t payload_mode;     /* BUILD: MGV_PAYLOAD_* */
+    uint32_t flags;           /* PROBE_SUM: MGV_PROBE_DIGEST */
+    int64_t row_base;         /* row id of row 0 (digest / ROWID payload) */
+} mgv_query;
+
+/* One vector of one query: rows [lo, lo + rows). */
+typedef struct mgv_work_item {
+    uint32_t query;
+    uint32_t rows;
+    uint64_t lo;
+} mgv_work_item;
+
+/* Host-side: cut every query (host copy of the table, same device pointers)
+ * into vectors of vector_rows rows (a multiple of 256), in query order;
+ * *nitems = number of items (also when items is NULL).  Validates the
+ * queries.  Replaces the generator's slicing of a column into buffers
+ * (src/operators/generators.cu:55-90). */
+int mgv_batch_plan(const mgv_query *queries, uint32_t nqueries, uint32_t vector_rows, mgv_work_item *items,
+                   uint64_t cap, uint64_t *nitems);
+
+/* One persistent launch drains the work items of a batch (device arrays):
+ * each wave claims a vector with one atomic on *counter (reset by this call
+ * with a fill kernel, so the call can be captured into a graph) and runs its
+ * query's chain.  Builds must be in an earlier call than the probes that
+ * use their tables.  Replaces one launch_consume_pipeline per vector
+ * (src/operators/exchange.cu:124-194,196-291) with one launch per batch. */
+int mgv_batch_run(const mgv_query *queries, const mgv_work_item *items, uint64_t nitems, uint32_t *counter,
+                  mgv_launch launch, mgv_stream_t s);
 
 /* ------------------------------------------------------------ partitioned probe */
 

@@ -42,13 +42,24 @@ inline int check_launch(const char *what) { return hip_status(hipGetLastError(),
 
 inline bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
-// Persistent grid for a kernel: CUs x resident blocks per CU (cached by caller).
+// Persistent grid for a kernel: CUs x resident blocks per CU, computed from
+// the kernel's register and LDS use (512 VGPRs per SIMD lane, 8 waves per
+// SIMD at most, 4 SIMDs and 160 KiB of LDS per CU).
 template <typename K>
 int persistent_blocks(K kernel, int block) {
     DeviceState *st = current_state();
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1)
-        per_cu = 1;
+    hipFuncAttributes a;
+    int per_cu = 1;
+    if (hipFuncGetAttributes(&a, (const void *)kernel) == hipSuccess) {
+        const int waves_per_block = (block + 63) / 64;
+        const int regs = a.numRegs > 0 ? ((a.numRegs + 7) / 8) * 8 : 128;
+        int waves_per_simd = 512 / regs;
+        if (waves_per_simd > 8) waves_per_simd = 8;
+        int by_regs = waves_per_simd * 4 / waves_per_block;
+        int by_lds = a.sharedSizeBytes > 0 ? (int)((160 * 1024) / a.sharedSizeBytes) : 64;
+        per_cu = by_regs < by_lds ? by_regs : by_lds;
+        if (per_cu < 1) per_cu = 1;
+    }
     int cus = (st && st->cus > 0) ? st->cus : 256;
     return cus * per_cu;
 }

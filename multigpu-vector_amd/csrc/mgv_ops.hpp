@@ -77,6 +77,20 @@ struct OpAggregate {
         }
     }
 
+    // One row per lane (probe output): c0 = payload, c1 = value, c2 = row id.
+    static constexpr bool kUsesC2 = DIGEST;
+    __device__ __forceinline__ void consume_one(int32_t c0, int32_t c1, int32_t c2, bool valid) {
+        cnt += valid ? 1u : 0u;
+        int64_t s = (int64_t)c0;
+        if (NSUM > 1) s += (int64_t)c1;
+        sum += valid ? (uint64_t)s : 0ull;
+        if (DIGEST && valid) {
+            uint64_t d = mix64(((uint64_t)(uint32_t)c2 << 32) | (uint64_t)(uint32_t)c0);
+            dsum += d;
+            dxor ^= d;
+        }
+    }
+
     __device__ void consume_close() {
         const uint32_t w = wave_id(), l = lane_id();
         uint64_t a = wave_sum_u64(cnt), b = wave_sum_u64(sum);
@@ -160,6 +174,24 @@ struct OpGroupBy {
         }
     }
 
+    static constexpr bool kUsesC2 = false;
+    __device__ __forceinline__ void consume_one(int32_t c0, int32_t c1, int32_t, bool valid) {
+        const uint32_t g = (uint32_t)c0;
+        if (valid) {
+            if (g < ngroups) {
+                if (LDS_GROUPS > 0) {
+                    atomicAdd(&sm->cnt[g], 1u);
+                    atomicAdd(&sm->sum[g], (unsigned long long)(int64_t)c1);
+                } else {
+                    atomicAdd((unsigned long long *)&gcnt[g], 1ull);
+                    atomicAdd((unsigned long long *)&gsum[g], (unsigned long long)(int64_t)c1);
+                }
+            } else {
+                ++errs;
+            }
+        }
+    }
+
     __device__ void consume_close() {
         if (LDS_GROUPS > 0) {
             __syncthreads();
@@ -174,6 +206,115 @@ struct OpGroupBy {
         uint64_t e = wave_sum_u64(errs);
         if (res && lane_id() == 0 && e) atomicAdd((unsigned long long *)&res->errors, (unsigned long long)e);
         __syncthreads();
+    }
+};
+
+// Wave-level variants for the fused multi-query kernel, whose waves work on
+// different queries at the same time: consume_close reduces within the wave
+// (DPP) and issues one atomic per result field from lane 0, with no block
+// barrier.
+template <int NSUM, bool DIGEST>
+struct OpAggWave {
+    struct Smem {
+        int dummy;
+    };
+    Smem *sm;
+    mgv_result *res;
+    uint64_t cnt, sum, dsum, dxor;
+
+    __device__ void consume_open() { cnt = sum = dsum = dxor = 0; }
+
+    __device__ __forceinline__ void consume(const Rows &r) {
+        cnt += (uint64_t)__popc(r.m & 0xFu);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if ((r.m >> j) & 1u) {
+                int64_t s = 0;
+#pragma unroll
+                for (int i = 0; i < NSUM; ++i) s += (int64_t)q_get(r.c[i], j);
+                sum += (uint64_t)s;
+                if (DIGEST) {
+                    uint64_t d = mix64(((uint64_t)(uint32_t)q_get(r.c[2], j) << 32) |
+                                       (uint64_t)(uint32_t)q_get(r.c[0], j));
+                    dsum += d;
+                    dxor ^= d;
+                }
+            }
+        }
+    }
+
+    // One row per lane (probe output): c0 = payload, c1 = value, c2 = row id.
+    static constexpr bool kUsesC2 = DIGEST;
+    __device__ __forceinline__ void consume_one(int32_t c0, int32_t c1, int32_t c2, bool valid) {
+        cnt += valid ? 1u : 0u;
+        int64_t s = (int64_t)c0;
+        if (NSUM > 1) s += (int64_t)c1;
+        sum += valid ? (uint64_t)s : 0ull;
+        if (DIGEST && valid) {
+            uint64_t d = mix64(((uint64_t)(uint32_t)c2 << 32) | (uint64_t)(uint32_t)c0);
+            dsum += d;
+            dxor ^= d;
+        }
+    }
+
+    __device__ void consume_close() {
+        const uint64_t a = wave_sum_u64(cnt), b = wave_sum_u64(sum);
+        const uint64_t c = DIGEST ? wave_sum_u64(dsum) : 0, d = DIGEST ? wave_xor_u64(dxor) : 0;
+        if (lane_id() == 0 && a) {
+            atomicAdd((unsigned long long *)&res->count, (unsigned long long)a);
+            atomicAdd((unsigned long long *)&res->sum, (unsigned long long)b);
+            if (DIGEST) {
+                atomicAdd((unsigned long long *)&res->digest_sum, (unsigned long long)c);
+                atomicXor((unsigned long long *)&res->digest_xor, (unsigned long long)d);
+            }
+        }
+    }
+};
+
+struct OpGroupWave {
+    struct Smem {
+        int dummy;
+    };
+    Smem *sm;
+    uint64_t *gcnt;
+    int64_t *gsum;
+    uint32_t ngroups;
+    mgv_result *res;
+    uint64_t errs;
+
+    __device__ void consume_open() { errs = 0; }
+
+    __device__ __forceinline__ void consume(const Rows &r) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if ((r.m >> j) & 1u) {
+                const uint32_t g = (uint32_t)q_get(r.c[0], j);
+                if (g < ngroups) {
+                    atomicAdd((unsigned long long *)&gcnt[g], 1ull);
+                    atomicAdd((unsigned long long *)&gsum[g], (unsigned long long)(int64_t)q_get(r.c[1], j));
+                } else {
+                    ++errs;
+                }
+            }
+        }
+    }
+
+    static constexpr bool kUsesC2 = false;
+    __device__ __forceinline__ void consume_one(int32_t c0, int32_t c1, int32_t, bool valid) {
+        const uint32_t g = (uint32_t)c0;
+        if (valid) {
+            if (g < ngroups) {
+                atomicAdd((unsigned long long *)&gcnt[g], 1ull);
+                atomicAdd((unsigned long long *)&gsum[g], (unsigned long long)(int64_t)c1);
+            } else {
+                ++errs;
+            }
+        }
+    }
+
+    __device__ void consume_close() {
+        const uint64_t e = wave_sum_u64(errs);
+        if (res && lane_id() == 0 && e) atomicAdd((unsigned long long *)&res->errors, (unsigned long long)e);
     }
 };
 
@@ -230,18 +371,42 @@ struct OpMaterialize {
         }
     }
 
+    static constexpr bool kUsesC2 = (CA == 2 || (TWO && CB == 2));
+    __device__ __forceinline__ void consume_one(int32_t c0, int32_t c1, int32_t c2, bool valid) {
+        const uint64_t b = ballot(valid);
+        const uint32_t tot = popc64(b);
+        if (tot == 0) return;
+        unsigned long long base = 0;
+        if (lane_id() == 0) {
+            base = atomicAdd((unsigned long long *)&res->count, (unsigned long long)tot);
+            unsigned long long end = base + tot;
+            if (end > cap) atomicAdd((unsigned long long *)&res->overflow, end - (base > cap ? base : cap));
+        }
+        base = __builtin_amdgcn_readlane((uint32_t)base, 0) |
+               ((unsigned long long)__builtin_amdgcn_readlane((uint32_t)(base >> 32), 0) << 32);
+        const uint64_t pos = base + mbcnt(b);
+        if (valid && pos < cap) {
+            const int32_t v[3] = {c0, c1, c2};
+            out_a[pos] = v[CA];
+            if (TWO) out_b[pos] = v[CB];
+        }
+    }
+
     __device__ void consume_close() {}
 };
 
 // ============================================================== hash join
 
-// Probe: c0 = key, c1.. = carried columns.  For every slot whose key equals
-// the probe key, the parent receives a row {c0 = slot payload, c1 = in.c1,
-// c2 = in.c2}.  Replaces hashjoin::consume_warp (src/operators/hashjoin.cu:
-// 135-182): the chain walk over first[]/next[] becomes a scan of 64-byte
-// buckets, each read with four dwordx4 loads per lane (one cache line), for
-// all four rows of a lane before any compare, and the per-lane atomicAdd of
-// the count becomes the parent's register accumulation.
+// Probe, one row per lane: c0 = key, c1 = val, c2 = row id; for every slot
+// whose key equals the probe key the parent receives one row {payload, val,
+// row id} through consume_one.  Each lane hashes its four rows and reads
+// their four 64-byte buckets with four dwordx4 loads each (16 loads in
+// flight), compares the 8 keys of a bucket and selects the matching payload
+// with a priority chain.  Per-row bookkeeping stays in scalar lane masks:
+// hits, "two or more hits" (duplicate build keys, handled one slot at a
+// time) and "bucket full" (continue into the next bucket) are built with
+// s_or/s_and on the compare results.  Replaces hashjoin::consume_warp
+// (src/operators/hashjoin.cu:135-182).
 template <class Parent>
 struct OpProbe {
     typedef typename Parent::Smem Smem;
@@ -252,62 +417,193 @@ struct OpProbe {
     __device__ void consume_open() { parent.consume_open(); }
     __device__ void consume_close() { parent.consume_close(); }
 
-    __device__ __forceinline__ void consume(const Rows &in) {
-        uint32_t b[4];
+    // Scan one bucket (8 slots in q[0..3]) for `key`; emit the hits.
+    // Returns whether the bucket is full (slot 7 used).  Rows that are not
+    // pending carry an all-empty bucket and key 0, which matches nothing;
+    // CAREFUL (some probe key is -1, the key half of an empty slot) adds the
+    // payload half to the compare.
+    template <bool CAREFUL>
+    __device__ __forceinline__ bool scan(const v4u (&q)[4], uint32_t key, int32_t val, int32_t rid) {
+        bool any = false, two = false;
+        uint32_t sel = 0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = bucket_of(q_get(in.c[0], j), nbuckets);
-        uint32_t pending = in.m & 0xFu;
-        // Every pending row advances one bucket per pass, so nbuckets passes
-        // cover the whole table even if it has no empty slot.
-        for (uint32_t pass = 0; ballot(pending != 0) && pass < nbuckets; ++pass) {
-            v4u q[4][4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if ((pending >> j) & 1u) {
-                    const v4u *p = reinterpret_cast<const v4u *>(slots + (uint64_t)b[j] * 8);
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) q[j][t] = p[t];
-                } else {
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) q[j][t] = v4u{~0u, ~0u, ~0u, ~0u};
-                }
-            }
+        for (int s = 7; s >= 0; --s) {
+            const v4u &w = q[s >> 1];
+            const uint32_t k = (s & 1) ? w.z : w.x;
+            const uint32_t p = (s & 1) ? w.w : w.y;
+            const bool h = CAREFUL ? (k == key && !(k == ~0u && p == ~0u)) : (k == key);
+            two = two || (any && h);
+            any = any || h;
+            sel = h ? p : sel;   // lowest hitting slot wins
+        }
+        if (ballot(two)) {   // duplicate build keys: emit every hitting slot
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
-                Rows out;
-                out.c[1] = in.c[1];
-                out.c[2] = in.c[2];
-                out.c[3] = in.c[3];
-                uint32_t mm = 0;
-                int32_t pay[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const v4u &w = q[j][s >> 1];
-                    uint32_t k = (s & 1) ? w.z : w.x;
-                    uint32_t p = (s & 1) ? w.w : w.y;
-                    pay[j] = (int32_t)p;
-                    bool hit = ((pending >> j) & 1u) && k == (uint32_t)q_get(in.c[0], j) && !(k == ~0u && p == ~0u);
-                    mm |= (uint32_t)hit << j;
-                }
-                out.c[0] = v4i{pay[0], pay[1], pay[2], pay[3]};
-                out.m = mm;
-                if (ballot(mm != 0)) parent.consume(out);
+                const v4u &w = q[s >> 1];
+                const uint32_t k = (s & 1) ? w.z : w.x;
+                const uint32_t p = (s & 1) ? w.w : w.y;
+                const bool h = CAREFUL ? (k == key && !(k == ~0u && p == ~0u)) : (k == key);
+                parent.consume_one((int32_t)p, val, rid, h);
             }
-            // A bucket with an empty slot ends the scan (buckets fill as a
-            // prefix); a full bucket continues into the next one.
+        } else {
+            parent.consume_one((int32_t)sel, val, rid, any);
+        }
+        return !(q[3].z == ~0u && q[3].w == ~0u);
+    }
+
+    __device__ __forceinline__ void load_bucket(v4u (&q)[4], uint32_t b, bool pend) const {
+        if (pend) {
+            const v4u *p = reinterpret_cast<const v4u *>(slots + (uint64_t)b * 8);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) q[t] = p[t];
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) q[t] = v4u{~0u, ~0u, ~0u, ~0u};
+        }
+    }
+
+    template <bool CAREFUL>
+    __device__ __forceinline__ void consume_rows(const Rows &in, const uint32_t (&b)[4]) {
+        uint32_t key[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) key[j] = ((in.m >> j) & 1u) ? (uint32_t)q_get(in.c[0], j) : 0u;
+        v4u q[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) load_bucket(q[j], b[j], (in.m >> j) & 1u);
+        uint32_t cont = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool full = scan<CAREFUL>(q[j], key[j], q_get(in.c[1], j), q_get(in.c[2], j));
+            cont |= (uint32_t)(((in.m >> j) & 1u) && full) << j;
+        }
+        // rare: rows whose bucket was full continue into the next bucket
+        if (ballot(cont != 0)) {
+#pragma unroll 1
+            for (int j = 0; j < 4; ++j) {
+                bool pend = (cont >> j) & 1u;
+                uint32_t bb = b[j];
+                for (uint32_t pass = 1; ballot(pend) && pass < nbuckets; ++pass) {
+                    bb = (bb + 1 == nbuckets) ? 0 : bb + 1;
+                    v4u qq[4];
+                    load_bucket(qq, bb, pend);
+                    const bool full = scan<CAREFUL>(qq, pend ? key[j] : 0u, q_get(in.c[1], j), q_get(in.c[2], j));
+                    pend = pend && full;
+                }
+            }
+        }
+    }
+
+    __device__ __forceinline__ void consume(const Rows &in) {
+        uint32_t b[4];
+        bool neg = false;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int32_t k = q_get(in.c[0], j);
+            b[j] = bucket_of(k, nbuckets);
+            neg |= ((in.m >> j) & 1u) && k == -1;
+        }
+        if (ballot(neg)) consume_rows<true>(in, b);
+        else consume_rows<false>(in, b);
+    }
+};
+
+// Wave-cooperative probe (same contract as OpProbe).  Each lane hashes its
+// own four rows; the chunk is then taken in 16 rounds in which the wave's 16
+// groups of 4 lanes take one row each (key, bucket and value arrive by
+// ds_bpermute) and every lane of a group reads one 16-byte quarter (2 slots)
+// of the row's bucket with one dwordx4 load.  A load instruction thus touches
+// 16 cache lines for 16 rows, where the one-row-per-lane probe touches 64
+// lines for 16 rows' worth of buckets; the texture addresser's per-line work,
+// which bounds the probe, drops accordingly.
+template <class Parent>
+struct OpProbeCoop {
+    typedef typename Parent::Smem Smem;
+    Parent parent;
+    const uint64_t *slots;
+    uint32_t nbuckets;
+
+    __device__ void consume_open() { parent.consume_open(); }
+    __device__ void consume_close() { parent.consume_close(); }
+
+    __device__ __forceinline__ static int32_t bperm(uint32_t src_lane, int32_t v) {
+        return __builtin_amdgcn_ds_bpermute((int)(src_lane << 2), v);
+    }
+
+    // Hits of this lane's two slots; returns "this group's bucket is full".
+    template <bool CAREFUL>
+    __device__ __forceinline__ bool round(const v4u &q, uint32_t key, int32_t val, int32_t rid, uint32_t grp,
+                                          uint32_t sub) {
+        const bool h0 = CAREFUL ? (q.x == key && !(q.x == ~0u && q.y == ~0u)) : (q.x == key);
+        const bool h1 = CAREFUL ? (q.z == key && !(q.z == ~0u && q.w == ~0u)) : (q.z == key);
+        if (ballot(h0 && h1)) {   // duplicate build keys in both slots
+            parent.consume_one((int32_t)q.y, val, rid, h0);
+            parent.consume_one((int32_t)q.w, val, rid, h1);
+        } else {
+            parent.consume_one((int32_t)(h0 ? q.y : q.w), val, rid, h0 || h1);
+        }
+        const uint64_t fb = ballot(sub == 3 && !(q.z == ~0u && q.w == ~0u));
+        return (fb >> (4 * grp + 3)) & 1ull;
+    }
+
+    template <bool CAREFUL>
+    __device__ __forceinline__ void consume_rows(const Rows &in, const int32_t (&bk)[4]) {
+        const uint32_t l = lane_id();
+        const uint32_t sub = l & 3u;
+        const uint32_t grp = l >> 2;
+        int32_t kin[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) kin[j] = ((in.m >> j) & 1u) ? q_get(in.c[0], j) : 0;
+#pragma unroll 1
+        for (int batch = 0; batch < 4; ++batch) {
+            const uint32_t src = 16u * batch + grp;
+            const uint32_t mv = (uint32_t)bperm(src, (int32_t)in.m) & 0xFu;
+            if (!ballot(mv != 0)) continue;
+            uint32_t key[4], b[4];
+            int32_t val[4], rid[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                if ((pending >> j) & 1u) {
-                    const v4u &w = q[j][3];
-                    bool full = !(w.z == ~0u && w.w == ~0u);
-                    if (full) {
-                        b[j] = (b[j] + 1 == nbuckets) ? 0 : b[j] + 1;
-                    } else {
-                        pending &= ~(1u << j);
+                key[j] = (uint32_t)bperm(src, kin[j]);
+                b[j] = (uint32_t)bperm(src, bk[j]);
+                val[j] = bperm(src, q_get(in.c[1], j));
+                rid[j] = Parent::kUsesC2 ? bperm(src, q_get(in.c[2], j)) : 0;
+            }
+            v4u q[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if ((mv >> j) & 1u) q[j] = *reinterpret_cast<const v4u *>(slots + (uint64_t)b[j] * 8 + 2 * sub);
+                else q[j] = v4u{~0u, ~0u, ~0u, ~0u};
+            }
+            uint32_t cont = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                cont |= (uint32_t)(round<CAREFUL>(q[j], key[j], val[j], rid[j], grp, sub) && ((mv >> j) & 1u)) << j;
+            if (ballot(cont != 0)) {
+#pragma unroll 1
+                for (int j = 0; j < 4; ++j) {
+                    bool pend = (cont >> j) & 1u;
+                    uint32_t bb = b[j];
+                    for (uint32_t pass = 1; ballot(pend) && pass < nbuckets; ++pass) {
+                        bb = (bb + 1 == nbuckets) ? 0 : bb + 1;
+                        v4u qq = v4u{~0u, ~0u, ~0u, ~0u};
+                        if (pend) qq = *reinterpret_cast<const v4u *>(slots + (uint64_t)bb * 8 + 2 * sub);
+                        pend = round<CAREFUL>(qq, pend ? key[j] : 0u, val[j], rid[j], grp, sub) && pend;
                     }
                 }
             }
         }
+    }
+
+    __device__ __forceinline__ void consume(const Rows &in) {
+        int32_t bk[4];
+        bool neg = false;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int32_t k = q_get(in.c[0], j);
+            bk[j] = (int32_t)bucket_of(k, nbuckets);
+            neg |= ((in.m >> j) & 1u) && k == -1;
+        }
+        if (ballot(neg)) consume_rows<true>(in, bk);
+        else consume_rows<false>(in, bk);
     }
 };
 

@@ -22,10 +22,10 @@
 namespace mgv {
 
 constexpr int kPartBlock = 256;
-constexpr int kTile = kPartBlock * 8;      // rows staged in LDS per scatter step
+constexpr int kTile = kPartBlock * 16;     // rows staged in LDS per scatter step
 constexpr uint32_t kMaxFanout = 512;
 constexpr uint32_t kPartAlign = 64;        // partition starts aligned to 64 rows
-constexpr uint32_t kItemRows = 16384;      // probe work item
+constexpr uint32_t kItemRows = 4096;       // probe work item (one wave)
 constexpr uint32_t kMaxPartBlocks = 4096;
 
 // XCC (XCD) id of the executing wave: HW_REG_XCC_ID (hwreg 20), bits [3:0].
@@ -181,34 +181,38 @@ __global__ __launch_bounds__(kMaxFanout) void k_part_scan(PartLayout L) {
     if (t == 0) L.pstart[L.F] = s[L.F - 1];
 }
 
-__device__ __forceinline__ uint32_t scan_block_excl(uint32_t v, uint32_t *tmp) {
-    // exclusive scan over kPartBlock threads through LDS (Hillis-Steele)
-    const uint32_t t = threadIdx.x;
-    tmp[t] = v;
-    __syncthreads();
-    for (uint32_t d = 1; d < kPartBlock; d <<= 1) {
-        uint32_t a = (t >= d) ? tmp[t - d] : 0;
-        __syncthreads();
-        tmp[t] += a;
-        __syncthreads();
+// Exclusive scan of one value per thread over a kPartBlock-thread block:
+// wave-level inclusive scan with ds_bpermute shuffles, then the wave totals.
+__device__ __forceinline__ uint32_t scan_block_excl(uint32_t v, uint32_t *wtot) {
+    const uint32_t l = lane_id(), w = wave_id();
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, kWave);
+        if (l >= (uint32_t)d) x += y;
     }
-    uint32_t incl = tmp[t];
+    if (l == kWave - 1) wtot[w] = x;
     __syncthreads();
-    return incl - v;
+    uint32_t before = 0;
+    for (uint32_t i = 0; i < w; ++i) before += wtot[i];
+    __syncthreads();
+    return before + x - v;
 }
 
 // LDS-staged scatter: each tile of kTile rows is counting-sorted by partition
 // in LDS, then written out so that consecutive lanes write consecutive slots
-// of a partition's run.
+// of a partition's run.  The next tile's rows are loaded before the current
+// tile is staged, so loads overlap the LDS work.
+template <bool ROWID>
 __global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartIn in) {
+    constexpr int Q = kTile / (4 * kPartBlock);   // quads per thread per tile
     __shared__ unsigned long long cursor[kMaxFanout];
     __shared__ uint32_t tcnt[kMaxFanout];
     __shared__ uint32_t toff[kMaxFanout];
-    __shared__ uint32_t tmp[kPartBlock];
+    __shared__ uint32_t wtot[kPartBlock / kWave];
     __shared__ int32_t sk[kTile];
     __shared__ int32_t sv[kTile];
-    __shared__ int32_t sr[kTile];
-    __shared__ uint16_t sp[kTile];
+    __shared__ int32_t sr[ROWID ? kTile : 1];
     const uint32_t F = L.F;
     const uint32_t t = threadIdx.x;
     for (uint32_t p = t; p < F; p += kPartBlock) {
@@ -216,47 +220,54 @@ __global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartI
         cursor[p] = c ? atomicAdd(&L.pcursor[p], (unsigned long long)c) : 0;
         tcnt[p] = 0;
     }
-    __syncthreads();
     const uint64_t lo = (uint64_t)blockIdx.x * L.rows_per_blk;
     const uint64_t hi = min(L.n, lo + L.rows_per_blk);
-    for (uint64_t tb = lo; tb < hi; tb += kTile) {
-        // 1. load 8 rows per thread (2 quads), rank each row within its partition
-        uint32_t rank[8], part[8];
-        int32_t kk[8], vv[8];
-        uint32_t mm = 0;
+    v4i nk[Q], nv[Q];
+    uint32_t nm[Q];
+    auto load_tile = [&](uint64_t tb) {
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < Q; ++q) {
             const uint64_t i0 = tb + 4ull * t + (uint64_t)q * 4 * kPartBlock;
             v4i c;
-            uint32_t m = (i0 < hi) ? quad_mask(in, i0, hi, &c) : 0u;
-            v4i k = {0, 0, 0, 0}, v = {0, 0, 0, 0};
-            if (m) {
-                k = load_quad(in.key, i0, hi);
-                if (in.val) v = load_quad(in.val, i0, hi);
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int r = q * 4 + j;
-                kk[r] = q_get(k, j);
-                vv[r] = q_get(v, j);
-                if ((m >> j) & 1u) {
-                    part[r] = part_of(kk[r], F);
-                    rank[r] = atomicAdd(&tcnt[part[r]], 1u);
-                    mm |= 1u << r;
-                }
+            nm[q] = (i0 < hi) ? quad_mask(in, i0, hi, &c) : 0u;
+            nk[q] = v4i{0, 0, 0, 0};
+            nv[q] = v4i{0, 0, 0, 0};
+            if (nm[q]) {
+                nk[q] = load_quad(in.key, i0, hi);
+                if (in.val) nv[q] = load_quad(in.val, i0, hi);
             }
         }
+    };
+    if (lo < hi) load_tile(lo);
+    __syncthreads();
+    for (uint64_t tb = lo; tb < hi; tb += kTile) {
+        v4i ck[Q], cv[Q];
+        uint32_t cm[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            ck[q] = nk[q];
+            cv[q] = nv[q];
+            cm[q] = nm[q];
+        }
+        if (tb + kTile < hi) load_tile(tb + kTile);
+        // 1. rank every row within its partition
+        uint32_t rank[Q][4];
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                rank[q][j] = ((cm[q] >> j) & 1u) ? atomicAdd(&tcnt[part_of(q_get(ck[q], j), F)], 1u) : 0u;
         __syncthreads();
-        // 2. exclusive scan of the tile counts
+        // 2. exclusive scan of the tile counts (<= 2 partitions per thread)
+        const uint32_t per = (F + kPartBlock - 1) / kPartBlock;
         uint32_t mine = 0;
-        const uint32_t per = (F + kPartBlock - 1) / kPartBlock;   // partitions per thread (<= 2)
         for (uint32_t e = 0; e < per; ++e) {
-            uint32_t p = t * per + e;
+            const uint32_t p = t * per + e;
             mine += (p < F) ? tcnt[p] : 0;
         }
-        uint32_t base = scan_block_excl(mine, tmp);
+        uint32_t base = scan_block_excl(mine, wtot);
         for (uint32_t e = 0; e < per; ++e) {
-            uint32_t p = t * per + e;
+            const uint32_t p = t * per + e;
             if (p < F) {
                 toff[p] = base;
                 base += tcnt[p];
@@ -266,26 +277,29 @@ __global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartI
         const uint32_t total = toff[F - 1] + tcnt[F - 1];
         // 3. stage rows in partition order
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            if ((mm >> r) & 1u) {
-                const uint32_t pos = toff[part[r]] + rank[r];
-                sk[pos] = kk[r];
-                sv[pos] = vv[r];
-                sp[pos] = (uint16_t)part[r];
-                if (in.want_rowid) {
-                    const int q = r / 4, j = r % 4;
-                    sr[pos] = (int32_t)(in.row_base + (int64_t)(tb + 4ull * t + (uint64_t)q * 4 * kPartBlock + j));
+        for (int q = 0; q < Q; ++q) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if ((cm[q] >> j) & 1u) {
+                    const int32_t k = q_get(ck[q], j);
+                    const uint32_t pos = toff[part_of(k, F)] + rank[q][j];
+                    sk[pos] = k;
+                    sv[pos] = q_get(cv[q], j);
+                    if (ROWID)
+                        sr[pos] = (int32_t)(in.row_base +
+                                            (int64_t)(tb + 4ull * t + (uint64_t)q * 4 * kPartBlock + j));
                 }
             }
         }
         __syncthreads();
-        // 4. write runs
+        // 4. write the runs
         for (uint32_t i = t; i < total; i += kPartBlock) {
-            const uint32_t p = sp[i];
+            const int32_t k = sk[i];
+            const uint32_t p = part_of(k, F);
             const unsigned long long dst = cursor[p] + (i - toff[p]);
-            L.col[0][dst] = sk[i];
+            L.col[0][dst] = k;
             if (L.ncols > 1) L.col[1][dst] = sv[i];
-            if (L.ncols > 2) L.col[2][dst] = sr[i];
+            if (ROWID) L.col[2][dst] = sr[i];
         }
         __syncthreads();
         for (uint32_t p = t; p < F; p += kPartBlock) {
@@ -296,61 +310,68 @@ __global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartI
     }
 }
 
-// Work item i of XCD x -> (partition, first row, last row).  Partitions of XCD
-// x are p = x, x + 8, ... (fewer than 8 partitions: all on XCD 0..F-1).
-struct ItemMap {
-    __device__ static uint32_t nparts_of(uint32_t F, uint32_t x) {
-        return F > x ? (F - x + 7) / 8 : 0;
-    }
-};
-
+// Persistent probe over the partitions.  Each block copies the partition
+// table into LDS and builds, for every XCD x, the cumulative item counts of
+// its partitions p = x, x + 8, ...; afterwards every wave claims work items
+// (kItemRows rows of one partition) on its own with one atomic, first from
+// its own XCD's queue, then from the others'.
 template <class Pipe, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_part_probe(typename Pipe::Params prm, PartLayout L) {
     constexpr int WAVES = BLOCK / kWave;
     typedef typename Pipe::template Op<WAVES> Op;
     __shared__ __align__(16) typename Op::Smem sm;
-    __shared__ unsigned long long item_lo, item_hi;
-    __shared__ uint32_t item_ok;
+    __shared__ unsigned long long s_start[kMaxFanout];
+    __shared__ unsigned long long s_size[kMaxFanout];
+    __shared__ uint32_t s_cum[kMaxFanout];   // cumulative items, in per-XCD order
+    __shared__ uint32_t s_tot[8];
     Op op;
     Pipe::setup(op, prm, &sm);
-    op.consume_open();
+    const uint32_t F = L.F;
+    for (uint32_t p = threadIdx.x; p < F; p += BLOCK) {
+        s_start[p] = L.pstart[p];
+        s_size[p] = L.ptotal[p];
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const uint32_t x = threadIdx.x;
+        uint32_t acc = 0;
+        for (uint32_t p = x; p < F; p += 8) {
+            acc += (uint32_t)((s_size[p] + kItemRows - 1) / kItemRows);
+            s_cum[p] = acc;
+        }
+        s_tot[x] = acc;
+    }
+    op.consume_open();   // contains a block barrier
+    __syncthreads();
     const uint32_t home = xcc_id();
     Cols cols = {{L.col[0], L.ncols > 1 ? L.col[1] : nullptr, L.ncols > 2 ? L.col[2] : nullptr, nullptr}, -1};
     for (uint32_t k = 0; k < 8; ++k) {
         const uint32_t x = (home + k) & 7u;
+        const uint32_t tot = s_tot[x];
+        const uint32_t np = F > x ? (F - x + 7) / 8 : 0;
         for (;;) {
-            if (threadIdx.x == 0) {
-                item_ok = 0;
-                const uint32_t i = atomicAdd(&L.xhead[x], 1u);
-                // find the partition of item i among p = x, x+8, ...
-                uint32_t acc = 0;
-                for (uint32_t p = x; p < L.F; p += 8) {
-                    const unsigned long long s0 = L.pstart[p];
-                    const unsigned long long sz = L.ptotal[p];
-                    const uint32_t ni = (uint32_t)((sz + kItemRows - 1) / kItemRows);
-                    if (i < acc + ni) {
-                        const unsigned long long a = s0 + (unsigned long long)(i - acc) * kItemRows;
-                        const unsigned long long e = s0 + sz;
-                        item_lo = a;
-                        item_hi = (a + kItemRows < e) ? a + kItemRows : e;
-                        item_ok = 1;
-                        break;
-                    }
-                    acc += ni;
-                }
+            uint32_t item = 0;
+            if (lane_id() == 0) item = atomicAdd(&L.xhead[x], 1u);
+            item = __builtin_amdgcn_readlane(item, 0);
+            if (item >= tot) break;
+            // smallest m with s_cum[x + 8m] > item
+            uint32_t a = 0, b = np - 1;
+            while (a < b) {
+                const uint32_t mid = (a + b) / 2;
+                if (s_cum[x + 8 * mid] > item) b = mid; else a = mid + 1;
             }
-            __syncthreads();
-            const uint32_t ok = item_ok;
-            const uint64_t lo = item_lo, hi = item_hi;
-            __syncthreads();
-            if (!ok) break;
-            for (uint64_t base = lo + (uint64_t)wave_id() * kChunk; base < hi; base += 2ull * WAVES * kChunk) {
-                Rows r0, r1;
-                load_rows(r0, cols, base, hi, 0);
-                const uint64_t b1 = base + (uint64_t)WAVES * kChunk;
-                if (b1 < hi) load_rows(r1, cols, b1, hi, 0);
-                op.consume(r0);
-                if (b1 < hi) op.consume(r1);
+            const uint32_t p = x + 8 * a;
+            const uint32_t first = a ? s_cum[x + 8 * (a - 1)] : 0;
+            const uint64_t ilo = s_start[p] + (uint64_t)(item - first) * kItemRows;
+            const uint64_t pend = s_start[p] + s_size[p];
+            const uint64_t ihi = (ilo + kItemRows < pend) ? ilo + kItemRows : pend;
+            Rows cur;
+            load_rows(cur, cols, ilo, ihi, 0);
+            for (uint64_t base = ilo; base < ihi; base += kChunk) {
+                Rows nxt;
+                if (base + kChunk < ihi) load_rows(nxt, cols, base + kChunk, ihi, 0);
+                op.consume(cur);
+                cur = nxt;
             }
         }
     }
@@ -364,24 +385,23 @@ __global__ __launch_bounds__(BLOCK) void k_part_probe(typename Pipe::Params prm,
 namespace mgv {
 
 template <class Pipe, int B>
-static int part_probe_b(const typename Pipe::Params &prm, const PartLayout &L, hipStream_t s) {
+static int part_probe_b(const typename Pipe::Params &prm, const PartLayout &L, const mgv_launch &LC, hipStream_t s) {
     auto kern = k_part_probe<Pipe, B>;
     static thread_local int cached = 0;
     if (!cached) cached = persistent_blocks(kern, B);
-    hipLaunchKernelGGL(kern, dim3(cached), dim3(B), 0, s, prm, L);
+    const uint32_t grid = LC.grid ? LC.grid : (uint32_t)cached;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(B), 0, s, prm, L);
     return check_launch("k_part_probe");
 }
 
 template <class Pipe>
 static int part_probe(const typename Pipe::Params &prm, const PartLayout &L, const mgv_launch &LC, hipStream_t s) {
     switch (LC.block) {
-        case 64: return part_probe_b<Pipe, 64>(prm, L, s);
-        case 128: return part_probe_b<Pipe, 128>(prm, L, s);
+        case 64: return part_probe_b<Pipe, 64>(prm, L, LC, s);
         case 0:
-        case 256: return part_probe_b<Pipe, 256>(prm, L, s);
-        case 512: return part_probe_b<Pipe, 512>(prm, L, s);
-        case 1024: return part_probe_b<Pipe, 1024>(prm, L, s);
-        default: return fail(MGV_E_INVALID, "block %u not in {64,128,256,512,1024}", LC.block);
+        case 256: return part_probe_b<Pipe, 256>(prm, L, LC, s);
+        case 1024: return part_probe_b<Pipe, 1024>(prm, L, LC, s);
+        default: return fail(MGV_E_INVALID, "block %u not in {64,256,1024}", LC.block);
     }
 }
 
@@ -395,7 +415,10 @@ static int partition_rows(const PartLayout &L, const PartIn &in, hipStream_t s) 
     if ((rc = check_launch("k_part_count"))) return rc;
     hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kMaxFanout), 0, s, L);
     if ((rc = check_launch("k_part_scan"))) return rc;
-    hipLaunchKernelGGL(k_part_scatter, dim3(L.nblk), dim3(kPartBlock), 0, s, L, in);
+    if (in.want_rowid)
+        hipLaunchKernelGGL(k_part_scatter<true>, dim3(L.nblk), dim3(kPartBlock), 0, s, L, in);
+    else
+        hipLaunchKernelGGL(k_part_scatter<false>, dim3(L.nblk), dim3(kPartBlock), 0, s, L, in);
     return check_launch("k_part_scatter");
 }
 
@@ -434,6 +457,10 @@ int mgv_hashjoin_probe_partitioned(const uint64_t *slots, uint32_t nbuckets, con
         return part_probe<PProbeSum<false, true, true>>(p, L, LC, s);
     }
     PProbeSum<false, true, false>::Params p = {0, slots, nbuckets, res};
+    if (LC.reserved & MGV_LAUNCH_COOP_PROBE) {
+        PProbeSum<false, true, false, OpProbeCoop>::Params pc = {0, slots, nbuckets, res};
+        return part_probe<PProbeSum<false, true, false, OpProbeCoop>>(pc, L, LC, s);
+    }
     return part_probe<PProbeSum<false, true, false>>(p, L, LC, s);
 }
 

@@ -11,6 +11,7 @@
 
 #include <type_traits>
 
+#include "mgv_internal.hpp"
 #include "mgv_ops.hpp"
 
 namespace mgv {
@@ -61,8 +62,8 @@ __device__ __forceinline__ void load_rows(Rows &r, const Cols &cols, uint64_t ba
     }
 }
 
-// Two wave chunks are loaded before either is consumed so that each lane keeps
-// 2 x 16 B per column in flight.
+// Each wave walks its chunks with a grid stride, loading chunk i+1 while it
+// consumes chunk i.
 template <class Pipe, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_pipeline(typename Pipe::Params prm, Cols cols, uint64_t n,
                                                     int64_t row_base) {
@@ -74,17 +75,16 @@ __global__ __launch_bounds__(BLOCK) void k_pipeline(typename Pipe::Params prm, C
     op.consume_open();
     const uint64_t w = (uint64_t)blockIdx.x * WAVES + wave_id();
     const uint64_t stride = (uint64_t)gridDim.x * WAVES * kChunk;
-    for (uint64_t base = w * kChunk; base < n; base += 2 * stride) {
-        Rows r0, r1;
-        load_rows(r0, cols, base, n, row_base);
-        const uint64_t b1 = base + stride;
-        if (b1 < n) {
-            load_rows(r1, cols, b1, n, row_base);
-        } else {
-            r1.m = 0;
-        }
-        op.consume(r0);
-        if (b1 < n) op.consume(r1);
+    // the next chunk's rows are loaded before the current chunk is consumed
+    uint64_t base = w * kChunk;
+    Rows cur;
+    if (base < n) load_rows(cur, cols, base, n, row_base);
+    for (; base < n; base += stride) {
+        Rows nxt;
+        const uint64_t nb = base + stride;
+        if (nb < n) load_rows(nxt, cols, nb, n, row_base);
+        op.consume(cur);
+        cur = nxt;
     }
     op.consume_close();
 }
@@ -162,7 +162,7 @@ struct PBuild {
 
 // Probe + COUNT/SUM(val + payload) [+ digest]: (c0 key, c1 val, c2 rowid) or,
 // with SEL, (c0 pred, c1 key, c2 val, c3 rowid).
-template <bool SEL, bool COMPACT, bool DIGEST>
+template <bool SEL, bool COMPACT, bool DIGEST, template <class> class PROBE = OpProbe>
 struct PProbeSum {
     struct Params {
         int32_t bound;
@@ -173,8 +173,8 @@ struct PProbeSum {
     template <int W>
     using Agg = OpAggregate<2, DIGEST, W>;
     template <int W>
-    using Op = typename std::conditional<SEL, OpSelect<OpProbe<Agg<W>>, COMPACT, DIGEST ? 3 : 2, W>,
-                                         OpProbe<Agg<W>>>::type;
+    using Op = typename std::conditional<SEL, OpSelect<PROBE<Agg<W>>, COMPACT, DIGEST ? 3 : 2, W>,
+                                         PROBE<Agg<W>>>::type;
     template <class O>
     __device__ static void setup(O &op, const Params &p, typename O::Smem *sm) {
         if constexpr (SEL) {
@@ -278,5 +278,41 @@ struct PGroupBy {
         op.res = p.res;
     }
 };
+
+// ------------------------------------------------------------------ launch
+
+template <class Pipe, int B>
+inline int launch_b(const typename Pipe::Params &prm, const Cols &cols, uint64_t n, int64_t row_base,
+                    const mgv_launch &L, hipStream_t s, const char *name) {
+    auto kern = k_pipeline<Pipe, B>;
+    uint64_t grid = L.grid;
+    if (grid == 0) {
+        static thread_local int cached = 0;
+        if (!cached) cached = persistent_blocks(kern, B);
+        grid = (uint64_t)cached;
+        const uint64_t chunks_per_block = (uint64_t)(B / kWave);
+        const uint64_t need = (n + (uint64_t)kChunk * chunks_per_block - 1) / ((uint64_t)kChunk * chunks_per_block);
+        if (need < grid) grid = need ? need : 1;
+    }
+    if (grid > 0x7fffffffull) return fail(MGV_E_INVALID, "%s: grid too large", name);
+    hipLaunchKernelGGL(kern, dim3((uint32_t)grid), dim3(B), 0, s, prm, cols, n, row_base);
+    return check_launch(name);
+}
+
+template <class Pipe>
+inline int launch_pipe(const typename Pipe::Params &prm, const Cols &cols, uint64_t n, int64_t row_base,
+                       const mgv_launch &L, mgv_stream_t st, const char *name) {
+    hipStream_t s = stream_of(st);
+    for (int c = 0; c < 4; ++c)
+        if (cols.p[c] && !aligned16(cols.p[c]))
+            return fail(MGV_E_ALIGN, "%s: column %d is not 16-byte aligned", name, c);
+    switch (L.block) {
+        case 64: return launch_b<Pipe, 64>(prm, cols, n, row_base, L, s, name);
+        case 0:
+        case 256: return launch_b<Pipe, 256>(prm, cols, n, row_base, L, s, name);
+        case 1024: return launch_b<Pipe, 1024>(prm, cols, n, row_base, L, s, name);
+        default: return fail(MGV_E_INVALID, "%s: block %u not in {64,256,1024}", name, L.block);
+    }
+}
 
 }  // namespace mgv

@@ -1,0 +1,453 @@
+// operators.cpp -- operator framework of the C++ drop-in layer
+// (counterparts: src/operators/operator.cu, exchange.cu, generators.cu,
+// gpu_to_cpu.cu, materializer.cu, hashjoin.cu:63-75).
+#include "mgv/operators.hpp"
+
+#include <chrono>
+#include <climits>
+#include <cstring>
+
+// ------------------------------------------------------------------ hashjoin
+
+template <>
+hashjoin<WARPSIZE>::hashjoin(d_operator_t *parent_, int32_t log_table_size, int32_t max_size, const launch_conf &conf)
+    : parent(parent_), table(std::make_shared<mgv_table_state>()) {
+    (void)log_table_size;
+    if (max_size < 0) throw mgv::error(MGV_E_INVALID, "hashjoin: negative max_size");
+    uint64_t bytes = 0;
+    gpu(mgv_ht_plan((uint64_t)max_size, 0.5, &table->nbuckets, &bytes));
+    void *p = nullptr;
+    gpu(mgv_malloc(&p, bytes));
+    table->slots = (uint64_t *)p;
+    table->max_size = (uint64_t)max_size;
+    gpu(mgv_ht_clear(table->slots, table->nbuckets, nullptr));
+    gpu(mgv_stream_synchronize(nullptr));
+    builder = d_operator_t::create<hashjoin_builder_t>(conf, table);
+}
+
+template <>
+hashjoin<WARPSIZE>::~hashjoin() {
+    d_operator_t::destroy(builder);
+}
+
+// ------------------------------------------------------------------ gpu_to_cpu
+
+template <>
+gpu_to_cpu<WARPSIZE, 64, buffer_t *>::gpu_to_cpu(h_operator_t *parent_, int device_) : parent(parent_), device(device_) {
+    gpu(mgv_channel_create(1, &channel));
+}
+
+template <>
+gpu_to_cpu<WARPSIZE, 64, buffer_t *>::~gpu_to_cpu() {
+    eof = true;
+    if (catcher.joinable()) catcher.join();
+    mgv_channel_destroy(channel);
+}
+
+// ------------------------------------------------------------------ d_operator_t
+
+namespace {
+
+d_operator_t *parent_of(const d_operator_t::op_t &op) {
+    if (auto p = std::get_if<select_t *>(&op)) return (*p)->parent;
+    if (auto p = std::get_if<aggregation_t *>(&op)) return (*p)->parent;
+    if (auto p = std::get_if<hashjoin_t *>(&op)) return (*p)->parent;
+    return nullptr;
+}
+
+}  // namespace
+
+void d_operator_t::resolve() {
+    // the chain from this operator up to its sink
+    std::vector<d_operator_t *> chain;
+    for (d_operator_t *c = this; c; c = parent_of(c->op)) {
+        chain.push_back(c);
+        if (chain.size() > 8) throw mgv::error(MGV_E_INVALID, "operator chain too long or cyclic");
+    }
+    auto is = [&](size_t i, auto tag) {
+        using P = decltype(tag);
+        return i < chain.size() && std::holds_alternative<P>(chain[i]->op);
+    };
+    size_t i = 0;
+    bool sel = false;
+    if (is(0, (select_t *)nullptr)) {
+        sel = true;
+        bound = std::get<select_t *>(chain[0]->op)->filt.bound;
+        i = 1;
+    }
+    if (is(i, (hashjoin_builder_t *)nullptr) && chain.size() == i + 1) {
+        kind = sel ? SELECT_BUILD : BUILD;
+        table = std::get<hashjoin_builder_t *>(chain[i]->op)->table;
+        return;
+    }
+    if (is(i, (hashjoin_t *)nullptr)) {
+        table = std::get<hashjoin_t *>(chain[i]->op)->table;
+        kind = sel ? SELECT_PROBE : PROBE;
+        ++i;
+        // hashjoin hands its match count to its parent once (hashjoin.cu:87-96):
+        // an aggregation above it sums that single value, i.e. the count
+        if (is(i, (aggregation_t *)nullptr)) ++i;
+    } else if (is(i, (aggregation_t *)nullptr)) {
+        kind = SELECT_SUM;
+        if (!sel) bound = INT32_MAX;
+        sum = true;
+        ++i;
+    } else if (sel) {
+        kind = SELECT_MAT;
+    }
+    if (kind == NONE || !is(i, (gpu_to_cpu_t *)nullptr) || i + 1 != chain.size())
+        throw mgv::error(MGV_E_INVALID, "unsupported device operator chain");
+    sink = std::get<gpu_to_cpu_t *>(chain[i]->op);
+}
+
+void d_operator_t::open() {
+    if (opened) return;
+    if (kind == NONE) resolve();
+    if (sink) {
+        // before_open: open the host parent and start the catcher
+        sink->parent->open();
+        sink->eof = false;
+        gpu_to_cpu_t *s = sink;
+        const bool rows = kind == SELECT_MAT;
+        const bool scalar_sum = sum;
+        s->catcher = std::thread([s, rows, scalar_sum]() {
+            mgv_result r;
+            uint32_t seq = 0;
+            for (;;) {
+                if (mgv_channel_poll(s->channel, 0, &r, &seq) == 0) {
+                    s->last = r;
+                    s->out.clear();
+                    if (rows) {
+                        // the selected rows, one host buffer per capacity-sized slice
+                        const size_t total = s->rows.size();
+                        const size_t cap = buffer_t::capacity();
+                        s->out.reserve(total / cap + 1);
+                        for (size_t off = 0; off < total || off == 0; off += cap) {
+                            buffer_t b(s->rows.data() + off, -1);
+                            b.cnt = (uint32_t)std::min(cap, total - off);
+                            s->out.push_back(b);
+                            if (total == 0) break;
+                        }
+                    } else {
+                        const int64_t v = scalar_sum ? r.sum : (int64_t)r.count;
+                        s->rows.assign(1, (int32_t)(uint32_t)(uint64_t)v);
+                        buffer_t b(s->rows.data(), -1);
+                        b.cnt = 1;
+                        b.scalar = true;
+                        b.value64 = v;
+                        s->out.push_back(b);
+                    }
+                    for (buffer_t &b : s->out) s->parent->consume(&b);
+                    return;
+                }
+                if (s->eof.load()) return;
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+            }
+        });
+    }
+    // at_open: reset the accumulators
+    if (kind != BUILD && kind != SELECT_BUILD) {
+        if (!res) {
+            void *p = nullptr;
+            gpu(mgv_malloc(&p, sizeof(mgv_result)));
+            res = (mgv_result *)p;
+        }
+        gpu(mgv_result_clear(res, 1, nullptr));
+    }
+    out_upper = 0;
+    gpu(mgv_stream_synchronize(nullptr));
+    opened = true;
+}
+
+void d_operator_t::consume(buffer_t *v, mgv_stream_t s) {
+    if (!opened) open();
+    last_stream = s;
+    const uint64_t n = v->count();
+    if (n == 0) return;
+    const mgv_launch L = conf.to_launch();
+    int32_t *const *c = v->cols;
+    const int nc = v->ncols;
+    switch (kind) {
+        case SELECT_SUM:
+            gpu(mgv_select_sum(c[0], nc > 1 ? c[1] : c[0], n, bound, res, L, s));
+            break;
+        case SELECT_MAT: {
+            if (out_upper + n > out_cap) {   // grow the output (rare: doubling)
+                gpu(mgv_stream_synchronize(s));
+                uint64_t cap = out_cap ? out_cap : (1u << 20);
+                while (cap < out_upper + n) cap *= 2;
+                void *p = nullptr;
+                gpu(mgv_malloc(&p, cap * sizeof(int32_t)));
+                if (out) {
+                    gpu(mgv_memcpy(p, out, out_cap * sizeof(int32_t)));
+                    gpu(mgv_free(out));
+                }
+                out = (int32_t *)p;
+                out_cap = cap;
+            }
+            out_upper += n;
+            gpu(mgv_select_materialize(c[0], nc > 1 ? c[1] : c[0], n, bound, out, out_cap, res, L, s));
+            break;
+        }
+        case PROBE:
+            gpu(mgv_hashjoin_probe(table->slots, table->nbuckets, nullptr, 0, c[0], nullptr, n, 0, 0, res, L, s));
+            break;
+        case SELECT_PROBE: {
+            const int32_t *keys = nc > 1 ? c[1] : c[0];
+            gpu(mgv_hashjoin_probe(table->slots, table->nbuckets, c[0], bound, keys, nullptr, n, 0, 0, res, L, s));
+            break;
+        }
+        case BUILD:
+            gpu(mgv_hashjoin_build(table->slots, table->nbuckets, c[0], nc > 1 ? c[1] : nullptr,
+                                   nc > 1 ? MGV_PAYLOAD_COLUMN : MGV_PAYLOAD_NONE, nullptr, 0, n, 0, nullptr, L, s));
+            break;
+        case SELECT_BUILD: {
+            const int32_t *keys = nc > 1 ? c[1] : c[0];
+            gpu(mgv_hashjoin_build(table->slots, table->nbuckets, keys, nc > 2 ? c[2] : nullptr,
+                                   nc > 2 ? MGV_PAYLOAD_COLUMN : MGV_PAYLOAD_NONE, c[0], bound, n, 0, nullptr, L, s));
+            break;
+        }
+        default:
+            throw mgv::error(MGV_E_INVALID, "d_operator_t::consume: operator is not the head of a chain");
+    }
+}
+
+void d_operator_t::close() {
+    if (!opened) return;
+    gpu(mgv_stream_synchronize(last_stream));
+    gpu(mgv_device_synchronize());
+    if (sink) {
+        // at_close: hand the result (and the selected rows) to the host
+        if (kind == SELECT_MAT) {
+            mgv_result r;
+            gpu(mgv_memcpy(&r, res, sizeof(r)));
+            const uint64_t n = r.count < out_cap ? r.count : out_cap;
+            sink->rows.resize(n);
+            if (n) gpu(mgv_memcpy(sink->rows.data(), out, n * sizeof(int32_t)));
+        }
+        gpu(mgv_channel_emit(sink->channel, 0, res, 1, nullptr));
+        gpu(mgv_stream_synchronize(nullptr));
+        // after_close: join the catcher, close the host parent
+        if (sink->catcher.joinable()) sink->catcher.join();
+        sink->parent->close();
+    }
+    opened = false;
+}
+
+d_operator_t::~d_operator_t() {
+    mgv_free(res);
+    mgv_free(out);
+}
+
+void d_operator_t::destroy(d_operator_t *op) {
+    if (!op) return;
+    std::visit([](auto *p) { delete p; }, op->op);
+    delete op;
+}
+
+// ------------------------------------------------------------------ h_operator_t
+
+void h_operator_t::open() {
+    std::visit([](auto *p) { p->open(); }, op);
+}
+
+void h_operator_t::consume(buffer_t *b) {
+    std::visit([b](auto *p) { p->consume(b); }, op);
+}
+
+void h_operator_t::close() {
+    std::visit([](auto *p) { p->close(); }, op);
+}
+
+void h_operator_t::destroy(h_operator_t *op) {
+    if (!op) return;
+    std::visit([](auto *p) { delete p; }, op->op);
+    delete op;
+}
+
+// ------------------------------------------------------------------ generator
+
+generator::generator(h_operator_t *parent_, int32_t *src, size_t N_)
+    : parent(parent_), cols{src}, N(N_), vector_rows(buffer_t::capacity()), device(-1) {}
+
+generator::generator(h_operator_t *parent_, std::vector<int32_t *> cols_, size_t N_, uint32_t vector_rows_, int dev)
+    : parent(parent_), cols(std::move(cols_)), N(N_), vector_rows(vector_rows_), device(dev) {
+    if (cols.empty() || cols.size() > (size_t)buffer_t::kMaxCols)
+        throw mgv::error(MGV_E_INVALID, "generator: 1 to 4 columns");
+    if (vector_rows == 0 || vector_rows % (4 * WARPSIZE) || vector_rows > buffer_t::capacity())
+        throw mgv::error(MGV_E_INVALID, "generator: vector rows must be a multiple of 256 up to the buffer capacity");
+}
+
+void generator::open() { parent->open(); }
+
+void generator::consume(buffer_t *) { throw mgv::error(MGV_E_INVALID, "generator::consume: a generator has no input"); }
+
+void generator::close() {
+    size_t off = 0;
+    while (off < N) {
+        buffer_t *b = buffer_manager<int32_t>::get_buffer();
+        const uint32_t m = (uint32_t)std::min<size_t>(N - off, vector_rows);
+        int32_t *c[buffer_t::kMaxCols];
+        for (size_t i = 0; i < cols.size(); ++i) c[i] = cols[i] + off;
+        b->set_columns(c, (int)cols.size());     // the vector aliases the columns (generators.cu:71-72)
+        b->cnt = m;
+        b->device = device;
+        parent->consume(b);
+        off += m;
+    }
+    parent->close();
+}
+
+// ------------------------------------------------------------------ materializer
+
+materializer::materializer(int32_t *dst_) : dst(dst_) {}
+
+void materializer::consume(buffer_t *data) {
+    std::lock_guard<std::mutex> lk(mu);
+    const uint32_t n = data->count();
+    if (dst && n) {
+        if (data->device >= 0) gpu(mgv_memcpy(dst + size, data->data, (size_t)n * sizeof(int32_t)));
+        else memcpy(dst + size, data->data, (size_t)n * sizeof(int32_t));
+    }
+    size += n;
+    if (data->scalar) {
+        result64 += data->value64;
+        results += 1;
+    }
+    // buffers of the manager go back to it (materializer.cu:34); the host
+    // buffers a gpu_to_cpu hands over stay owned by that gpu_to_cpu
+    if (buffer_manager<int32_t>::owns(data)) buffer_manager<int32_t>::release_buffer(data);
+}
+
+// ------------------------------------------------------------------ consumer
+
+consumer::consumer(p_operator_t parent_, const launch_conf &conf_) : parent(parent_), conf(conf_) {}
+
+consumer::~consumer() {
+    for (int i = 0; i < 2; ++i) {
+        mgv_event_destroy(copied[i]);
+        mgv_event_destroy(used[i]);
+    }
+    mgv_stream_destroy(strm);
+    mgv_stream_destroy(copy_strm);
+}
+
+void consumer::open() {
+    gpu(mgv_stream_create(&strm));
+    gpu(mgv_stream_create(&copy_strm));
+    for (int i = 0; i < 2; ++i) {
+        gpu(mgv_event_create(&copied[i]));
+        gpu(mgv_event_create(&used[i]));
+        gpu(mgv_event_record(used[i], strm));
+    }
+    if (parent.d) parent.d->open();
+    else parent.h->open();
+}
+
+void consumer::consume(buffer_t *b) {
+    if (!parent.d) {   // host parent: pass through (exchange.cu:286-289)
+        parent.h->consume(b);
+        return;
+    }
+    if (b->device >= 0) {   // resident vector: launch on it directly
+        parent.d->consume(b, strm);
+        buffer_manager<int32_t>::release_buffer(b);
+        ++nvec;
+        return;
+    }
+    // host vector: copy into the next staging buffer (double buffered), then launch
+    const int i = (int)(nvec & 1);
+    if (!stage[i]) stage[i] = buffer_manager<int32_t>::h_get_buffer(0);
+    const uint32_t n = b->count();
+    const uint32_t pitch = (n + 255) / 256 * 256;
+    if ((uint64_t)pitch * b->ncols > buffer_t::capacity())
+        throw mgv::error(MGV_E_CAPACITY, "consumer: vector does not fit a device buffer");
+    gpu(mgv_stream_wait_event(copy_strm, used[i]));
+    int32_t *c[buffer_t::kMaxCols];
+    for (int k = 0; k < b->ncols; ++k) {
+        c[k] = stage[i]->cols[0] + (size_t)k * pitch;
+        gpu(mgv_memcpy_async(c[k], b->cols[k], (size_t)n * sizeof(int32_t), copy_strm));
+    }
+    gpu(mgv_event_record(copied[i], copy_strm));
+    buffer_t v = *stage[i];
+    v.set_columns(c, b->ncols);
+    v.cnt = n;
+    v.device = 0;
+    gpu(mgv_stream_wait_event(strm, copied[i]));
+    parent.d->consume(&v, strm);
+    gpu(mgv_event_record(used[i], strm));
+    // the host vector may be reused once its copy is done
+    gpu(mgv_event_synchronize(copied[i]));
+    buffer_manager<int32_t>::release_buffer(b);
+    ++nvec;
+}
+
+void consumer::close() {
+    gpu(mgv_stream_synchronize(strm));
+    if (parent.d) parent.d->close();
+    else parent.h->close();
+    for (int i = 0; i < 2; ++i) {
+        if (stage[i]) buffer_manager<int32_t>::release_buffer(stage[i]);
+        stage[i] = nullptr;
+    }
+}
+
+// ------------------------------------------------------------------ exchange
+
+exchange::exchange(const std::vector<p_operator_t> &parents, const std::vector<launch_conf> &parent_conf) {
+    if (parents.size() != parent_conf.size()) throw mgv::error(MGV_E_INVALID, "exchange: parents/confs mismatch");
+    for (size_t i = 0; i < parents.size(); ++i) consumers.emplace_back(new consumer(parents[i], parent_conf[i]));
+    for (auto &c : consumers) firers.emplace_back(&exchange::fire, this, c.get());
+}
+
+exchange::~exchange() {
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        remaining_producers = 0;
+    }
+    cv.notify_all();
+    for (auto &t : firers)
+        if (t.joinable()) t.join();
+}
+
+void exchange::fire(consumer *c) {
+    try {
+        c->open();
+        for (;;) {
+            buffer_t *b = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return !ready.empty() || remaining_producers <= 0; });
+                if (ready.empty()) break;
+                b = ready.back();   // LIFO like the reference's ready_pool (exchange.cu:98)
+                ready.pop_back();
+            }
+            c->consume(b);
+        }
+        c->close();
+    } catch (...) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!failure) failure = std::current_exception();
+        remaining_producers = 0;
+        cv.notify_all();
+    }
+}
+
+void exchange::consume(buffer_t *b) {
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        ready.push_back(b);
+    }
+    cv.notify_one();
+}
+
+void exchange::close() {
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        --remaining_producers;
+    }
+    cv.notify_all();
+    for (auto &t : firers)
+        if (t.joinable()) t.join();
+    if (failure) std::rethrow_exception(failure);
+}

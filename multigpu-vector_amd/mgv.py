@@ -72,6 +72,26 @@ class DeviceProps(C.Structure):
 
 VP = C.c_void_p
 U64 = C.c_uint64
+
+
+MGV_Q_SELECT_SUM = 1
+MGV_Q_PROBE_SUM = 2
+MGV_Q_PROBE_GROUPBY = 3
+MGV_Q_GROUPBY = 4
+MGV_Q_BUILD = 5
+
+
+class Query(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32), ("bound", C.c_int32), ("pred", C.c_void_p), ("key", C.c_void_p),
+        ("val", C.c_void_p), ("n", C.c_uint64), ("slots", C.c_void_p), ("nbuckets", C.c_uint32),
+        ("ngroups", C.c_uint32), ("gcnt", C.c_void_p), ("gsum", C.c_void_p), ("res", C.c_void_p),
+        ("payload_mode", C.c_int32), ("flags", C.c_uint32), ("row_base", C.c_int64),
+    ]
+
+
+class WorkItem(C.Structure):
+    _fields_ = [("query", C.c_uint32), ("rows", C.c_uint32), ("lo", C.c_uint64)]
 U32 = C.c_uint32
 I32 = C.c_int32
 I64 = C.c_int64
@@ -128,7 +148,26 @@ SIGNATURES = {
     "mgv_hashjoin_probe_groupby_partitioned": [VP, U32, VP, I32, VP, VP, U64, VP, VP, U32, U32, VP, U64, Launch,
                                                VP],
     "mgv_debug_xcc_histogram": [VP, U32, VP],
+    "mgv_batch_plan": [VP, U32, U32, VP, U64, C.POINTER(U64)],
+    "mgv_batch_run": [VP, VP, U64, VP, Launch, VP],
+    "mgv_pool_create": [U32, U32, C.POINTER(VP)],
+    "mgv_pool_destroy": [VP],
+    "mgv_pool_acquire": [VP, C.POINTER(I32), VP],
+    "mgv_pool_release": [VP, I32, VP],
+    "mgv_pool_register_producer": [VP, VP],
+    "mgv_pool_unregister_producer": [VP, VP],
+    "mgv_pool_buffer": [VP, I32, C.POINTER(VP), C.POINTER(VP)],
+    "mgv_pool_info": [VP, C.POINTER(U32), C.POINTER(U32)],
+    "mgv_pool_stats": [VP, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), VP],
+    "mgv_pool_stress": [VP, U32, U32, VP, VP],
+    "mgv_select_to_buffer": [VP, VP, U64, I32, VP, I32, VP],
+    "mgv_channel_create": [U32, C.POINTER(VP)],
+    "mgv_channel_destroy": [VP],
+    "mgv_channel_emit": [VP, U32, VP, U32, VP],
+    "mgv_channel_poll": [VP, U32, VP, C.POINTER(U32)],
 }
+POOL_END = -1
+POOL_RETRY = -2
 _RESTYPE = {"mgv_last_error_string": C.c_char_p}
 
 _lib: Optional[C.CDLL] = None
@@ -150,6 +189,46 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         fn.restype = _RESTYPE.get(name, C.c_int)
     _lib = lib
     return lib
+
+
+# ------------------------------------------------------------ libmgv_ops.so
+
+OPS_PATH = os.path.join(HERE, "lib", "libmgv_ops.so")
+OPS_SIGNATURES = {
+    "mgv_ops_init": [U32, U32, U32],
+    "mgv_ops_shutdown": [],
+    "mgv_ops_last_error": [],
+    "mgv_ops_sum_select": [VP, VP, U64, I32, U32, C.c_int, U32, U32, VP, C.POINTER(I64)],
+    "mgv_ops_select": [VP, VP, U64, I32, C.c_int, U32, C.c_int, U32, U32, VP, U64, C.POINTER(U64)],
+    "mgv_ops_count_join": [VP, VP, U64, VP, VP, U64, I32, U32, C.c_int, U32, U32, VP, C.POINTER(U64)],
+    "mgv_ops_run_plain": [VP, U32, U32, Launch, VP],
+}
+_ops: Optional[C.CDLL] = None
+
+
+def load_ops(path: str = OPS_PATH) -> C.CDLL:
+    """Load libmgv_ops.so (the C++ operator layer); raises if it is missing."""
+    global _ops
+    if _ops is not None:
+        return _ops
+    load()
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"libmgv_ops.so not built at {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    for name, args in OPS_SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.argtypes = args
+        fn.restype = C.c_char_p if name == "mgv_ops_last_error" else C.c_int
+    _ops = lib
+    return lib
+
+
+def ops_call(name: str, *args) -> None:
+    rc = getattr(load_ops(), name)(*args)
+    if rc != 0:
+        raise MgvError(rc, _ops.mgv_ops_last_error().decode())
 
 
 def check(rc: int) -> None:

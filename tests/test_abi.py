@@ -63,3 +63,21 @@ def test_calls_fail_cleanly_without_init_or_gpu():
     if rc == 0:
         lib.mgv_free(p)
     assert isinstance(lib.mgv_last_error_string(), bytes)
+
+
+def test_ops_library_exports_every_declared_symbol():
+    lib = mgv.load_ops()
+    names = declared_functions(os.path.join(ROOT, "include", "mgv_ops.h"))
+    assert len(names) >= 7
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert not [n for n in names if n not in mgv.OPS_SIGNATURES]
+
+
+def test_ops_calls_fail_cleanly_before_init():
+    mgv.load_ops()
+    s = C.c_int64()
+    with pytest.raises(mgv.MgvError) as e:
+        mgv.ops_call("mgv_ops_sum_select", None, None, 10, 5, 4096, 0, 1, 256, None, C.byref(s))
+    assert e.value.code == mgv.E_NOT_INIT
+    assert "mgv_ops_init" in mgv.load_ops().mgv_ops_last_error().decode()

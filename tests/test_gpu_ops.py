@@ -1,0 +1,163 @@
+"""GPU tests of the C++ operator layer (libmgv_ops.so): the reference's plans
+(generator -> exchange -> select / hashjoin / aggregation -> gpu_to_cpu ->
+exchange -> materializer, src/main9.cu:150-367) built from the drop-in
+operators, with device-resident and host-resident columns, compared with the
+oracle; and the plain per-vector path of a query batch compared with the
+fused batch kernel."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import mgv
+
+pytestmark = pytest.mark.gpu
+
+INT_MAX = 2**31 - 1
+
+
+@pytest.fixture(scope="module")
+def ops(gpu):
+    mgv.load_ops()
+    mgv.ops_call("mgv_ops_init", 8, 2, 4)
+    yield mgv
+    mgv.ops_call("mgv_ops_shutdown")
+
+
+def dev(a):
+    return mgv.device_array(np.ascontiguousarray(a, dtype=np.int32))
+
+
+def host_ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def cols(where, *arrays):
+    """Column pointers in device memory (where=0) or host memory (where=-1);
+    the returned keep-alive list holds the buffers."""
+    if where == 0:
+        bufs = [dev(a) for a in arrays]
+        return [b.vp for b in bufs], bufs
+    arrs = [np.ascontiguousarray(a, dtype=np.int32) for a in arrays]
+    return [host_ptr(a) for a in arrs], arrs
+
+
+@pytest.mark.parametrize("where", [0, -1])
+@pytest.mark.parametrize("n,vector_rows,block", [(1 << 20, 4096, 256), (300001, 65536, 1024), (1000, 256, 64)])
+def test_sum_select_plan(ops, orc, where, n, vector_rows, block):
+    a = orc.gen_bits(n, orc.seed(1, 0), 0, 20)
+    b = orc.gen_bits(n, orc.seed(1, 1), 0, 31)
+    bound = (1 << 30) - 1
+    (pb, pa), keep = cols(where, b, a)
+    dst = np.zeros(4, np.int32)
+    s = C.c_int64()
+    mgv.ops_call("mgv_ops_sum_select", pb, pa, n, bound, vector_rows, where, 64, block, host_ptr(dst), C.byref(s))
+    cnt, want, want32 = orc.select_sum(b, a, bound)
+    assert s.value == want
+    assert dst[0] == want32          # the int32 value the reference's aggregation delivers
+    del keep
+
+
+def test_sum_select_single_column_and_empty(ops, orc):
+    n = 70000
+    x = orc.gen_range(n, orc.seed(1, 2), 0, 10000) + 1
+    (px,), keep = cols(0, x)
+    s = C.c_int64()
+    mgv.ops_call("mgv_ops_sum_select", px, px, n, 5000, 4096, 0, 32, 256, None, C.byref(s))
+    assert s.value == orc.select_sum(x, x, 5000)[1]
+    mgv.ops_call("mgv_ops_sum_select", px, px, 0, 5000, 4096, 0, 32, 256, None, C.byref(s))
+    assert s.value == 0
+    del keep
+
+
+@pytest.mark.parametrize("where", [0, -1])
+@pytest.mark.parametrize("apply_select", [1, 0])
+def test_select_plan_multiset(ops, orc, where, apply_select):
+    n = 200003
+    a = orc.gen_bits(n, orc.seed(1, 0), 1, 20)
+    b = orc.gen_bits(n, orc.seed(1, 1), 1, 31)
+    bound = (1 << 29) - 1
+    (pb, pa), keep = cols(where, b, a)
+    dst = np.full(n, -7, np.int32)
+    cnt = C.c_uint64()
+    mgv.ops_call("mgv_ops_select", pb, pa, n, bound, apply_select, 8192, where, 64, 256, host_ptr(dst), n,
+                 C.byref(cnt))
+    want = a[b <= bound] if apply_select else a
+    assert cnt.value == len(want)
+    assert np.array_equal(np.sort(dst[:cnt.value]), np.sort(want))
+    assert np.all(dst[cnt.value:] == -7)
+    del keep
+
+
+def test_select_plan_rejects_small_dst(ops):
+    x = dev(np.arange(1024, dtype=np.int32))
+    dst = np.zeros(10, np.int32)
+    cnt = C.c_uint64()
+    with pytest.raises(mgv.MgvError) as e:
+        mgv.ops_call("mgv_ops_select", x.vp, x.vp, 1024, 5, 1, 256, 0, 8, 256, host_ptr(dst), 10, C.byref(cnt))
+    assert e.value.code == mgv.E_CAPACITY
+
+
+@pytest.mark.parametrize("where", [0, -1])
+@pytest.mark.parametrize("with_select", [False, True])
+def test_count_join_plan(ops, orc, where, with_select):
+    nb, npr = 1 << 16, 1 << 20
+    R = orc.gen_perm(nb, nb, orc.seed(2, 0), 5)
+    L = orc.gen_range(npr, orc.seed(2, 2), 5, 2 * nb)
+    pr = orc.gen_bits(nb, orc.seed(3, 0), 5, 31)
+    pl = orc.gen_bits(npr, orc.seed(3, 1), 5, 31)
+    bound = (1 << 30) - 1
+    (dR, dL, dpr, dpl), keep = cols(where, R, L, pr, pl)
+    cnt = C.c_uint64()
+    dst = np.zeros(2, np.int32)
+    mgv.ops_call("mgv_ops_count_join", dR, dpr if with_select else None, nb, dL, dpl if with_select else None, npr,
+                 bound, 16384, where, 128, 256, host_ptr(dst), C.byref(cnt))
+    if with_select:
+        m = pr <= bound
+        t = orc.table(np.ascontiguousarray(R[m]))
+        want = orc.probe(t, L, pred=pl, bound=bound)
+    else:
+        t = orc.table(R)
+        want = orc.probe(t, L)
+    orc.free(t)
+    assert cnt.value == want["count"]
+    assert dst[0] == want["count32"]
+    del keep
+
+
+def test_count_join_plan_with_duplicate_keys(ops, orc):
+    """Self-join closed form: sum over values v <= 5000 of hist[v]^2 (main7.cu:62-68)."""
+    n = 100000
+    x = orc.gen_range(n, orc.seed(2, 5), 0, 10000) + 1
+    (px,), keep = cols(0, x)
+    pred = dev(x)
+    cnt = C.c_uint64()
+    mgv.ops_call("mgv_ops_count_join", px, pred.vp, n, px, None, n, 5000, 4096, 0, 64, 256, None, C.byref(cnt))
+    assert cnt.value == orc.selfjoin_count(x, 5000, 10000)
+    del keep
+
+
+def test_plain_path_equals_fused_batch(ops, orc):
+    """Each query of a batch gives the same result through one launch per
+    vector (the plain path) and through the fused persistent kernel."""
+    from test_gpu_batch_pool import make_queries, run_batch
+    qs = make_queries(orc, 10, seed_cfg=21)
+    fused, _, _ = run_batch(mgv, qs, 1024, 1, 256)
+    nq = len(qs)
+    tables = [mgv.HashTable(q["B"], 0.5) for q in qs]
+    res = mgv.ResultBuffer(2 * nq)
+    qarr = (mgv.Query * (2 * nq))()
+    for i, (q, t) in enumerate(zip(qs, tables)):
+        d = q["d"]
+        qarr[i] = mgv.Query(mgv.MGV_Q_BUILD, 0, None, d["sk"].ptr, d["sw"].ptr, q["B"], t.buf.ptr, t.nbuckets, 0,
+                            None, None, res.ptr + 64 * i, mgv.MGV_PAYLOAD_COLUMN, 0, 0)
+        qarr[nq + i] = mgv.Query(mgv.MGV_Q_PROBE_SUM, q["bound"], d["c"].ptr, d["rk"].ptr, d["rv"].ptr, q["P"],
+                                 t.buf.ptr, t.nbuckets, 0, None, None, res.ptr + 64 * (nq + i), 0, 0, 0)
+    mgv.ops_call("mgv_ops_run_plain", qarr, 2 * nq, 1024, mgv.launch(64, 256), None)
+    mgv.sync()
+    for i, q in enumerate(qs):
+        r = res.read(nq + i)
+        assert res.read(i).errors == 0
+        assert (r.count, r.sum) == (fused[i].count, fused[i].sum) == (q["want"]["count"], q["want"]["sum"])

@@ -264,7 +264,7 @@ def test_select_probe_sum_matches_oracle(gpu, orc):
     for bound in (INT_MIN, 2**31 // 20, 2**30, INT_MAX):
         want = orc.probe(t, rk, rv, pred=c, bound=bound)
         for compact in (0, 1):
-            for block in (64, 512):
+            for block in (64, 1024):
                 r = probe(gpu, ht, dk, dv, nr, pred=dc, bound=bound, launch=mgv.launch(0, block, compact))
                 assert (r.count, r.sum) == (want["count"], want["sum"]), (bound, compact, block)
     orc.free(t)
