@@ -296,6 +296,110 @@ def leg_c1(args, st):
     return out
 
 
+# ------------------------------------------------------------------ C4 batch
+
+def tuned(workload: str):
+    """Autotuned launch parameters (multigpu-vector_amd/tuned.json), if any."""
+    p = os.path.join(ROOT, "multigpu-vector_amd", "tuned.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+
+
+def leg_c4(args, st):
+    """256 select -> join -> SUM queries: one hipGraph (clear + one persistent
+    launch for the builds + one for the probes) against the plain path (one
+    launch per vector per query).  Both are timed by wall clock from issue to
+    completion; each query's (COUNT, SUM) must agree between the two paths,
+    and with the oracle for the small queries."""
+    import workloads
+    t = tuned("c4") or {}
+    V = int(t.get("vector_rows", 16384))
+    blk = int(t.get("block", 256))
+    compact = int(t.get("compact", 1))
+    lf = float(t.get("load_factor", 0.5))
+    shapes = workloads.c4_shapes(256)
+    batch = workloads.JoinBatch(shapes, lf, st.vp)
+    st.sync()
+    log(f"C4: {len(shapes)} queries, {batch.r_rows} probe rows, {batch.s_rows} build rows, V={V} block={blk} "
+        f"compact={compact} LF={lf}")
+    L = mgv.launch(0, blk, compact)
+    g = batch.capture_fused(V, L, st.vp)
+
+    def run_graph():
+        mgv.call("mgv_graph_launch", g, st.vp)
+
+    def timed(fn, reps):
+        walls, evs = [], []
+        for _ in range(reps):
+            a, b = mgv.Event(), mgv.Event()
+            st.sync()
+            t0 = time.perf_counter()
+            a.record(st.vp)
+            fn()
+            b.record(st.vp)
+            st.sync()
+            walls.append(time.perf_counter() - t0)
+            evs.append(a.elapsed_ms(b))
+            a.close()
+            b.close()
+        return walls, evs
+
+    timed(run_graph, 2)
+    bw, be = timed(run_graph, args.c4_reps)
+    fused = batch.results()
+    assert int(fused["build_errors"].sum()) == 0
+    plain_out = {}
+    plain_res = None
+    for pv in (4096, 65536):
+        Lp = mgv.launch(0, 256, 1)
+        timed(lambda: batch.issue_plain(pv, Lp, st.vp), 1)
+        pw, pe = timed(lambda: batch.issue_plain(pv, Lp, st.vp), max(1, args.c4_reps // 2))
+        r = batch.results()
+        same = bool(np.array_equal(r["count"], fused["count"]) and np.array_equal(r["sum"], fused["sum"]))
+        plain_out[str(pv)] = {"wall_ms": min(pw) * 1e3, "gpu_ms": min(pe), "queries_per_s": len(shapes) / min(pw),
+                              "launches": 2 * (batch.items(pv)), "results_equal_fused": same}
+        log(f"C4 plain V={pv}: {min(pw) * 1e3:.2f} ms wall, equal to fused: {same}")
+        plain_res = r
+    best_plain = max(plain_out.values(), key=lambda x: x["queries_per_s"])
+    # oracle check of the smaller queries (host copies of their columns)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py
+    orc = oracle_py.Oracle()
+    checked, ok = 0, True
+    for i, x in enumerate(shapes):
+        if x.P > (1 << 17):
+            continue
+        h = batch.host_columns(i)
+        tb = orc.table(h["sk"], h["sw"])
+        want = orc.probe(tb, h["k"], h["v"], pred=h["c"], bound=x.bound)
+        orc.free(tb)
+        ok &= (int(fused["count"][i]), int(fused["sum"][i])) == (want["count"], want["sum"])
+        checked += 1
+    mgv.call("mgv_graph_destroy", g)
+    fused_qps = len(shapes) / min(bw)
+    algo = batch.algorithmic_bytes()
+    out = {
+        "queries": len(shapes), "probe_rows": batch.r_rows, "build_rows": batch.s_rows,
+        "tuned": {"vector_rows": V, "block": blk, "compact": compact, "load_factor": lf, "from_file": bool(t)},
+        "batched": {"wall_ms": min(bw) * 1e3, "gpu_ms": min(be), "queries_per_s": fused_qps,
+                    "algorithmic_gbps": algo / (min(be) / 1e3) / 1e9,
+                    "frac": algo / (min(be) / 1e3) / 1e9 / HBM_PEAK_GBPS, "work_items": batch.items(V)},
+        "plain": plain_out,
+        "speedup_vs_best_plain": fused_qps / best_plain["queries_per_s"],
+        "parity": {"fused_equals_plain": all(v["results_equal_fused"] for v in plain_out.values()),
+                   "oracle_checked_queries": checked, "oracle_equal": bool(ok)},
+        "total_count": int(fused["count"].sum()), "total_sum": int(fused["sum"].sum()),
+    }
+    log(f"C4 batched {min(bw) * 1e3:.2f} ms wall ({fused_qps:.0f} q/s), x{out['speedup_vs_best_plain']:.2f} vs best "
+        f"plain; oracle {checked} queries ok={ok}")
+    del plain_res
+    batch.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,7 +408,8 @@ def main():
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--load-factor", type=float, default=0.5)
     ap.add_argument("--fanout", type=int, default=-1, help="1 = direct probe, >1 = partitioned, -1 = sweep")
-    ap.add_argument("--legs", default="c1")
+    ap.add_argument("--legs", default="c1,c4")
+    ap.add_argument("--c4-reps", type=int, default=5)
     ap.add_argument("--cpu", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=1 << 25)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -325,6 +430,9 @@ def main():
     legs = [x for x in args.legs.split(",") if x]
     if "c1" in legs:
         extra["c1"] = leg_c1(args, st)
+    if "c4" in legs:
+        mgv.load_ops()
+        extra["c4"] = leg_c4(args, st)
 
     line = {
         "metric": "probe Grows/s & %HBM peak; batched vs plain queries/s at bit-exact, p99 lat",

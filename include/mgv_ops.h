@@ -66,6 +66,72 @@ int mgv_ops_count_join(const int32_t *R, const int32_t *pred_r, uint64_t rn, con
 int mgv_ops_run_plain(const mgv_query *queries, uint32_t nqueries, uint32_t vector_rows, mgv_launch launch,
                       mgv_stream_t s);
 
+/* ------------------------------------------------------------ serving loop */
+
+/* A request of the serving loop over resident columns (device pointers). */
+#define MGV_SERVE_JOIN_SUM 1   /* build S(bkey, bval) then COUNT, SUM(val + w) of R JOIN S WHERE pred <= bound */
+#define MGV_SERVE_GROUPBY 2    /* GROUP BY key (in [0, ngroups)): COUNT, SUM(val) */
+
+typedef struct mgv_serve_request {
+    uint32_t kind;
+    int32_t bound;
+    const int32_t *pred;       /* JOIN_SUM: filter column (NULL = no filter) */
+    const int32_t *key;        /* probe keys / group keys */
+    const int32_t *val;
+    uint64_t n;                /* probe / group-by rows */
+    const int32_t *bkey;       /* JOIN_SUM: build keys (unique) and payloads */
+    const int32_t *bval;
+    uint64_t bn;
+    uint32_t ngroups;          /* GROUPBY */
+    uint32_t reserved;
+    double arrival_s;          /* arrival time, seconds after the start of the run */
+} mgv_serve_request;
+
+typedef struct mgv_serve_result {
+    uint64_t count;            /* join matches / grouped rows */
+    int64_t sum;
+    uint64_t digest;           /* GROUPBY: digest of the per-group (COUNT, SUM) arrays; JOIN_SUM: 0 */
+    uint64_t footprint;        /* HBM bytes the query held while it ran */
+    double admit_s;            /* admitted (memory reserved, launched) */
+    double done_s;             /* result on the host */
+    double latency_s;          /* done_s - arrival_s */
+    uint32_t batch;            /* index of the batch that ran it */
+    uint32_t errors;           /* build rejects / out-of-range group keys */
+} mgv_serve_result;
+
+typedef struct mgv_serve_config {
+    uint64_t budget_bytes;     /* HBM budget for hash tables + group arrays */
+    uint32_t max_inflight;     /* batches on the GPU at once (one stream each), 1..8 */
+    uint32_t max_batch;        /* queries per batch, 1..1024 */
+    uint32_t vector_rows;      /* work-item size of the fused kernel */
+    uint32_t reserved;
+    double load_factor;        /* hash tables */
+    mgv_launch launch;
+} mgv_serve_config;
+
+typedef struct mgv_serve_stats {
+    double elapsed_s;          /* first arrival to last result */
+    double queries_per_s;
+    double p50_s, p99_s, max_s, mean_s;
+    uint32_t batches;
+    uint32_t max_inflight_seen;
+    uint64_t peak_bytes;       /* peak reserved HBM of the budget */
+    double mean_batch;         /* queries per batch */
+} mgv_serve_stats;
+
+/* Continuous-batching request loop.  Requests (sorted by arrival_s) are
+ * released at their arrival time; a waiting request is admitted when its
+ * footprint (hash table at load_factor, or group arrays, + result) fits the
+ * free HBM budget and a stream slot is free; the admitted requests form one
+ * batch: one persistent launch for its builds and one for its probes and
+ * group-bys (the fused kernel of mgv_batch_run), after which the results are
+ * copied to pinned host memory.  Up to max_inflight batches overlap on the
+ * GPU; a batch's memory is returned when its results are on the host.
+ * Admission is FIFO (a request that does not fit blocks the ones behind it).
+ * Fails with MGV_E_CAPACITY for a request larger than the whole budget. */
+int mgv_serve_run(const mgv_serve_request *requests, uint32_t n, const mgv_serve_config *config,
+                  mgv_serve_result *results, mgv_serve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

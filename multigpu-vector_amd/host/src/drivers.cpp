@@ -8,6 +8,11 @@
 #include "../../../include/mgv_ops.h"
 #include "mgv/operators.hpp"
 
+namespace mgvhost {
+void serve_run(const mgv_serve_request *requests, uint32_t n, const mgv_serve_config *config,
+               mgv_serve_result *results, mgv_serve_stats *stats);
+}
+
 namespace {
 
 thread_local std::string g_err;
@@ -234,6 +239,11 @@ int mgv_ops_run_plain(const mgv_query *queries, uint32_t nqueries, uint32_t vect
             }
         }
     });
+}
+
+int mgv_serve_run(const mgv_serve_request *requests, uint32_t n, const mgv_serve_config *config,
+                  mgv_serve_result *results, mgv_serve_stats *stats) {
+    return guarded("mgv_serve_run", [&] { mgvhost::serve_run(requests, n, config, results, stats); });
 }
 
 }  // extern "C"

@@ -193,6 +193,44 @@ def load(path: str = LIB_PATH) -> C.CDLL:
 
 # ------------------------------------------------------------ libmgv_ops.so
 
+MGV_SERVE_JOIN_SUM = 1
+MGV_SERVE_GROUPBY = 2
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32), ("bound", C.c_int32), ("pred", C.c_void_p), ("key", C.c_void_p),
+        ("val", C.c_void_p), ("n", C.c_uint64), ("bkey", C.c_void_p), ("bval", C.c_void_p), ("bn", C.c_uint64),
+        ("ngroups", C.c_uint32), ("reserved", C.c_uint32), ("arrival_s", C.c_double),
+    ]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [
+        ("count", C.c_uint64), ("sum", C.c_int64), ("digest", C.c_uint64), ("footprint", C.c_uint64),
+        ("admit_s", C.c_double), ("done_s", C.c_double), ("latency_s", C.c_double), ("batch", C.c_uint32),
+        ("errors", C.c_uint32),
+    ]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [
+        ("budget_bytes", C.c_uint64), ("max_inflight", C.c_uint32), ("max_batch", C.c_uint32),
+        ("vector_rows", C.c_uint32), ("reserved", C.c_uint32), ("load_factor", C.c_double), ("launch", Launch),
+    ]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [
+        ("elapsed_s", C.c_double), ("queries_per_s", C.c_double), ("p50_s", C.c_double), ("p99_s", C.c_double),
+        ("max_s", C.c_double), ("mean_s", C.c_double), ("batches", C.c_uint32), ("max_inflight_seen", C.c_uint32),
+        ("peak_bytes", C.c_uint64), ("mean_batch", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 OPS_PATH = os.path.join(HERE, "lib", "libmgv_ops.so")
 OPS_SIGNATURES = {
     "mgv_ops_init": [U32, U32, U32],
@@ -202,6 +240,7 @@ OPS_SIGNATURES = {
     "mgv_ops_select": [VP, VP, U64, I32, C.c_int, U32, C.c_int, U32, U32, VP, U64, C.POINTER(U64)],
     "mgv_ops_count_join": [VP, VP, U64, VP, VP, U64, I32, U32, C.c_int, U32, U32, VP, C.POINTER(U64)],
     "mgv_ops_run_plain": [VP, U32, U32, Launch, VP],
+    "mgv_serve_run": [VP, U32, VP, VP, VP],
 }
 _ops: Optional[C.CDLL] = None
 

@@ -81,3 +81,10 @@ def test_ops_calls_fail_cleanly_before_init():
         mgv.ops_call("mgv_ops_sum_select", None, None, 10, 5, 4096, 0, 1, 256, None, C.byref(s))
     assert e.value.code == mgv.E_NOT_INIT
     assert "mgv_ops_init" in mgv.load_ops().mgv_ops_last_error().decode()
+
+
+def test_serve_struct_sizes_match_header():
+    # sizeof of mgv_serve_request / _result / _config / _stats as gcc lays out include/mgv_ops.h
+    assert (C.sizeof(mgv.ServeRequest), C.sizeof(mgv.ServeResult), C.sizeof(mgv.ServeConfig),
+            C.sizeof(mgv.ServeStats)) == (80, 64, 48, 72)
+    assert C.sizeof(mgv.Query) == 96 and C.sizeof(mgv.WorkItem) == 16
