@@ -94,6 +94,11 @@ void d_operator_t::resolve() {
         ++i;
     } else if (sel) {
         kind = SELECT_MAT;
+    } else if (is(0, (gpu_to_cpu_t *)nullptr)) {
+        // gpu_to_cpu fed directly: every row goes to the host
+        // (create_select_all_pipeline, main9.cu:176-200)
+        kind = SELECT_MAT;
+        bound = INT32_MAX;
     }
     if (kind == NONE || !is(i, (gpu_to_cpu_t *)nullptr) || i + 1 != chain.size())
         throw mgv::error(MGV_E_INVALID, "unsupported device operator chain");
@@ -408,6 +413,10 @@ exchange::~exchange() {
     cv.notify_all();
     for (auto &t : firers)
         if (t.joinable()) t.join();
+    // vectors left by a failed run go back to the manager
+    for (buffer_t *b : ready)
+        if (buffer_manager<int32_t>::owns(b)) buffer_manager<int32_t>::release_buffer(b);
+    ready.clear();
 }
 
 void exchange::fire(consumer *c) {
@@ -429,6 +438,11 @@ void exchange::fire(consumer *c) {
         std::lock_guard<std::mutex> lk(mu);
         if (!failure) failure = std::current_exception();
         remaining_producers = 0;
+        // hand queued vectors back so a producer blocked on a free buffer
+        // wakes up and sees the failure
+        for (buffer_t *b : ready)
+            if (buffer_manager<int32_t>::owns(b)) buffer_manager<int32_t>::release_buffer(b);
+        ready.clear();
         cv.notify_all();
     }
 }
@@ -436,6 +450,12 @@ void exchange::fire(consumer *c) {
 void exchange::consume(buffer_t *b) {
     {
         std::lock_guard<std::mutex> lk(mu);
+        if (failure) {
+            // a consumer failed: stop the producer instead of queueing
+            // vectors nobody will release
+            if (buffer_manager<int32_t>::owns(b)) buffer_manager<int32_t>::release_buffer(b);
+            std::rethrow_exception(failure);
+        }
         ready.push_back(b);
     }
     cv.notify_one();

@@ -161,3 +161,18 @@ def test_plain_path_equals_fused_batch(ops, orc):
         r = res.read(nq + i)
         assert res.read(i).errors == 0
         assert (r.count, r.sum) == (fused[i].count, fused[i].sum) == (q["want"]["count"], q["want"]["sum"])
+
+
+def test_plan_failure_stops_the_generator(ops):
+    """A consumer whose launch fails (misaligned column, rejected on the host
+    before any kernel runs) ends the plan with that error instead of leaving
+    the generator blocked on a free buffer; the manager gets its buffers back."""
+    x = dev(np.arange(1 << 17, dtype=np.int32))
+    bad = C.c_void_p(x.ptr + 4)
+    s = C.c_int64()
+    for _ in range(2):
+        with pytest.raises(mgv.MgvError) as e:
+            mgv.ops_call("mgv_ops_sum_select", bad, bad, (1 << 17) - 64, 5, 4096, 0, 8, 256, None, C.byref(s))
+        assert e.value.code == mgv.E_ALIGN
+    mgv.ops_call("mgv_ops_sum_select", x.vp, x.vp, 1 << 17, 5, 4096, 0, 8, 256, None, C.byref(s))
+    assert s.value == sum(range(6))
