@@ -95,7 +95,10 @@ def leg_c2(args, st):
     ht = mgv.HashTable(ns, args.load_factor)
     res = mgv.ResultBuffer()
     st.sync()
-    L = mgv.launch(0, args.block, -1)
+    t2 = tuned("c2") or {}
+    block = args.block or int(t2.get("block", 256))
+    coop = int(t2.get("coop", 0)) if args.coop < 0 else args.coop
+    L = mgv.Launch(0, block, -1, coop)
 
     # build (timed: clear excluded)
     build_ms = []
@@ -128,7 +131,10 @@ def leg_c2(args, st):
                                 res.vp, L, st.vp)
 
     # choose the probe path: direct (fan-out 1) or partitioned
-    candidates = [args.fanout] if args.fanout >= 0 else [1, 64, 128, 256]
+    if args.fanout >= 0:
+        candidates = [args.fanout]
+    else:
+        candidates = sorted({1, 64, 128, 256} | ({int(t2["fanout"])} if "fanout" in t2 else set()))
     sweep = {}
     for f in candidates:
         fn = make_probe(f)
@@ -192,6 +198,8 @@ def leg_c2(args, st):
                        "mgv_hashjoin_probe_partitioned: k_part_prep+k_part_count+k_part_scan+k_part_scatter+"
                        "k_part_probe<PProbeSum<false,true,false>,256> (events around the whole call)"),
             "fanout": fanout,
+            "block": block,
+            "coop_probe": coop,
             "algorithmic_bytes_per_launch": algo_bytes,
         },
         "build": {"ms": build_t, "grows_per_s": gb(ns) / (build_t / 1e3),
@@ -260,6 +268,8 @@ def cpu_model():
 def leg_c1(args, st):
     out = {}
     bound = (1 << 30) - 1
+    t1 = tuned("c1") or {}
+    block = int(t1.get("block", 256))
     for n, reps_in_graph in ((1 << 24, 100), (1 << 28, 1)):
         a = gen_bits(n, mgv.seed(1, 0), 0, 20, st.vp)
         b = gen_bits(n, mgv.seed(1, 1), 0, 31, st.vp)
@@ -267,7 +277,7 @@ def leg_c1(args, st):
         st.sync()
         best = None
         for compact in (0, 1):
-            L = mgv.launch(0, args.block, compact)
+            L = mgv.launch(0, block, compact)
             # capture reps_in_graph back-to-back scans in one graph
             mgv.call("mgv_graph_begin_capture", st.vp)
             for i in range(reps_in_graph):
@@ -287,7 +297,7 @@ def leg_c1(args, st):
             gbps = n * 8 / (kms / 1e3) / 1e9
             log(f"C1 n={n} compact={compact}: {kms:.4f} ms/scan, {gbps:.0f} GB/s, count {r.count} sum {r.sum}")
             if best is None or kms < best["ms"]:
-                best = {"ms": kms, "gbps": gbps, "frac": gbps / HBM_PEAK_GBPS, "compact": compact,
+                best = {"ms": kms, "gbps": gbps, "frac": gbps / HBM_PEAK_GBPS, "compact": compact, "block": block,
                         "count": r.count, "sum": r.sum, "grows_per_s": n / (kms / 1e3) / 1e9}
         key = "c1_2^24_graph100" if n == 1 << 24 else "c1_2^28"
         out[key] = best
@@ -400,15 +410,79 @@ def leg_c4(args, st):
     return out
 
 
+# ------------------------------------------------------------------ C5 serving
+
+def leg_c5(args, st):
+    """Serving: requests over 64 GiB of resident int32 columns (70 %
+    select -> join -> SUM over build tables of 2^16..2^26 rows, 30 % GROUP BY
+    over 2^16 Zipf(0.8) groups), admitted against a 16 GiB HBM budget.
+    Three runs of the same requests: saturated (all arrive at t=0) to find
+    the capacity, Poisson arrivals at 70 % of that capacity (the reported
+    latencies), and one request at a time (the alone reference): every
+    request's (COUNT, SUM, group digest) must be identical in all three."""
+    import workloads
+    t0 = time.perf_counter()
+    tables = workloads.ServeTables(build_logs=range(16, 27), probe_cap=1 << 29,
+                                   group_rows=(1 << 30,) * 5 + (1 << 29,), ngroups=1 << 16, zipf_s=0.8,
+                                   stream=st.vp)
+    st.sync()
+    log(f"C5: {tables.bytes / 2**30:.1f} GiB of resident columns generated in {time.perf_counter() - t0:.1f} s")
+    n = args.c5_requests
+    budget = 16 << 30
+    V = 65536
+    reqs0, desc = tables.requests(n, 0.0, seed=0xC5, arrivals=False)
+    sat, ss = workloads.serve(reqs0, n, budget, max_inflight=4, max_batch=64, vector_rows=V)
+    log(f"C5 saturated: {ss.queries_per_s:.1f} q/s, {ss.batches} batches")
+    alone, sa = workloads.serve(reqs0, n, budget, max_inflight=1, max_batch=1, vector_rows=V)
+    log(f"C5 one at a time: {sa.queries_per_s:.1f} q/s")
+    rate = 0.7 * ss.queries_per_s
+    reqs, _ = tables.requests(n, rate, seed=0xC5)
+    mix, sm = workloads.serve(reqs, n, budget, max_inflight=4, max_batch=64, vector_rows=V)
+    log(f"C5 Poisson {rate:.1f} q/s: p50 {sm.p50_s * 1e3:.2f} ms p99 {sm.p99_s * 1e3:.2f} ms, "
+        f"{sm.queries_per_s:.1f} q/s, mean batch {sm.mean_batch:.1f}")
+
+    def trip(r):
+        return [(r[i].count, r[i].sum, r[i].digest) for i in range(n)]
+
+    same = trip(mix) == trip(alone) == trip(sat)
+    errs = sum(mix[i].errors for i in range(n))
+    kinds = [d["kind"] for d in desc]
+    lat_join = sorted(mix[i].latency_s for i in range(n) if kinds[i] == "join")
+    lat_grp = sorted(mix[i].latency_s for i in range(n) if kinds[i] == "groupby")
+
+    def pct(v, p):
+        return v[min(len(v) - 1, max(0, int(np.ceil(p * len(v))) - 1))] * 1e3 if v else None
+
+    out = {
+        "requests": n, "join_requests": kinds.count("join"), "groupby_requests": kinds.count("groupby"),
+        "resident_column_gib": tables.bytes / 2**30, "hbm_budget_gib": budget / 2**30, "vector_rows": V,
+        "max_inflight": 4, "max_batch": 64,
+        "poisson": {"rate_qps": rate, "queries_per_s": sm.queries_per_s, "p50_ms": sm.p50_s * 1e3,
+                    "p99_ms": sm.p99_s * 1e3, "max_ms": sm.max_s * 1e3, "mean_ms": sm.mean_s * 1e3,
+                    "batches": sm.batches, "mean_batch": sm.mean_batch, "max_inflight_seen": sm.max_inflight_seen,
+                    "peak_budget_gib": sm.peak_bytes / 2**30,
+                    "join_p50_ms": pct(lat_join, 0.5), "join_p99_ms": pct(lat_join, 0.99),
+                    "groupby_p50_ms": pct(lat_grp, 0.5), "groupby_p99_ms": pct(lat_grp, 0.99)},
+        "saturated": {"queries_per_s": ss.queries_per_s, "p50_ms": ss.p50_s * 1e3, "p99_ms": ss.p99_s * 1e3,
+                      "mean_batch": ss.mean_batch},
+        "one_at_a_time": {"queries_per_s": sa.queries_per_s, "p50_ms": sa.p50_s * 1e3, "p99_ms": sa.p99_s * 1e3},
+        "parity": {"mix_equals_alone_equals_saturated": bool(same), "build_or_group_errors": int(errs)},
+    }
+    tables.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--block", type=int, default=0, help="probe workgroup size (0 = tuned, else 256)")
+    ap.add_argument("--coop", type=int, default=-1, help="4-lane cooperative probe (-1 = tuned)")
     ap.add_argument("--load-factor", type=float, default=0.5)
     ap.add_argument("--fanout", type=int, default=-1, help="1 = direct probe, >1 = partitioned, -1 = sweep")
-    ap.add_argument("--legs", default="c1,c4")
+    ap.add_argument("--legs", default="c1,c4,c5")
+    ap.add_argument("--c5-requests", type=int, default=300)
     ap.add_argument("--c4-reps", type=int, default=5)
     ap.add_argument("--cpu", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=1 << 25)
@@ -433,6 +507,9 @@ def main():
     if "c4" in legs:
         mgv.load_ops()
         extra["c4"] = leg_c4(args, st)
+    if "c5" in legs:
+        mgv.load_ops()
+        extra["c5"] = leg_c5(args, st)
 
     line = {
         "metric": "probe Grows/s & %HBM peak; batched vs plain queries/s at bit-exact, p99 lat",
@@ -450,7 +527,7 @@ def main():
         "config": {"workload": "C2 hash join + aggregate: probe 2^28 R rows against a resident open-addressing "
                                "table of 2^24 S rows at load factor 0.5; COUNT(*), SUM(R.v+S.w) int64",
                    "build_rows": 1 << 24, "probe_rows": 1 << 28, "load_factor": args.load_factor,
-                   "block": args.block, "fanout": c2["roofline"]["fanout"], "parallelism": "1 GPU"},
+                   "block": c2["roofline"]["block"], "fanout": c2["roofline"]["fanout"], "parallelism": "1 GPU"},
         "roofline": c2["roofline"],
         "cpu_baseline": c2.get("cpu_baseline"),
         "sample_parity": c2.get("sample_parity"),

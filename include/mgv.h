@@ -364,6 +364,19 @@ int mgv_select_materialize(const int32_t *pred, const int32_t *val, uint64_t n, 
 int mgv_groupby(const int32_t *keys, const int32_t *vals, uint64_t n, uint64_t *gcnt, int64_t *gsum,
                 uint32_t ngroups, mgv_result *res, mgv_launch launch, mgv_stream_t s);
 
+/* GROUP BY over many groups (more than fit an LDS table): the rows are
+ * scattered into key ranges of 8192 groups (range = key >> 13), then each
+ * range is aggregated in an LDS table by persistent blocks that flush one
+ * pair of atomics per touched group.  Same results and error counting as
+ * mgv_groupby (which it calls for ngroups <= 4096); n < 2^31; ngroups <=
+ * 2^22.  The workspace (device, size from mgv_groupby_workspace_size) holds
+ * the partitioned copy of keys and vals (8 bytes per row).  No reference
+ * counterpart (aggregation.cu is scalar only). */
+int mgv_groupby_workspace_size(uint64_t n, uint32_t ngroups, uint64_t *bytes);
+int mgv_groupby_partitioned(const int32_t *keys, const int32_t *vals, uint64_t n, uint64_t *gcnt, int64_t *gsum,
+                            uint32_t ngroups, void *ws, uint64_t ws_bytes, mgv_result *res, mgv_launch launch,
+                            mgv_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

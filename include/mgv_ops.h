@@ -124,8 +124,9 @@ typedef struct mgv_serve_stats {
  * footprint (hash table at load_factor, or group arrays, + result) fits the
  * free HBM budget and a stream slot is free; the admitted requests form one
  * batch: one persistent launch for its builds and one for its probes and
- * group-bys (the fused kernel of mgv_batch_run), after which the results are
- * copied to pinned host memory.  Up to max_inflight batches overlap on the
+ * group-bys (the fused kernel of mgv_batch_run; a group-by over more than
+ * 4096 groups runs as mgv_groupby_partitioned, its workspace counted in its
+ * footprint), after which the results are copied to pinned host memory.  Up to max_inflight batches overlap on the
  * GPU; a batch's memory is returned when its results are on the host.
  * Admission is FIFO (a request that does not fit blocks the ones behind it).
  * Fails with MGV_E_CAPACITY for a request larger than the whole budget. */

@@ -1,0 +1,52 @@
+// mgv_part.hpp -- layout and entry points of the row partitioning shared by
+// the partitioned probe (mgv_partition.hip) and the partitioned group-by
+// (mgv_groupby.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/mgv.h"
+
+namespace mgv {
+
+constexpr int kPartBlock = 256;
+constexpr int kTile = kPartBlock * 16;     // rows staged in LDS per scatter step
+constexpr uint32_t kMaxFanout = 512;
+constexpr uint32_t kPartAlign = 64;        // partition starts aligned to 64 rows
+constexpr uint32_t kItemRows = 4096;       // probe work item (one wave)
+constexpr uint32_t kMaxPartBlocks = 4096;
+
+struct PartLayout {
+    uint32_t F, nblk, ncols;
+    uint64_t rows_per_blk, n, cap;   // cap = rows per partitioned column
+    uint32_t *hist;                  // [nblk][F]
+    unsigned long long *ptotal;      // [F]
+    unsigned long long *pstart;      // [F + 1]
+    unsigned long long *pcursor;     // [F]
+    uint32_t *xhead;                 // [8]
+    int32_t *col[3];                 // partitioned k, v, rowid
+};
+
+// Input of the partitioning kernels.
+struct PartIn {
+    const int32_t *pred;
+    int32_t bound;
+    const int32_t *key;
+    const int32_t *val;   // may be null (partitioned as zeros)
+    int64_t row_base;
+    uint32_t want_rowid;  // third partitioned column = row ids
+    uint32_t rshift;      // key-range partitioning: partition = key >> rshift ...
+    uint32_t rlimit;      // ... for keys in [0, rlimit); 0 = hash partitioning
+    mgv_result *res;      // rows dropped for an out-of-range key -> res->errors
+};
+
+// Plans the workspace of a partitioning of n rows into F partitions with
+// ncols partitioned columns; with L == nullptr only *need is set.
+int plan_layout(uint64_t n, uint32_t F, uint32_t ncols, void *ws, uint64_t ws_bytes, PartLayout *L, uint64_t *need);
+
+// Launches prep, count, scan and scatter on stream s.
+int partition_rows(const PartLayout &L, const PartIn &in, hipStream_t s);
+
+}  // namespace mgv

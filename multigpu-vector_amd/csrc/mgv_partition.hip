@@ -17,16 +17,10 @@
 #include <hip/hip_runtime.h>
 
 #include "mgv_internal.hpp"
+#include "mgv_part.hpp"
 #include "mgv_pipes.hpp"
 
 namespace mgv {
-
-constexpr int kPartBlock = 256;
-constexpr int kTile = kPartBlock * 16;     // rows staged in LDS per scatter step
-constexpr uint32_t kMaxFanout = 512;
-constexpr uint32_t kPartAlign = 64;        // partition starts aligned to 64 rows
-constexpr uint32_t kItemRows = 4096;       // probe work item (one wave)
-constexpr uint32_t kMaxPartBlocks = 4096;
 
 // XCC (XCD) id of the executing wave: HW_REG_XCC_ID (hwreg 20), bits [3:0].
 // Used only to choose which partitions a block drains first.
@@ -38,20 +32,26 @@ __device__ __forceinline__ uint32_t part_of(int32_t key, uint32_t F) {
     return (uint32_t)(((uint64_t)fmix32((uint32_t)key) * F) >> 32);
 }
 
-struct PartLayout {
-    uint32_t F, nblk, ncols;
-    uint64_t rows_per_blk, n, cap;   // cap = rows per partitioned column
-    uint32_t *hist;                  // [nblk][F]
-    unsigned long long *ptotal;      // [F]
-    unsigned long long *pstart;      // [F + 1]
-    unsigned long long *pcursor;     // [F]
-    uint32_t *xhead;                 // [8]
-    int32_t *col[3];                 // partitioned k, v, rowid
-};
+// Partition of a key: hash partitioning (joins) or key ranges (group-by:
+// partition = key >> rshift, keys outside [0, rlimit) are dropped).
+__device__ __forceinline__ uint32_t part_key(const PartIn &in, int32_t key, uint32_t F) {
+    return in.rlimit ? ((uint32_t)key >> in.rshift) : part_of(key, F);
+}
+
+// Bit j set when row j of a quad has a key the partitioning accepts.
+__device__ __forceinline__ uint32_t key_mask(const PartIn &in, const v4i &k) {
+    if (!in.rlimit) return 0xfu;
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) m |= (uint32_t)((uint32_t)q_get(k, j) < in.rlimit) << j;
+    return m;
+}
+
+
 
 __host__ __device__ static inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
-static int plan_layout(uint64_t n, uint32_t F, uint32_t ncols, void *ws, uint64_t ws_bytes, PartLayout *L,
+int plan_layout(uint64_t n, uint32_t F, uint32_t ncols, void *ws, uint64_t ws_bytes, PartLayout *L,
                        uint64_t *need) {
     if (F < 2 || F > kMaxFanout || (F & (F - 1))) return fail(MGV_E_INVALID, "fan-out %u not a power of two in [2,512]", F);
     if (ncols < 1 || ncols > 3) return fail(MGV_E_INVALID, "partitioned columns %u not in [1,3]", ncols);
@@ -96,15 +96,7 @@ static int plan_layout(uint64_t n, uint32_t F, uint32_t ncols, void *ws, uint64_
     return MGV_OK;
 }
 
-// Input of the partitioning kernels.
-struct PartIn {
-    const int32_t *pred;
-    int32_t bound;
-    const int32_t *key;
-    const int32_t *val;   // may be null (partitioned as zeros)
-    int64_t row_base;
-    uint32_t want_rowid;  // third partitioned column = row ids
-};
+
 
 __global__ void k_part_prep(PartLayout L) {
     for (uint32_t p = threadIdx.x; p < L.F; p += blockDim.x) L.ptotal[p] = 0;
@@ -138,21 +130,29 @@ __device__ __forceinline__ uint32_t quad_mask(const PartIn &in, uint64_t i0, uin
 
 __global__ __launch_bounds__(kPartBlock) void k_part_count(PartLayout L, PartIn in) {
     __shared__ uint32_t h[kMaxFanout];
+    __shared__ uint32_t s_err;
     for (uint32_t p = threadIdx.x; p < L.F; p += kPartBlock) h[p] = 0;
+    if (threadIdx.x == 0) s_err = 0;
     __syncthreads();
     const uint64_t lo = (uint64_t)blockIdx.x * L.rows_per_blk;
     const uint64_t hi = min(L.n, lo + L.rows_per_blk);
+    uint32_t errs = 0;
     for (uint64_t i0 = lo + 4ull * threadIdx.x; i0 < hi; i0 += 4ull * kPartBlock) {
         v4i c;
         uint32_t m = quad_mask(in, i0, hi, &c);
         if (m) {
             v4i k = load_quad(in.key, i0, hi);
+            const uint32_t ok = m & key_mask(in, k);
+            errs += (uint32_t)__builtin_popcount(m & ~ok);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                if ((m >> j) & 1u) atomicAdd(&h[part_of(q_get(k, j), L.F)], 1u);
+                if ((ok >> j) & 1u) atomicAdd(&h[part_key(in, q_get(k, j), L.F)], 1u);
         }
     }
+    if (errs) atomicAdd(&s_err, errs);
     __syncthreads();
+    if (threadIdx.x == 0 && s_err && in.res)
+        atomicAdd((unsigned long long *)&in.res->errors, (unsigned long long)s_err);
     for (uint32_t p = threadIdx.x; p < L.F; p += kPartBlock) {
         uint32_t c = h[p];
         L.hist[(uint64_t)blockIdx.x * L.F + p] = c;
@@ -234,6 +234,7 @@ __global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartI
             nv[q] = v4i{0, 0, 0, 0};
             if (nm[q]) {
                 nk[q] = load_quad(in.key, i0, hi);
+                nm[q] &= key_mask(in, nk[q]);
                 if (in.val) nv[q] = load_quad(in.val, i0, hi);
             }
         }
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartI
         for (int q = 0; q < Q; ++q)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                rank[q][j] = ((cm[q] >> j) & 1u) ? atomicAdd(&tcnt[part_of(q_get(ck[q], j), F)], 1u) : 0u;
+                rank[q][j] = ((cm[q] >> j) & 1u) ? atomicAdd(&tcnt[part_key(in, q_get(ck[q], j), F)], 1u) : 0u;
         __syncthreads();
         // 2. exclusive scan of the tile counts (<= 2 partitions per thread)
         const uint32_t per = (F + kPartBlock - 1) / kPartBlock;
@@ -282,7 +283,7 @@ __global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartI
             for (int j = 0; j < 4; ++j) {
                 if ((cm[q] >> j) & 1u) {
                     const int32_t k = q_get(ck[q], j);
-                    const uint32_t pos = toff[part_of(k, F)] + rank[q][j];
+                    const uint32_t pos = toff[part_key(in, k, F)] + rank[q][j];
                     sk[pos] = k;
                     sv[pos] = q_get(cv[q], j);
                     if (ROWID)
@@ -295,7 +296,7 @@ __global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartI
         // 4. write the runs
         for (uint32_t i = t; i < total; i += kPartBlock) {
             const int32_t k = sk[i];
-            const uint32_t p = part_of(k, F);
+            const uint32_t p = part_key(in, k, F);
             const unsigned long long dst = cursor[p] + (i - toff[p]);
             L.col[0][dst] = k;
             if (L.ncols > 1) L.col[1][dst] = sv[i];
@@ -405,7 +406,7 @@ static int part_probe(const typename Pipe::Params &prm, const PartLayout &L, con
     }
 }
 
-static int partition_rows(const PartLayout &L, const PartIn &in, hipStream_t s) {
+int partition_rows(const PartLayout &L, const PartIn &in, hipStream_t s) {
     for (const int32_t *p : {in.pred, in.key, in.val})
         if (p && !aligned16(p)) return fail(MGV_E_ALIGN, "partitioned column is not 16-byte aligned");
     hipLaunchKernelGGL(k_part_prep, dim3(1), dim3(kMaxFanout), 0, s, L);
@@ -450,7 +451,7 @@ int mgv_hashjoin_probe_partitioned(const uint64_t *slots, uint32_t nbuckets, con
     int rc = plan_layout(n, fanout, digest ? 3 : 2, ws, ws_bytes, &L, nullptr);
     if (rc) return rc;
     hipStream_t s = stream_of(st);
-    PartIn in = {pred, bound, keys, vals, row_base, digest ? 1u : 0u};
+    PartIn in = {pred, bound, keys, vals, row_base, digest ? 1u : 0u, 0u, 0u, res};
     if ((rc = partition_rows(L, in, s))) return rc;
     if (digest) {
         PProbeSum<false, true, true>::Params p = {0, slots, nbuckets, res};
@@ -477,7 +478,7 @@ int mgv_hashjoin_probe_groupby_partitioned(const uint64_t *slots, uint32_t nbuck
     int rc = plan_layout(n, fanout, 2, ws, ws_bytes, &L, nullptr);
     if (rc) return rc;
     hipStream_t s = stream_of(st);
-    PartIn in = {pred, bound, keys, vals, 0, 0u};
+    PartIn in = {pred, bound, keys, vals, 0, 0u, 0u, 0u, nullptr};
     if ((rc = partition_rows(L, in, s))) return rc;
     if (ngroups <= 1024) {
         PProbeGroup<false, true, 1024>::Params p = {0, slots, nbuckets, gcnt, gsum, ngroups};

@@ -79,9 +79,10 @@ private:
 
 struct Admitted {
     uint32_t req;
-    uint64_t off, bytes;     // arena reservation: table or group arrays
+    uint64_t off, bytes;     // arena reservation: table, or group arrays [+ group-by workspace]
     uint32_t nbuckets;
     uint64_t group_stage;    // offset (in groups) into the slot's pinned group staging
+    uint64_t ws;             // group-by workspace bytes (0: group-by runs in the fused launch)
 };
 
 struct Slot {
@@ -132,15 +133,21 @@ struct Server {
         mgv_free(d_arena);
     }
 
-    uint64_t footprint(const mgv_serve_request &q, uint32_t *nbuckets) const {
+    // HBM bytes a request holds while it runs: its hash table, or its group
+    // arrays plus, for many groups, the partitioned group-by's workspace.
+    uint64_t footprint(const mgv_serve_request &q, uint32_t *nbuckets, uint64_t *ws = nullptr) const {
         if (q.kind == MGV_SERVE_JOIN_SUM) {
             uint64_t bytes = 0;
             uint32_t nb = 0;
             gpu(mgv_ht_plan(q.bn, cfg.load_factor, &nb, &bytes));
             if (nbuckets) *nbuckets = nb;
+            if (ws) *ws = 0;
             return bytes;
         }
-        return (uint64_t)q.ngroups * 16;
+        uint64_t w = 0;
+        gpu(mgv_groupby_workspace_size(q.n, q.ngroups, &w));
+        if (ws) *ws = w;
+        return align_up((uint64_t)q.ngroups * 16, 256) + w;
     }
 
     uint64_t items_of(uint64_t rows) const { return (rows + cfg.vector_rows - 1) / cfg.vector_rows; }
@@ -227,6 +234,7 @@ struct Server {
                 m.slots = b.slots;
                 m.nbuckets = a.nbuckets;
             } else {
+                if (a.ws) continue;   // many groups: partitioned group-by, issued below
                 m.kind = MGV_Q_GROUPBY;
                 m.ngroups = r.ngroups;
                 m.gcnt = (uint64_t *)(d_arena + a.off);
@@ -235,12 +243,12 @@ struct Server {
             m.key = r.key;
             m.val = r.val;
             m.n = r.n;
-            m.res = s.d_res + s.nmain;
+            m.res = s.d_res + i;   // result slot i belongs to the batch's i-th request
             mq[s.nmain++] = m;
         }
         uint64_t nb = 0, nm = 0;
         if (s.nbuild) gpu(mgv_batch_plan(bq, s.nbuild, cfg.vector_rows, items, items_cap, &nb));
-        gpu(mgv_batch_plan(mq, s.nmain, cfg.vector_rows, items + nb, items_cap - nb, &nm));
+        if (s.nmain) gpu(mgv_batch_plan(mq, s.nmain, cfg.vector_rows, items + nb, items_cap - nb, &nm));
         if (nb + nm > items_cap) throw mgv::error(MGV_E_CAPACITY, "serve: work-item staging too small");
         s.nbitems = nb;
         s.nmitems = nm;
@@ -258,7 +266,17 @@ struct Server {
         const mgv_query *dmq = dbq + cfg.max_batch;
         const mgv_work_item *di = (const mgv_work_item *)(s.d_desc + items_at);
         if (nb) gpu(mgv_batch_run(dbq, di, nb, s.d_counter, cfg.launch, s.stream));
-        gpu(mgv_batch_run(dmq, di + nb, nm, s.d_counter + 32, cfg.launch, s.stream));
+        if (nm) gpu(mgv_batch_run(dmq, di + nb, nm, s.d_counter + 32, cfg.launch, s.stream));
+        for (size_t i = 0; i < s.q.size(); ++i) {
+            const Admitted &a = s.q[i];
+            const mgv_serve_request &r = R[a.req];
+            if (r.kind != MGV_SERVE_GROUPBY || !a.ws) continue;
+            uint8_t *g = d_arena + a.off;
+            const uint64_t garr = align_up((uint64_t)r.ngroups * 16, 256);
+            gpu(mgv_groupby_partitioned(r.key, r.val, r.n, (uint64_t *)g, (int64_t *)(g + (uint64_t)r.ngroups * 8),
+                                        r.ngroups, g + garr, a.ws, s.d_res + i, mgv_launch{0, 1024, -1, 0},
+                                        s.stream));
+        }
         gpu(mgv_memcpy_async(s.h_res, s.d_res, 2 * (size_t)cfg.max_batch * sizeof(mgv_result), s.stream));
         for (size_t i = 0; i < s.q.size(); ++i) {
             const Admitted &a = s.q[i];
@@ -340,13 +358,14 @@ struct Server {
                     const uint32_t i = waiting.front();
                     const mgv_serve_request &r = R[i];
                     uint32_t nb = 0;
-                    const uint64_t bytes = footprint(r, &nb);
+                    uint64_t ws = 0;
+                    const uint64_t bytes = footprint(r, &nb, &ws);
                     const uint64_t it = items_of(r.n) + (r.kind == MGV_SERVE_JOIN_SUM ? items_of(r.bn) : 0);
                     const uint64_t gs = r.kind == MGV_SERVE_GROUPBY ? r.ngroups : 0;
                     if (items + it > items_cap || gstage + gs > s.group_cap) break;
                     uint64_t off = 0;
                     if (!arena.alloc(bytes, &off)) break;   // FIFO: wait for memory
-                    s.q.push_back(Admitted{i, off, bytes, nb, gstage});
+                    s.q.push_back(Admitted{i, off, bytes, nb, gstage, ws});
                     out[i].admit_s = now_s();
                     items += it;
                     gstage += gs;

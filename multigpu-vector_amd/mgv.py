@@ -165,6 +165,8 @@ SIGNATURES = {
     "mgv_channel_destroy": [VP],
     "mgv_channel_emit": [VP, U32, VP, U32, VP],
     "mgv_channel_poll": [VP, U32, VP, C.POINTER(U32)],
+    "mgv_groupby_workspace_size": [U64, U32, C.POINTER(U64)],
+    "mgv_groupby_partitioned": [VP, VP, U64, VP, VP, U32, VP, U64, VP, Launch, VP],
 }
 POOL_END = -1
 POOL_RETRY = -2

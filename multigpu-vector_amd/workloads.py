@@ -254,14 +254,16 @@ class ServeTables:
 
     def requests(self, n, rate_qps, seed=0xC5, join_frac=0.7, min_group_log=20, arrivals=True):
         """n requests with Poisson(rate) arrivals (or all at t=0); returns
-        (ctypes array of ServeRequest, list of dicts describing them)."""
-        rng = np.random.default_rng(seed)
+        (ctypes array of ServeRequest, list of dicts describing them).  The
+        queries depend only on `seed`, not on the arrival process."""
+        rng = np.random.default_rng(seed)               # the queries
+        rng_t = np.random.default_rng(seed ^ 0xA5A5A5)   # the arrival times
         arr = (mgv.ServeRequest * max(1, n))()
         desc = []
         t = 0.0
         for i in range(n):
             if arrivals and rate_qps > 0:
-                t += float(rng.exponential(1.0 / rate_qps))
+                t += float(rng_t.exponential(1.0 / rate_qps))
             if rng.random() < join_frac:
                 j = int(rng.integers(0, len(self.pairs)))
                 p = self.pairs[j]

@@ -90,3 +90,20 @@ def test_serve_rejects_unsorted_arrivals(tables):
     with pytest.raises(mgv.MgvError) as e:
         workloads.serve(reqs, 4, 256 << 20)
     assert e.value.code == mgv.E_INVALID
+
+
+def test_serve_many_groups_use_partitioned_groupby(gpu, orc):
+    """Group-bys over more than 4096 groups run as the partitioned group-by
+    inside the loop; results equal the oracle alone and in the mix."""
+    t = workloads.ServeTables(build_logs=range(8, 10), probe_cap=1 << 14, group_rows=(1 << 19, 5 << 16),
+                              ngroups=1 << 14, zipf_s=0.8)
+    try:
+        n = 16
+        reqs, desc = t.requests(n, rate_qps=50000.0, seed=11, join_frac=0.25, min_group_log=12)
+        mix, st = workloads.serve(reqs, n, 512 << 20, max_inflight=3, max_batch=8, vector_rows=4096)
+        alone, _ = workloads.serve(reqs, n, 512 << 20, max_inflight=1, max_batch=1, vector_rows=4096)
+        assert triples(mix, n) == triples(alone, n) == oracle_results(orc, t, desc)
+        big = [i for i, d in enumerate(desc) if d["kind"] == "groupby"]
+        assert big and all(mix[i].footprint > (1 << 14) * 16 for i in big)   # workspace counted
+    finally:
+        t.close()
