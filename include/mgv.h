@@ -124,6 +124,12 @@ typedef struct mgv_launch {
 /* Launch flag: the probe-sum pipelines use the wave-cooperative bucket scan
  * (4 lanes per row) instead of one row per lane. */
 #define MGV_LAUNCH_COOP_PROBE 1u
+/* Workgroup shape of the partitioning scatter (bits 8-11 of the flags) used
+ * by the partitioned probes and group-by: threads x rows per LDS tile. */
+#define MGV_SCATTER_256x4096 0u
+#define MGV_SCATTER_512x8192 1u
+#define MGV_SCATTER_1024x8192 2u
+#define MGV_LAUNCH_SCATTER(shape) ((uint32_t)(shape) << 8)
 
 /* ------------------------------------------------------------ datagen */
 
@@ -341,6 +347,22 @@ int mgv_hashjoin_probe_groupby_partitioned(const uint64_t *slots, uint32_t nbuck
 
 /* Diagnostic: out8[x] += number of `grid` one-wave blocks that ran on XCD x. */
 int mgv_debug_xcc_histogram(uint32_t *out8, uint32_t grid, mgv_stream_t s);
+
+/* Probe + aggregate through LDS-resident table slices (same results as
+ * mgv_hashjoin_probe without flags): the probe rows (optionally only those
+ * with pred <= bound) are hash-partitioned twice -- `fanout` partitions,
+ * then nbuckets / S partitions whose table slices of S <= 2048 buckets
+ * (128 KiB) fit in LDS -- and each slice is probed from LDS by one block.
+ * nbuckets must split into power-of-two slices of at most 2048 buckets (a
+ * power-of-two multiple of 2 * fanout slices, at most 2048 of them);
+ * n < 2^31.  The workspace (size from mgv_probe_lds_workspace_size) holds
+ * both partitionings (16 bytes per row).  Replaces hashjoin::consume_warp
+ * (src/operators/hashjoin.cu:135-182) fused with aggregation
+ * (src/operators/aggregation.cu:27-66). */
+int mgv_probe_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes);
+int mgv_hashjoin_probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
+                           const int32_t *keys, const int32_t *vals, uint64_t n, uint32_t fanout, void *ws,
+                           uint64_t ws_bytes, mgv_result *res, mgv_launch launch, mgv_stream_t s);
 
 /* ------------------------------------------------------------ select / aggregate */
 

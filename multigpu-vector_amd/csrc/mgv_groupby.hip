@@ -168,8 +168,8 @@ int mgv_groupby_partitioned(const int32_t *keys, const int32_t *vals, uint64_t n
     int rc = plan_layout(n, F, 2, (char *)ws + 256, ws_bytes - 256, &L, nullptr);
     if (rc) return rc;
     hipStream_t s = stream_of(st);
-    PartIn in = {nullptr, 0, keys, vals, 0, 0u, kGroupLog, ngroups, res};
-    if ((rc = partition_rows(L, in, s))) return rc;
+    PartIn in = {nullptr, 0, keys, vals, 0, 0u, kGroupLog, ngroups, res, nullptr};
+    if ((rc = partition_rows(L, in, s, scatter_shape(LC)))) return rc;
     uint32_t *next = (uint32_t *)ws;
     if ((rc = mgv_fill_u32(next, 1, 0, (mgv_stream_t)s))) return rc;
     static thread_local int cached = 0;

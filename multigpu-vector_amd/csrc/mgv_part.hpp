@@ -11,15 +11,17 @@
 
 namespace mgv {
 
-constexpr int kPartBlock = 256;
-constexpr int kTile = kPartBlock * 16;     // rows staged in LDS per scatter step
-constexpr uint32_t kMaxFanout = 512;
-constexpr uint32_t kPartAlign = 64;        // partition starts aligned to 64 rows
+constexpr int kPartBlock = 256;            // count kernel
+constexpr int kTileMax = 8192;             // largest scatter tile (rows per block are a multiple)
+constexpr uint32_t kMaxFanout = 512;        // partitioned probe / group-by
+constexpr uint32_t kMaxFanoutWide = 2048;   // second level of the two-level partitioning
+constexpr uint32_t kPartAlign = 64;         // partition starts aligned to 64 rows (default)
 constexpr uint32_t kItemRows = 4096;       // probe work item (one wave)
 constexpr uint32_t kMaxPartBlocks = 4096;
 
 struct PartLayout {
     uint32_t F, nblk, ncols;
+    uint32_t align;                  // partition starts are multiples of `align` rows
     uint64_t rows_per_blk, n, cap;   // cap = rows per partitioned column
     uint32_t *hist;                  // [nblk][F]
     unsigned long long *ptotal;      // [F]
@@ -40,13 +42,20 @@ struct PartIn {
     uint32_t rshift;      // key-range partitioning: partition = key >> rshift ...
     uint32_t rlimit;      // ... for keys in [0, rlimit); 0 = hash partitioning
     mgv_result *res;      // rows dropped for an out-of-range key -> res->errors
+    const unsigned long long *nlimit;   // optional device row count: rows >= *nlimit are ignored
 };
 
 // Plans the workspace of a partitioning of n rows into F partitions with
 // ncols partitioned columns; with L == nullptr only *need is set.
-int plan_layout(uint64_t n, uint32_t F, uint32_t ncols, void *ws, uint64_t ws_bytes, PartLayout *L, uint64_t *need);
+// F is a power of two in [2, 2048]; align (rows) a power of two <= 64.
+int plan_layout(uint64_t n, uint32_t F, uint32_t ncols, void *ws, uint64_t ws_bytes, PartLayout *L, uint64_t *need,
+                uint32_t align = kPartAlign);
 
-// Launches prep, count, scan and scatter on stream s.
-int partition_rows(const PartLayout &L, const PartIn &in, hipStream_t s);
+// Launches prep, count, scan and scatter on stream s; shape is one of the
+// MGV_SCATTER_* workgroup shapes of the scatter (0 = 256 threads x 4096 rows).
+int partition_rows(const PartLayout &L, const PartIn &in, hipStream_t s, uint32_t shape = 0);
+
+// The scatter shape requested by a launch descriptor (bits 8-11 of its flags).
+inline uint32_t scatter_shape(const mgv_launch &L) { return (L.reserved >> 8) & 0xfu; }
 
 }  // namespace mgv

@@ -52,16 +52,18 @@ __device__ __forceinline__ uint32_t key_mask(const PartIn &in, const v4i &k) {
 __host__ __device__ static inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
 int plan_layout(uint64_t n, uint32_t F, uint32_t ncols, void *ws, uint64_t ws_bytes, PartLayout *L,
-                       uint64_t *need) {
-    if (F < 2 || F > kMaxFanout || (F & (F - 1))) return fail(MGV_E_INVALID, "fan-out %u not a power of two in [2,512]", F);
+                uint64_t *need, uint32_t align) {
+    if (F < 2 || F > kMaxFanoutWide || (F & (F - 1)))
+        return fail(MGV_E_INVALID, "fan-out %u not a power of two in [2,%u]", F, kMaxFanoutWide);
+    if (align < 1 || align > 64 || (align & (align - 1))) return fail(MGV_E_INVALID, "partition alignment %u", align);
     if (ncols < 1 || ncols > 3) return fail(MGV_E_INVALID, "partitioned columns %u not in [1,3]", ncols);
     uint64_t nblk = (n + 16383) / 16384;
     if (nblk < 1) nblk = 1;
     if (nblk > kMaxPartBlocks) nblk = kMaxPartBlocks;
-    uint64_t rpb = align_up((n + nblk - 1) / nblk, kTile);
+    uint64_t rpb = align_up((n + nblk - 1) / nblk, kTileMax);
     nblk = (n + rpb - 1) / rpb;
     if (nblk < 1) nblk = 1;
-    const uint64_t cap = align_up(n + (uint64_t)F * kPartAlign, 64);
+    const uint64_t cap = align_up(n + (uint64_t)F * align, 64);
     uint64_t off = 0;
     auto take = [&](uint64_t bytes) {
         uint64_t o = off;
@@ -82,6 +84,7 @@ int plan_layout(uint64_t n, uint32_t F, uint32_t ncols, void *ws, uint64_t ws_by
                     (unsigned long long)off);
     char *b = (char *)ws;
     L->F = F;
+    L->align = align;
     L->nblk = (uint32_t)nblk;
     L->ncols = ncols;
     L->rows_per_blk = rpb;
@@ -128,14 +131,16 @@ __device__ __forceinline__ uint32_t quad_mask(const PartIn &in, uint64_t i0, uin
     return m;
 }
 
+template <uint32_t MAXF>
 __global__ __launch_bounds__(kPartBlock) void k_part_count(PartLayout L, PartIn in) {
-    __shared__ uint32_t h[kMaxFanout];
+    __shared__ uint32_t h[MAXF];
     __shared__ uint32_t s_err;
     for (uint32_t p = threadIdx.x; p < L.F; p += kPartBlock) h[p] = 0;
     if (threadIdx.x == 0) s_err = 0;
     __syncthreads();
     const uint64_t lo = (uint64_t)blockIdx.x * L.rows_per_blk;
-    const uint64_t hi = min(L.n, lo + L.rows_per_blk);
+    uint64_t hi = min(L.n, lo + L.rows_per_blk);
+    if (in.nlimit) hi = min(hi, (uint64_t)*in.nlimit);
     uint32_t errs = 0;
     for (uint64_t i0 = lo + 4ull * threadIdx.x; i0 < hi; i0 += 4ull * kPartBlock) {
         v4i c;
@@ -160,29 +165,42 @@ __global__ __launch_bounds__(kPartBlock) void k_part_count(PartLayout L, PartIn 
     }
 }
 
-// One block: aligned exclusive scan of the partition totals.
-__global__ __launch_bounds__(kMaxFanout) void k_part_scan(PartLayout L) {
-    __shared__ unsigned long long s[kMaxFanout];
+// One block of kScanThreads threads: exclusive scan of the partition totals,
+// each rounded up to the alignment; thread t owns E = F / kScanThreads
+// consecutive partitions (or one, for F <= kScanThreads).
+constexpr uint32_t kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_part_scan(PartLayout L) {
+    __shared__ unsigned long long s[kScanThreads];
     const uint32_t t = threadIdx.x;
-    unsigned long long v = (t < L.F) ? align_up(L.ptotal[t], kPartAlign) : 0;
-    s[t] = v;
+    const uint32_t E = L.F > kScanThreads ? L.F / kScanThreads : 1;
+    unsigned long long mine = 0;
+    for (uint32_t e = 0; e < E; ++e) {
+        const uint32_t p = t * E + e;
+        if (p < L.F) mine += align_up(L.ptotal[p], L.align);
+    }
+    s[t] = mine;
     __syncthreads();
-    for (uint32_t d = 1; d < kMaxFanout; d <<= 1) {
-        unsigned long long a = (t >= d) ? s[t - d] : 0;
+    for (uint32_t d = 1; d < kScanThreads; d <<= 1) {
+        const unsigned long long a = (t >= d) ? s[t - d] : 0;
         __syncthreads();
         s[t] += a;
         __syncthreads();
     }
-    if (t < L.F) {
-        unsigned long long start = s[t] - v;   // exclusive
-        L.pstart[t] = start;
-        L.pcursor[t] = start;
+    unsigned long long start = s[t] - mine;   // exclusive
+    for (uint32_t e = 0; e < E; ++e) {
+        const uint32_t p = t * E + e;
+        if (p < L.F) {
+            L.pstart[p] = start;
+            L.pcursor[p] = start;
+            start += align_up(L.ptotal[p], L.align);
+        }
     }
-    if (t == 0) L.pstart[L.F] = s[L.F - 1];
+    if (t == kScanThreads - 1) L.pstart[L.F] = s[t];
 }
 
-// Exclusive scan of one value per thread over a kPartBlock-thread block:
+// Exclusive scan of one value per thread over a BLOCK-thread block:
 // wave-level inclusive scan with ds_bpermute shuffles, then the wave totals.
+template <int BLOCK>
 __device__ __forceinline__ uint32_t scan_block_excl(uint32_t v, uint32_t *wtot) {
     const uint32_t l = lane_id(), w = wave_id();
     uint32_t x = v;
@@ -199,35 +217,37 @@ __device__ __forceinline__ uint32_t scan_block_excl(uint32_t v, uint32_t *wtot) 
     return before + x - v;
 }
 
-// LDS-staged scatter: each tile of kTile rows is counting-sorted by partition
+// LDS-staged scatter: each tile of TILE rows is counting-sorted by partition
 // in LDS, then written out so that consecutive lanes write consecutive slots
 // of a partition's run.  The next tile's rows are loaded before the current
 // tile is staged, so loads overlap the LDS work.
-template <bool ROWID>
-__global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartIn in) {
-    constexpr int Q = kTile / (4 * kPartBlock);   // quads per thread per tile
-    __shared__ unsigned long long cursor[kMaxFanout];
-    __shared__ uint32_t tcnt[kMaxFanout];
-    __shared__ uint32_t toff[kMaxFanout];
-    __shared__ uint32_t wtot[kPartBlock / kWave];
-    __shared__ int32_t sk[kTile];
-    __shared__ int32_t sv[kTile];
-    __shared__ int32_t sr[ROWID ? kTile : 1];
+template <bool ROWID, uint32_t MAXF, int BLOCK, int TILE>
+__global__ __launch_bounds__(BLOCK) void k_part_scatter(PartLayout L, PartIn in) {
+    constexpr int Q = TILE / (4 * BLOCK);   // quads per thread per tile
+    static_assert(Q >= 1 && TILE % (4 * BLOCK) == 0 && kTileMax % TILE == 0, "scatter shape");
+    __shared__ unsigned long long cursor[MAXF];
+    __shared__ uint32_t tcnt[MAXF];
+    __shared__ uint32_t toff[MAXF];
+    __shared__ uint32_t wtot[BLOCK / kWave];
+    __shared__ int32_t sk[TILE];
+    __shared__ int32_t sv[TILE];
+    __shared__ int32_t sr[ROWID ? TILE : 1];
     const uint32_t F = L.F;
     const uint32_t t = threadIdx.x;
-    for (uint32_t p = t; p < F; p += kPartBlock) {
+    for (uint32_t p = t; p < F; p += BLOCK) {
         uint32_t c = L.hist[(uint64_t)blockIdx.x * F + p];
         cursor[p] = c ? atomicAdd(&L.pcursor[p], (unsigned long long)c) : 0;
         tcnt[p] = 0;
     }
     const uint64_t lo = (uint64_t)blockIdx.x * L.rows_per_blk;
-    const uint64_t hi = min(L.n, lo + L.rows_per_blk);
+    uint64_t hi = min(L.n, lo + L.rows_per_blk);
+    if (in.nlimit) hi = min(hi, (uint64_t)*in.nlimit);
     v4i nk[Q], nv[Q];
     uint32_t nm[Q];
     auto load_tile = [&](uint64_t tb) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const uint64_t i0 = tb + 4ull * t + (uint64_t)q * 4 * kPartBlock;
+            const uint64_t i0 = tb + 4ull * t + (uint64_t)q * 4 * BLOCK;
             v4i c;
             nm[q] = (i0 < hi) ? quad_mask(in, i0, hi, &c) : 0u;
             nk[q] = v4i{0, 0, 0, 0};
@@ -241,7 +261,7 @@ __global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartI
     };
     if (lo < hi) load_tile(lo);
     __syncthreads();
-    for (uint64_t tb = lo; tb < hi; tb += kTile) {
+    for (uint64_t tb = lo; tb < hi; tb += TILE) {
         v4i ck[Q], cv[Q];
         uint32_t cm[Q];
 #pragma unroll
@@ -250,7 +270,7 @@ __global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartI
             cv[q] = nv[q];
             cm[q] = nm[q];
         }
-        if (tb + kTile < hi) load_tile(tb + kTile);
+        if (tb + TILE < hi) load_tile(tb + TILE);
         // 1. rank every row within its partition
         uint32_t rank[Q][4];
 #pragma unroll
@@ -260,13 +280,13 @@ __global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartI
                 rank[q][j] = ((cm[q] >> j) & 1u) ? atomicAdd(&tcnt[part_key(in, q_get(ck[q], j), F)], 1u) : 0u;
         __syncthreads();
         // 2. exclusive scan of the tile counts (<= 2 partitions per thread)
-        const uint32_t per = (F + kPartBlock - 1) / kPartBlock;
+        const uint32_t per = (F + BLOCK - 1) / BLOCK;
         uint32_t mine = 0;
         for (uint32_t e = 0; e < per; ++e) {
             const uint32_t p = t * per + e;
             mine += (p < F) ? tcnt[p] : 0;
         }
-        uint32_t base = scan_block_excl(mine, wtot);
+        uint32_t base = scan_block_excl<BLOCK>(mine, wtot);
         for (uint32_t e = 0; e < per; ++e) {
             const uint32_t p = t * per + e;
             if (p < F) {
@@ -288,13 +308,13 @@ __global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartI
                     sv[pos] = q_get(cv[q], j);
                     if (ROWID)
                         sr[pos] = (int32_t)(in.row_base +
-                                            (int64_t)(tb + 4ull * t + (uint64_t)q * 4 * kPartBlock + j));
+                                            (int64_t)(tb + 4ull * t + (uint64_t)q * 4 * BLOCK + j));
                 }
             }
         }
         __syncthreads();
         // 4. write the runs
-        for (uint32_t i = t; i < total; i += kPartBlock) {
+        for (uint32_t i = t; i < total; i += BLOCK) {
             const int32_t k = sk[i];
             const uint32_t p = part_key(in, k, F);
             const unsigned long long dst = cursor[p] + (i - toff[p]);
@@ -303,7 +323,7 @@ __global__ __launch_bounds__(kPartBlock) void k_part_scatter(PartLayout L, PartI
             if (ROWID) L.col[2][dst] = sr[i];
         }
         __syncthreads();
-        for (uint32_t p = t; p < F; p += kPartBlock) {
+        for (uint32_t p = t; p < F; p += BLOCK) {
             cursor[p] += tcnt[p];
             tcnt[p] = 0;
         }
@@ -406,21 +426,40 @@ static int part_probe(const typename Pipe::Params &prm, const PartLayout &L, con
     }
 }
 
-int partition_rows(const PartLayout &L, const PartIn &in, hipStream_t s) {
-    for (const int32_t *p : {in.pred, in.key, in.val})
-        if (p && !aligned16(p)) return fail(MGV_E_ALIGN, "partitioned column is not 16-byte aligned");
-    hipLaunchKernelGGL(k_part_prep, dim3(1), dim3(kMaxFanout), 0, s, L);
+template <bool ROWID, uint32_t MAXF>
+static void launch_scatter(const PartLayout &L, const PartIn &in, uint32_t shape, hipStream_t s) {
+    switch (shape) {
+        case MGV_SCATTER_512x8192:
+            hipLaunchKernelGGL((k_part_scatter<ROWID, MAXF, 512, 8192>), dim3(L.nblk), dim3(512), 0, s, L, in);
+            break;
+        case MGV_SCATTER_1024x8192:
+            hipLaunchKernelGGL((k_part_scatter<ROWID, MAXF, 1024, 8192>), dim3(L.nblk), dim3(1024), 0, s, L, in);
+            break;
+        default:
+            hipLaunchKernelGGL((k_part_scatter<ROWID, MAXF, 256, 4096>), dim3(L.nblk), dim3(256), 0, s, L, in);
+    }
+}
+
+template <uint32_t MAXF>
+static int partition_rows_f(const PartLayout &L, const PartIn &in, uint32_t shape, hipStream_t s) {
+    hipLaunchKernelGGL(k_part_prep, dim3(1), dim3(1024), 0, s, L);
     int rc = check_launch("k_part_prep");
     if (rc) return rc;
-    hipLaunchKernelGGL(k_part_count, dim3(L.nblk), dim3(kPartBlock), 0, s, L, in);
+    hipLaunchKernelGGL(k_part_count<MAXF>, dim3(L.nblk), dim3(kPartBlock), 0, s, L, in);
     if ((rc = check_launch("k_part_count"))) return rc;
-    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kMaxFanout), 0, s, L);
+    hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kScanThreads), 0, s, L);
     if ((rc = check_launch("k_part_scan"))) return rc;
-    if (in.want_rowid)
-        hipLaunchKernelGGL(k_part_scatter<true>, dim3(L.nblk), dim3(kPartBlock), 0, s, L, in);
-    else
-        hipLaunchKernelGGL(k_part_scatter<false>, dim3(L.nblk), dim3(kPartBlock), 0, s, L, in);
+    if (in.want_rowid) launch_scatter<true, MAXF>(L, in, 0, s);   // row ids: default shape only
+    else launch_scatter<false, MAXF>(L, in, shape, s);
     return check_launch("k_part_scatter");
+}
+
+int partition_rows(const PartLayout &L, const PartIn &in, hipStream_t s, uint32_t shape) {
+    for (const int32_t *p : {in.pred, in.key, in.val})
+        if (p && !aligned16(p)) return fail(MGV_E_ALIGN, "partitioned column is not 16-byte aligned");
+    if (L.F <= kMaxFanout) return partition_rows_f<kMaxFanout>(L, in, shape, s);
+    if (in.want_rowid) return fail(MGV_E_INVALID, "row ids are partitioned with fan-out <= %u only", kMaxFanout);
+    return partition_rows_f<kMaxFanoutWide>(L, in, shape, s);
 }
 
 __global__ void k_xcc_histogram(uint32_t *out) {
@@ -435,6 +474,7 @@ extern "C" {
 
 int mgv_partition_workspace_size(uint64_t n, uint32_t fanout, uint32_t ncols, uint64_t *bytes) {
     if (!bytes) return fail(MGV_E_INVALID, "null bytes");
+    if (fanout > kMaxFanout) return fail(MGV_E_INVALID, "fan-out above %u", kMaxFanout);
     return plan_layout(n, fanout, ncols, nullptr, 0, nullptr, bytes);
 }
 
@@ -446,13 +486,14 @@ int mgv_hashjoin_probe_partitioned(const uint64_t *slots, uint32_t nbuckets, con
     if (!slots || !keys || !res || nbuckets == 0) return fail(MGV_E_INVALID, "probe_partitioned: null argument");
     if (n == 0) return MGV_OK;
     if (n >= (1ull << 31)) return fail(MGV_E_INVALID, "probe_partitioned: n must be < 2^31");
+    if (fanout > kMaxFanout) return fail(MGV_E_INVALID, "probe_partitioned: fan-out above %u", kMaxFanout);
     const bool digest = (flags & MGV_PROBE_DIGEST) != 0;
     PartLayout L;
     int rc = plan_layout(n, fanout, digest ? 3 : 2, ws, ws_bytes, &L, nullptr);
     if (rc) return rc;
     hipStream_t s = stream_of(st);
-    PartIn in = {pred, bound, keys, vals, row_base, digest ? 1u : 0u, 0u, 0u, res};
-    if ((rc = partition_rows(L, in, s))) return rc;
+    PartIn in = {pred, bound, keys, vals, row_base, digest ? 1u : 0u, 0u, 0u, res, nullptr};
+    if ((rc = partition_rows(L, in, s, scatter_shape(LC)))) return rc;
     if (digest) {
         PProbeSum<false, true, true>::Params p = {0, slots, nbuckets, res};
         return part_probe<PProbeSum<false, true, true>>(p, L, LC, s);
@@ -474,12 +515,13 @@ int mgv_hashjoin_probe_groupby_partitioned(const uint64_t *slots, uint32_t nbuck
         return fail(MGV_E_INVALID, "probe_groupby_partitioned: bad argument");
     if (n == 0) return MGV_OK;
     if (n >= (1ull << 31)) return fail(MGV_E_INVALID, "probe_groupby_partitioned: n must be < 2^31");
+    if (fanout > kMaxFanout) return fail(MGV_E_INVALID, "probe_groupby_partitioned: fan-out above %u", kMaxFanout);
     PartLayout L;
     int rc = plan_layout(n, fanout, 2, ws, ws_bytes, &L, nullptr);
     if (rc) return rc;
     hipStream_t s = stream_of(st);
-    PartIn in = {pred, bound, keys, vals, 0, 0u, 0u, 0u, nullptr};
-    if ((rc = partition_rows(L, in, s))) return rc;
+    PartIn in = {pred, bound, keys, vals, 0, 0u, 0u, 0u, nullptr, nullptr};
+    if ((rc = partition_rows(L, in, s, scatter_shape(LC)))) return rc;
     if (ngroups <= 1024) {
         PProbeGroup<false, true, 1024>::Params p = {0, slots, nbuckets, gcnt, gsum, ngroups};
         return part_probe<PProbeGroup<false, true, 1024>>(p, L, LC, s);

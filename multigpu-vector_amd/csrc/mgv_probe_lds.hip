@@ -1,0 +1,337 @@
+// mgv_probe_lds.hip -- hash-join probe against table slices held in LDS.
+//
+// The partitioned probe keeps a partition's table slice in the XCD's L2, but
+// every probe row still costs one L1->L2 request, and those requests' latency
+// and the L1's limit on outstanding misses bound it.  This path partitions
+// the probe rows twice by the top bits of fmix32(key) -- the bits that also
+// pick the bucket --:
+//   level 1: F1 partitions (default 64), packed without gaps;
+//   level 2: F2 = nbuckets / S partitions of the level-1 output, where a
+//            slice of S <= 2048 buckets (128 KiB) fits in LDS; because the
+//            level-1 output is already ordered by the top log2(F1) bits, a
+//            level-2 tile spreads over only F2 / F1 partitions, so the second
+//            scatter writes long runs.
+// Then one block per CU claims level-2 partitions: it copies the partition's
+// table slice (plus a few following buckets for chains that run past it)
+// into LDS and probes the partition's rows from LDS; a chain that leaves the
+// loaded range continues in global memory.  Semantics are those of
+// mgv_hashjoin_probe: COUNT of matching (row, slot) pairs and SUM(val +
+// payload), exact in 64 bits.  Replaces hashjoin::consume_warp
+// (src/operators/hashjoin.cu:135-182) with its aggregation parent
+// (src/operators/aggregation.cu:27-66).
+#include <hip/hip_runtime.h>
+
+#include "mgv_internal.hpp"
+#include "mgv_ops.hpp"
+#include "mgv_part.hpp"
+
+namespace mgv {
+
+constexpr uint32_t kSliceMax = 2048;     // buckets of a slice in LDS (128 KiB)
+constexpr uint32_t kSliceExtra = 8;      // buckets loaded past the slice
+constexpr int kLdsBlock = 1024;
+
+struct LdsProbeArgs {
+    const uint64_t *slots;
+    uint32_t nbuckets;
+    uint32_t slice;          // buckets per level-2 partition
+    PartLayout L;            // level-2 layout
+    uint32_t *next;          // partition counter (zeroed before the launch)
+    mgv_result *res;
+};
+
+// A table slice in LDS keeps the global layout: 64-byte buckets of 8
+// {key, payload} slots, empty = (-1, -1).
+
+// Scans one bucket for key: number of hitting slots and the sum of their
+// payloads.  CAREFUL (key == -1 somewhere in the wave) excludes the empty
+// pattern (-1, -1) from the compare.  The common case (at most one hit)
+// selects the hit's payload instead of summing per slot.
+template <bool CAREFUL>
+__device__ __forceinline__ void bucket_scan(const v4u (&q)[4], uint32_t key, uint32_t &hits, int64_t &psum) {
+    uint32_t h = 0;
+    int32_t sel = 0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const uint32_t sk = (s & 1) ? q[s >> 1].z : q[s >> 1].x;
+        const uint32_t sp = (s & 1) ? q[s >> 1].w : q[s >> 1].y;
+        const bool hit = CAREFUL ? (sk == key && !(sk == ~0u && sp == ~0u)) : (sk == key);
+        h += hit ? 1u : 0u;
+        sel = hit ? (int32_t)sp : sel;
+    }
+    if (ballot(h > 1) != 0) {   // duplicate build keys: sum every hit
+        int64_t t = 0;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const uint32_t sk = (s & 1) ? q[s >> 1].z : q[s >> 1].x;
+            const uint32_t sp = (s & 1) ? q[s >> 1].w : q[s >> 1].y;
+            const bool hit = CAREFUL ? (sk == key && !(sk == ~0u && sp == ~0u)) : (sk == key);
+            t += hit ? (int64_t)(int32_t)sp : 0;
+        }
+        psum += t;
+    } else {
+        psum += h ? (int64_t)sel : 0;
+    }
+    hits += h;
+}
+
+__device__ __forceinline__ bool bucket_full(const v4u (&q)[4]) { return !(q[3].z == ~0u && q[3].w == ~0u); }
+
+// Rows whose chain continues past their home bucket (about 5 % at load
+// factor 0.5, but some row of nearly every wave) are not followed on the
+// spot, which would cost the whole wave a scan per row: they are appended to
+// a per-wave queue in LDS {key, value, bucket} and probed 64 at a time, one
+// per lane, once the queue holds a full wave's worth (and at the end).
+constexpr uint32_t kQueue = 128;   // entries per wave (a power of two)
+
+struct WaveQueue {
+    uint32_t key[kQueue];
+    int32_t val[kQueue];
+    uint32_t bkt[kQueue];
+};
+
+// Appends this lane's entry when `want`; returns the new tail.
+__device__ __forceinline__ uint32_t queue_push(WaveQueue &Q, uint32_t tail, bool want, uint32_t key, int32_t val,
+                                               uint32_t bkt) {
+    const uint64_t bal = ballot(want);
+    if (want) {
+        const uint32_t pos = (tail + mbcnt(bal)) & (kQueue - 1);
+        Q.key[pos] = key;
+        Q.val[pos] = val;
+        Q.bkt[pos] = bkt;
+    }
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)(tail + (uint32_t)popc64(bal)));
+}
+
+// Probes up to 64 queued entries (one per lane) and re-queues the chains
+// that go on; returns the new head and, through tail, the new tail.
+__device__ __forceinline__ uint32_t queue_round(WaveQueue &Q, uint32_t head, uint32_t &tail, const v4u *tab,
+                                                const uint64_t *slots, uint32_t nb, uint32_t lo, uint32_t nload,
+                                                uint64_t &cnt, uint64_t &sum) {
+    const uint32_t avail = tail - head;
+    const uint32_t take = avail < kWave ? avail : kWave;
+    const uint32_t l = lane_id();
+    const bool mine = l < take;
+    uint32_t key = 0, bkt = 0;
+    int32_t val = 0;
+    if (mine) {
+        const uint32_t pos = (head + l) & (kQueue - 1);
+        key = Q.key[pos];
+        val = Q.val[pos];
+        bkt = Q.bkt[pos];
+    }
+    wave_sync();   // every lane has read its entry before slots are reused
+    v4u q[4];
+    const uint32_t idx = bkt - lo;
+    if (mine && idx < nload) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) q[w] = tab[idx * 4 + w];
+    } else if (mine) {
+        const v4u *g = reinterpret_cast<const v4u *>(slots + (uint64_t)bkt * 8);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) q[w] = g[w];
+    } else {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) q[w] = v4u{~0u, ~0u, ~0u, ~0u};
+    }
+    uint32_t hits = 0;
+    int64_t psum = 0;
+    bucket_scan<true>(q, key, hits, psum);
+    if (mine && hits) {
+        cnt += hits;
+        sum += (uint64_t)(psum + (int64_t)hits * (int64_t)val);
+    }
+    // a chain ends at a bucket with an empty slot, or after visiting all nb
+    // buckets once (when its next bucket is its home bucket again)
+    const uint32_t nxt = (bkt + 1 == nb) ? 0 : bkt + 1;
+    const bool more = mine && bucket_full(q) && nxt != bucket_of((int32_t)key, nb);
+    head += take;
+    tail = queue_push(Q, tail, more, key, val, nxt);
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)head);
+}
+
+// A lane's four rows: the four home buckets are read from LDS together (16
+// reads in flight) and scanned; rows whose bucket is full continue at the
+// next bucket, and rows whose home bucket lies outside the slice start at
+// their home bucket, through the wave's queue.
+template <bool CAREFUL>
+__device__ __forceinline__ void lds_probe_quad(const v4i &k, const v4i &v, uint32_t m, const v4u *tab,
+                                               const uint64_t *slots, uint32_t nb, uint32_t lo, uint32_t nload,
+                                               uint64_t &cnt, uint64_t &sum, WaveQueue &Q, uint32_t &head,
+                                               uint32_t &tail) {
+    uint32_t b[4], ii[4];
+    bool ins[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        b[j] = bucket_of(q_get(k, j), nb);
+        const uint32_t idx = b[j] - lo;
+        ins[j] = ((m >> j) & 1u) && idx < nload;
+        ii[j] = ins[j] ? idx : 0u;
+    }
+    v4u q[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) q[j][w] = tab[ii[j] * 4 + w];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        uint32_t hits = 0;
+        int64_t psum = 0;
+        bucket_scan<CAREFUL>(q[j], ins[j] ? (uint32_t)q_get(k, j) : 0u, hits, psum);
+        if (ins[j] && hits) {   // rows outside the slice scanned bucket `lo`: discarded
+            cnt += hits;
+            sum += (uint64_t)(psum + (int64_t)hits * (int64_t)q_get(v, j));
+        }
+        const bool live = (m >> j) & 1u;
+        const uint32_t from = ins[j] ? ((b[j] + 1 == nb) ? 0 : b[j] + 1) : b[j];
+        const bool go = live && (!ins[j] || (bucket_full(q[j]) && from != b[j]));
+        tail = queue_push(Q, tail, go, (uint32_t)q_get(k, j), q_get(v, j), from);
+        // fewer than kWave entries were queued before this push and at most
+        // kWave were added, so the queue never holds more than kQueue - 1
+        while (tail - head >= kWave) head = queue_round(Q, head, tail, tab, slots, nb, lo, nload, cnt, sum);
+    }
+}
+
+// Rows [i0, i0 + 4) of a partition ending at r1, as one unconditional
+// dwordx4 load per column (no branch, so the loads of the next quad stay in
+// flight while the current one is probed); the mask marks the rows < r1.
+// Partition columns hold at least 4 rows past any in-range quad start
+// (capacities are multiples of 64 rows), so the load stays in the buffer.
+__device__ __forceinline__ uint32_t load_rows4(const int32_t *K, const int32_t *V, uint64_t i0, uint64_t r1, v4i &k,
+                                               v4i &v) {
+    k = load_stream4(K + i0);
+    v = load_stream4(V + i0);
+    const uint64_t left = r1 > i0 ? r1 - i0 : 0;
+    return left >= 4 ? 0xfu : (1u << left) - 1u;
+}
+
+__global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a) {
+    __shared__ v4u tab[(kSliceMax + kSliceExtra) * 4];
+    __shared__ WaveQueue s_queue[kLdsBlock / kWave];
+    __shared__ uint32_t s_part;
+    const uint32_t t = threadIdx.x;
+    WaveQueue &Q = s_queue[t / kWave];
+    const uint32_t nb = a.nbuckets;
+    const int32_t *K = a.L.col[0];
+    const int32_t *V = a.L.col[1];
+    uint64_t cnt = 0;
+    uint64_t sum = 0;   // two's-complement int64
+    for (;;) {
+        if (t == 0) s_part = atomicAdd(a.next, 1u);
+        __syncthreads();
+        const uint32_t p = s_part;
+        __syncthreads();   // s_part is rewritten next round; the slice is free again
+        if (p >= a.L.F) break;
+        const uint64_t r0 = a.L.pstart[p];
+        const uint64_t r1 = r0 + a.L.ptotal[p];
+        if (r0 == r1) continue;
+        const uint32_t lo = p * a.slice;
+        const uint32_t nload = min(a.slice + kSliceExtra, nb - lo);
+        // the first rows' loads overlap the slice copy
+        v4i k, v;
+        uint64_t i0 = r0 + 4ull * t;
+        uint32_t m = load_rows4(K, V, i0 < r1 ? i0 : r0, i0 < r1 ? r1 : r0, k, v);
+        const v4u *src = reinterpret_cast<const v4u *>(a.slots + (uint64_t)lo * 8);
+        for (uint32_t i = t; i < nload * 4; i += kLdsBlock) tab[i] = src[i];
+        __syncthreads();
+        // r0 is a multiple of 64 rows, so quads are 16-byte aligned
+        uint32_t head = 0, tail = 0;   // the wave's queue (wave-uniform counters)
+        // every lane of a wave iterates until the wave's last quad (lanes past
+        // the end carry an empty mask), so the queue counters and ballots stay
+        // wave-uniform
+        while (ballot(i0 < r1)) {
+            const uint64_t i1 = i0 + 4ull * kLdsBlock;
+            v4i kn, vn;
+            const uint32_t mn = load_rows4(K, V, i1 < r1 ? i1 : r0, i1 < r1 ? r1 : r0, kn, vn);
+            bool neg = false;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) neg |= ((m >> j) & 1u) && q_get(k, j) == -1;
+            if (ballot(neg)) lds_probe_quad<true>(k, v, m, tab, a.slots, nb, lo, nload, cnt, sum, Q, head, tail);
+            else lds_probe_quad<false>(k, v, m, tab, a.slots, nb, lo, nload, cnt, sum, Q, head, tail);
+            k = kn;
+            v = vn;
+            m = mn;
+            i0 = i1;
+        }
+        // the rest of the queue, before the slice is replaced
+        while (tail != head) head = queue_round(Q, head, tail, tab, a.slots, nb, lo, nload, cnt, sum);
+        __syncthreads();
+    }
+    cnt = wave_sum_u64(cnt);
+    sum = wave_sum_u64(sum);
+    if (lane_id() == 0) {
+        if (cnt) atomicAdd((unsigned long long *)&a.res->count, (unsigned long long)cnt);
+        if (sum) atomicAdd((unsigned long long *)&a.res->sum, (unsigned long long)sum);
+    }
+}
+
+// Level-2 fan-out for a table of nb buckets: the smallest power of two
+// F2 >= 2 * F1 that divides nb into slices of at most kSliceMax buckets.
+static int lds_fanout(uint32_t nb, uint32_t F1, uint32_t *F2) {
+    uint32_t f = 2 * F1;
+    while (f <= kMaxFanoutWide && (nb % f != 0 || nb / f > kSliceMax)) f <<= 1;
+    if (f > kMaxFanoutWide || nb % f != 0 || nb / f > kSliceMax || nb / f == 0)
+        return fail(MGV_E_INVALID, "LDS probe: %u buckets do not split into <= %u slices of <= %u buckets",
+                    nb, kMaxFanoutWide, kSliceMax);
+    *F2 = f;
+    return MGV_OK;
+}
+
+static int lds_plan(uint64_t n, uint32_t nb, uint32_t F1, void *ws, uint64_t ws_bytes, PartLayout *L1,
+                    PartLayout *L2, uint32_t *F2, uint64_t *need) {
+    if (F1 < 2 || F1 > kMaxFanout || (F1 & (F1 - 1)))
+        return fail(MGV_E_INVALID, "LDS probe: level-1 fan-out %u not a power of two in [2,%u]", F1, kMaxFanout);
+    int rc = lds_fanout(nb, F1, F2);
+    if (rc) return rc;
+    uint64_t b1 = 0, b2 = 0;
+    if ((rc = plan_layout(n, F1, 2, nullptr, 0, nullptr, &b1, 1))) return rc;
+    if ((rc = plan_layout(n, *F2, 2, nullptr, 0, nullptr, &b2, kPartAlign))) return rc;
+    const uint64_t o1 = 256, o2 = 256 + (b1 + 255) / 256 * 256;
+    if (need) *need = o2 + b2;
+    if (!L1) return MGV_OK;
+    if (!ws || ws_bytes < o2 + b2) return fail(MGV_E_CAPACITY, "LDS probe: workspace too small");
+    if ((rc = plan_layout(n, F1, 2, (char *)ws + o1, b1, L1, nullptr, 1))) return rc;
+    return plan_layout(n, *F2, 2, (char *)ws + o2, b2, L2, nullptr, kPartAlign);
+}
+
+}  // namespace mgv
+
+using namespace mgv;
+
+extern "C" {
+
+int mgv_probe_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes) {
+    if (!bytes || nbuckets == 0) return fail(MGV_E_INVALID, "mgv_probe_lds_workspace_size: bad argument");
+    uint32_t F2 = 0;
+    return lds_plan(n, nbuckets, fanout, nullptr, 0, nullptr, nullptr, &F2, bytes);
+}
+
+int mgv_hashjoin_probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
+                           const int32_t *keys, const int32_t *vals, uint64_t n, uint32_t fanout, void *ws,
+                           uint64_t ws_bytes, mgv_result *res, mgv_launch LC, mgv_stream_t st) {
+    if (!current_state()) return fail(MGV_E_NOT_INIT, "mgv_init was not called");
+    if (!slots || !keys || !res || nbuckets == 0) return fail(MGV_E_INVALID, "probe_lds: null argument");
+    if (n == 0) return MGV_OK;
+    if (n >= (1ull << 31)) return fail(MGV_E_INVALID, "probe_lds: n must be < 2^31");
+    PartLayout L1, L2;
+    uint32_t F2 = 0;
+    int rc = lds_plan(n, nbuckets, fanout, ws, ws_bytes, &L1, &L2, &F2, nullptr);
+    if (rc) return rc;
+    hipStream_t s = stream_of(st);
+    // level 1: select + hash partitions, packed (the total is the survivor count)
+    PartIn in1 = {pred, bound, keys, vals, 0, 0u, 0u, 0u, nullptr, nullptr};
+    if ((rc = partition_rows(L1, in1, s, scatter_shape(LC)))) return rc;
+    // level 2: the level-1 output, ordered by the same hash bits, split finer
+    PartIn in2 = {nullptr, 0, L1.col[0], L1.col[1], 0, 0u, 0u, 0u, nullptr, L1.pstart + L1.F};
+    if ((rc = partition_rows(L2, in2, s, scatter_shape(LC)))) return rc;
+    uint32_t *next = (uint32_t *)ws;
+    if ((rc = mgv_fill_u32(next, 1, 0, (mgv_stream_t)s))) return rc;
+    static thread_local int cached = 0;
+    if (!cached) cached = persistent_blocks(k_lds_probe, kLdsBlock);
+    const uint32_t grid = LC.grid ? LC.grid : (uint32_t)cached;
+    LdsProbeArgs a = {slots, nbuckets, nbuckets / F2, L2, next, res};
+    hipLaunchKernelGGL(k_lds_probe, dim3(grid), dim3(kLdsBlock), 0, s, a);
+    return check_launch("k_lds_probe");
+}
+
+}  // extern "C"

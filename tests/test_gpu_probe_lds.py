@@ -1,0 +1,135 @@
+"""GPU tests of the two-level partitioned probe with LDS-resident table slices
+(mgv_hashjoin_probe_lds): COUNT and int64 SUM must equal the oracle's
+chained-table join and the direct GPU probe, including duplicate build keys,
+chains that run past a slice and wrap around the table, the key -1 (half of
+the empty-slot pattern), a filter column, and a missing value column."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import mgv
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return mgv.device_array(np.ascontiguousarray(a, dtype=np.int32))
+
+
+def build(keys, payload, lf):
+    ht = mgv.HashTable(len(keys), lf)
+    dk, dp = dev(keys), dev(payload)
+    res = mgv.ResultBuffer()
+    mgv.call("mgv_hashjoin_build", ht.buf.vp, ht.nbuckets, dk.vp, dp.vp, mgv.MGV_PAYLOAD_COLUMN, None, 0, len(keys), 0,
+             res.vp, mgv.launch(0, 256), None)
+    mgv.sync()
+    assert res.read().errors == 0
+    return ht
+
+
+def probe_lds(ht, rk, rv, pred=None, bound=0, fanout=64, grid=0):
+    n = len(rk)
+    need = C.c_uint64()
+    mgv.call("mgv_probe_lds_workspace_size", n, ht.nbuckets, fanout, C.byref(need))
+    ws = mgv.DeviceBuffer(need.value)
+    dk = dev(rk)
+    dv = dev(rv) if rv is not None else None
+    dp = dev(pred) if pred is not None else None
+    res = mgv.ResultBuffer()
+    mgv.call("mgv_hashjoin_probe_lds", ht.buf.vp, ht.nbuckets, dp.vp if dp else None, bound, dk.vp,
+             dv.vp if dv else None, n, fanout, ws.vp, ws.nbytes, res.vp, mgv.launch(grid, 1024), None)
+    mgv.sync()
+    r = res.read()
+    return r.count, r.sum
+
+
+def probe_direct(ht, rk, rv, pred=None, bound=0):
+    dk = dev(rk)
+    dv = dev(rv) if rv is not None else None
+    dp = dev(pred) if pred is not None else None
+    res = mgv.ResultBuffer()
+    mgv.call("mgv_hashjoin_probe", ht.buf.vp, ht.nbuckets, dp.vp if dp else None, bound, dk.vp,
+             dv.vp if dv else None, len(rk), 0, 0, res.vp, mgv.launch(0, 256), None)
+    mgv.sync()
+    r = res.read()
+    return r.count, r.sum
+
+
+@pytest.mark.parametrize("nb_keys,n,fanout,lf", [(1 << 16, 1 << 20, 64, 0.5), (1 << 20, 3 << 20, 64, 0.5),
+                                                 (16384, 100003, 2, 0.5), (51200, 200000, 8, 0.75)])
+def test_lds_probe_matches_oracle(gpu, orc, nb_keys, n, fanout, lf):
+    sk = orc.gen_perm(nb_keys, nb_keys, orc.seed(2, 0), 7)
+    sw = orc.gen_bits(nb_keys, orc.seed(2, 1), 7, 16)
+    rk = orc.gen_range(n, orc.seed(2, 2), 7, 2 * nb_keys)
+    rv = orc.gen_bits(n, orc.seed(2, 3), 7, 16)
+    ht = build(sk, sw, lf)
+    got = probe_lds(ht, rk, rv, fanout=fanout)
+    t = orc.table(sk, sw)
+    want = orc.probe(t, rk, rv)
+    orc.free(t)
+    assert got == (want["count"], want["sum"]) == probe_direct(ht, rk, rv)
+
+
+def test_lds_probe_with_filter_and_without_values(gpu, orc):
+    nb_keys, n = 1 << 17, 1 << 21
+    sk = orc.gen_perm(nb_keys, nb_keys, orc.seed(2, 0), 8)
+    sw = orc.gen_bits(nb_keys, orc.seed(2, 1), 8, 16)
+    rk = orc.gen_range(n, orc.seed(2, 2), 8, 2 * nb_keys)
+    rv = orc.gen_bits(n, orc.seed(2, 3), 8, 16)
+    c = orc.gen_bits(n, orc.seed(3, 0), 8, 31)
+    ht = build(sk, sw, 0.5)
+    t = orc.table(sk, sw)
+    for bound in ((1 << 30) - 1, 21474836, -1, 2**31 - 1):
+        want = orc.probe(t, rk, rv, pred=c, bound=bound)
+        assert probe_lds(ht, rk, rv, pred=c, bound=bound) == (want["count"], want["sum"])
+    want = orc.probe(t, rk)
+    orc.free(t)
+    assert probe_lds(ht, rk, None) == (want["count"], want["sum"])
+
+
+def test_lds_probe_duplicates_long_chains_and_wraparound(gpu, orc):
+    """Load factor 0.9 with heavy duplicates: chains leave their slice (and the
+    extra buckets loaded past it) and wrap from the last bucket to bucket 0."""
+    nb_keys, n = 60000, 400000
+    sk = orc.gen_range(nb_keys, orc.seed(2, 6), 0, 6000)          # ~10 copies of each key
+    sw = orc.gen_bits(nb_keys, orc.seed(2, 7), 0, 16)
+    rk = orc.gen_range(n, orc.seed(2, 8), 0, 7000)
+    rv = orc.gen_bits(n, orc.seed(2, 9), 0, 16)
+    ht = build(sk, sw, 0.9)
+    t = orc.table(sk, sw)
+    want = orc.probe(t, rk, rv)
+    orc.free(t)
+    for fanout in (2, 4):
+        assert probe_lds(ht, rk, rv, fanout=fanout) == (want["count"], want["sum"]) == probe_direct(ht, rk, rv)
+
+
+def test_lds_probe_key_minus_one(gpu, orc):
+    nb_keys, n = 8192, 50000
+    sk = orc.gen_perm(nb_keys, nb_keys, orc.seed(2, 0), 9) - 1          # includes -1 (payload != -1)
+    sw = orc.gen_bits(nb_keys, orc.seed(2, 1), 9, 16)
+    rk = orc.gen_range(n, orc.seed(2, 2), 9, 2 * nb_keys) - 1
+    rk[::97] = -1
+    rv = orc.gen_bits(n, orc.seed(2, 3), 9, 16)
+    ht = build(sk, sw, 0.5)
+    t = orc.table(sk, sw)
+    want = orc.probe(t, rk, rv)
+    orc.free(t)
+    assert probe_lds(ht, rk, rv, fanout=2) == (want["count"], want["sum"])
+
+
+def test_lds_probe_few_blocks_and_rejects_unsplittable_tables(gpu, orc):
+    nb_keys, n = 1 << 16, 1 << 19
+    sk = orc.gen_perm(nb_keys, nb_keys, orc.seed(2, 0), 10)
+    sw = orc.gen_bits(nb_keys, orc.seed(2, 1), 10, 16)
+    rk = orc.gen_range(n, orc.seed(2, 2), 10, 2 * nb_keys)
+    rv = orc.gen_bits(n, orc.seed(2, 3), 10, 16)
+    ht = build(sk, sw, 0.5)
+    assert probe_lds(ht, rk, rv, fanout=16, grid=3) == probe_direct(ht, rk, rv)
+    need = C.c_uint64()
+    with pytest.raises(mgv.MgvError):
+        mgv.call("mgv_probe_lds_workspace_size", 1000, 25, 2, C.byref(need))     # odd bucket count
+    with pytest.raises(mgv.MgvError):
+        mgv.call("mgv_probe_lds_workspace_size", 1000, 1 << 24, 64, C.byref(need))  # slices above 2048 buckets
