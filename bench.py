@@ -98,7 +98,9 @@ def leg_c2(args, st):
     t2 = tuned("c2") or {}
     block = args.block or int(t2.get("block", 256))
     coop = int(t2.get("coop", 0)) if args.coop < 0 else args.coop
-    L = mgv.Launch(0, block, -1, coop)
+    shape = int(t2.get("scatter", 2)) if args.scatter < 0 else args.scatter
+    L = mgv.Launch(0, block, -1, coop | (shape << 8))
+    Llds = mgv.Launch(0, 1024, -1, shape << 8)
 
     # build (timed: clear excluded)
     build_ms = []
@@ -116,37 +118,47 @@ def leg_c2(args, st):
 
     ws = None
 
-    def make_probe(fanout):
+    def make_probe(mode, fanout):
+        """mode "direct" (one pass over R), "part" (one-level partitioned,
+        fan-out F) or "lds" (two-level partitioned, slices probed from LDS)."""
         nonlocal ws
-        if fanout > 1:
-            need = C.c_uint64()
+        need = C.c_uint64()
+        if mode == "part":
             mgv.call("mgv_partition_workspace_size", nr, fanout, 2, C.byref(need))
-            if ws is None or ws.nbytes < need.value:
-                if ws is not None:
-                    ws.close()
-                ws = mgv.DeviceBuffer(need.value)
+        elif mode == "lds":
+            mgv.call("mgv_probe_lds_workspace_size", nr, ht.nbuckets, fanout, C.byref(need))
+        if need.value and (ws is None or ws.nbytes < need.value):
+            if ws is not None:
+                ws.close()
+            ws = mgv.DeviceBuffer(need.value)
+        if mode == "part":
             return lambda: mgv.call("mgv_hashjoin_probe_partitioned", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp,
                                     nr, 0, 0, fanout, ws.vp, ws.nbytes, res.vp, L, st.vp)
+        if mode == "lds":
+            return lambda: mgv.call("mgv_hashjoin_probe_lds", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp, nr,
+                                    fanout, ws.vp, ws.nbytes, res.vp, Llds, st.vp)
         return lambda: mgv.call("mgv_hashjoin_probe", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp, nr, 0, 0,
                                 res.vp, L, st.vp)
 
-    # choose the probe path: direct (fan-out 1) or partitioned
-    if args.fanout >= 0:
-        candidates = [args.fanout]
+    # choose the probe path
+    if args.probe != "auto":
+        candidates = [(args.probe, args.fanout if args.fanout > 0 else (1 if args.probe == "direct" else 64))]
     else:
-        candidates = sorted({1, 64, 128, 256} | ({int(t2["fanout"])} if "fanout" in t2 else set()))
+        candidates = [("direct", 1), ("part", 64), ("lds", 32), ("lds", 64)]
+        if "mode" in t2 and (t2["mode"], int(t2["fanout"])) not in candidates:
+            candidates.append((t2["mode"], int(t2["fanout"])))
     sweep = {}
-    for f in candidates:
-        fn = make_probe(f)
+    for mode, f in candidates:
+        fn = make_probe(mode, f)
         t = Timer(st.vp)
         for rep in range(3):
             res.clear(st.vp)
             t.around(fn)
         st.sync()
-        sweep[f] = min(t.durations_ms()[1:] or t.durations_ms())
-        log(f"C2: probe fan-out {f}: {sweep[f]:.3f} ms")
-    fanout = min(sweep, key=sweep.get)
-    probe_fn = make_probe(fanout)
+        sweep[(mode, f)] = min(t.durations_ms()[1:] or t.durations_ms())
+        log(f"C2: probe {mode} fan-out {f}: {sweep[(mode, f)]:.3f} ms")
+    best_mode, fanout = min(sweep, key=sweep.get)
+    probe_fn = make_probe(best_mode, fanout)
 
     def probe_step(timer=None):
         res.clear(st.vp)
@@ -193,11 +205,15 @@ def leg_c2(args, st):
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": algo_bytes / (avg_kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBPS,
-            "traffic": traffic_from_profiles(f"probe_c2_f{fanout}"),
-            "kernel": ("k_pipeline<PProbeSum<false,true,false>,256>" if fanout == 1 else
-                       "mgv_hashjoin_probe_partitioned: k_part_prep+k_part_count+k_part_scan+k_part_scatter+"
-                       "k_part_probe<PProbeSum<false,true,false>,256> (events around the whole call)"),
+            "traffic": traffic_from_profiles(f"probe_c2_{best_mode}{fanout}"),
+            "kernel": {"direct": "k_pipeline<PProbeSum<false,true,false>> (mgv_hashjoin_probe)",
+                       "part": "mgv_hashjoin_probe_partitioned: k_part_prep + k_part_count + k_part_scan + "
+                               "k_part_scatter + k_part_probe (events around the whole call)",
+                       "lds": "mgv_hashjoin_probe_lds: two partitionings (k_part_prep/count/scan/scatter x 2) + "
+                              "k_lds_probe (events around the whole call)"}[best_mode],
+            "probe_path": best_mode,
             "fanout": fanout,
+            "scatter_shape": shape,
             "block": block,
             "coop_probe": coop,
             "algorithmic_bytes_per_launch": algo_bytes,
@@ -205,7 +221,7 @@ def leg_c2(args, st):
         "build": {"ms": build_t, "grows_per_s": gb(ns) / (build_t / 1e3),
                   "gbps_algorithmic": (ns * 8 + ht.nbuckets * 64) / (build_t / 1e3) / 1e9},
         "result": {"count": r.count, "sum": r.sum, "count_closed_form_ok": ok_count},
-        "fanout_sweep_ms": {str(k): v for k, v in sweep.items()},
+        "probe_path_sweep_ms": {f"{m}{f}": v for (m, f), v in sweep.items()},
     }
 
     if args.cpu:
@@ -480,7 +496,10 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="probe workgroup size (0 = tuned, else 256)")
     ap.add_argument("--coop", type=int, default=-1, help="4-lane cooperative probe (-1 = tuned)")
     ap.add_argument("--load-factor", type=float, default=0.5)
-    ap.add_argument("--fanout", type=int, default=-1, help="1 = direct probe, >1 = partitioned, -1 = sweep")
+    ap.add_argument("--probe", default="auto", choices=["auto", "direct", "part", "lds"],
+                    help="probe path (auto = time the candidates, keep the fastest)")
+    ap.add_argument("--fanout", type=int, default=-1, help="fan-out of the partitioned paths (with --probe)")
+    ap.add_argument("--scatter", type=int, default=-1, help="scatter workgroup shape MGV_SCATTER_* (-1 = tuned)")
     ap.add_argument("--legs", default="c1,c4,c5")
     ap.add_argument("--c5-requests", type=int, default=300)
     ap.add_argument("--c4-reps", type=int, default=5)
@@ -527,12 +546,13 @@ def main():
         "config": {"workload": "C2 hash join + aggregate: probe 2^28 R rows against a resident open-addressing "
                                "table of 2^24 S rows at load factor 0.5; COUNT(*), SUM(R.v+S.w) int64",
                    "build_rows": 1 << 24, "probe_rows": 1 << 28, "load_factor": args.load_factor,
-                   "block": c2["roofline"]["block"], "fanout": c2["roofline"]["fanout"], "parallelism": "1 GPU"},
+                   "probe_path": c2["roofline"]["probe_path"], "fanout": c2["roofline"]["fanout"],
+                   "parallelism": "1 GPU"},
         "roofline": c2["roofline"],
         "cpu_baseline": c2.get("cpu_baseline"),
         "sample_parity": c2.get("sample_parity"),
         "probe_kernel_ms": {k: c2[k] for k in ("kernel_ms_avg", "kernel_ms_min", "kernel_ms_median")},
-        "fanout_sweep_ms": c2["fanout_sweep_ms"],
+        "probe_path_sweep_ms": c2["probe_path_sweep_ms"],
         "build": c2["build"],
         "result": c2["result"],
         "device": {"name": p.name.decode(), "arch": p.arch.decode(), "cus": p.cus},

@@ -5,9 +5,10 @@ Workloads and search spaces:
   c4  256-query select -> join -> SUM batch in one fused graph:
       vector size {1K, 4K, 16K, 64K} x workgroup {64, 256, 1024} x
       select compaction {0, 1} x hash-table load factor {0.25, 0.5, 0.75}
-  c2  2^28-row probe of a 2^24-key table (load factor 0.5):
-      fan-out {1 (direct), 16, 32, 64, 128, 256} x workgroup {64, 256, 1024}
-      x probe form {per-lane, 4-lane cooperative}
+  c2  2^28-row probe of a 2^24-key table (load factor 0.5): the direct probe
+      (workgroup {64, 256, 1024} x {per-lane, 4-lane cooperative}), the
+      partitioned probe (fan-out {32..256} x scatter shape) and the two-level
+      LDS probe (fan-out {16..128} x scatter shape)
   c1  filtered-sum scan of 2^28 rows: workgroup x compaction
 
 Every candidate must give results identical to the first candidate's
@@ -99,34 +100,51 @@ def tune_c2(st, quick=False):
     mgv.call("mgv_hashjoin_build", ht.buf.vp, ht.nbuckets, sk.vp, cols["sw"].vp, mgv.MGV_PAYLOAD_COLUMN, None, 0,
              ns, 0, res.vp, mgv.launch(0, 256), st.vp)
     st.sync()
-    fanouts = (1, 64) if quick else (1, 16, 32, 64, 128, 256)
-    blocks = (256,) if quick else (64, 256, 1024)
-    ws = mgv.partition_workspace(nr, max(fanouts), 2)
+    cands = []
+    for blk, coop in itertools.product((256,) if quick else (64, 256, 1024), (0, 1)):
+        cands.append(("direct", 1, blk, coop, 0))
+    for f, shape in itertools.product((64,) if quick else (32, 64, 128, 256), (0, 2) if quick else (0, 1, 2)):
+        cands.append(("part", f, 256, 0, shape))
+    for f, shape in itertools.product((32, 64) if quick else (16, 32, 64, 128), (0, 2) if quick else (0, 1, 2)):
+        cands.append(("lds", f, 1024, 0, shape))
+    need = 0
+    for mode, f, _, _, _ in cands:
+        b = C.c_uint64()
+        if mode == "part":
+            mgv.call("mgv_partition_workspace_size", nr, f, 2, C.byref(b))
+        elif mode == "lds":
+            mgv.call("mgv_probe_lds_workspace_size", nr, ht.nbuckets, f, C.byref(b))
+        need = max(need, b.value)
+    ws = mgv.DeviceBuffer(need)
     ref, table = None, []
-    for f, blk, coop in itertools.product(fanouts, blocks, (0, 1)):
-        L = mgv.Launch(0, blk, -1, coop)
+    for mode, f, blk, coop, shape in cands:
+        L = mgv.Launch(0, blk, -1, coop | (shape << 8))
 
         def run():
             res.clear(st.vp)
-            if f == 1:
+            if mode == "direct":
                 mgv.call("mgv_hashjoin_probe", ht.buf.vp, ht.nbuckets, None, 0, cols["rk"].vp, cols["rv"].vp, nr, 0,
                          0, res.vp, L, st.vp)
-            else:
+            elif mode == "part":
                 mgv.call("mgv_hashjoin_probe_partitioned", ht.buf.vp, ht.nbuckets, None, 0, cols["rk"].vp,
                          cols["rv"].vp, nr, 0, 0, f, ws.vp, ws.nbytes, res.vp, L, st.vp)
+            else:
+                mgv.call("mgv_hashjoin_probe_lds", ht.buf.vp, ht.nbuckets, None, 0, cols["rk"].vp, cols["rv"].vp,
+                         nr, f, ws.vp, ws.nbytes, res.vp, L, st.vp)
 
         ms = time_ms(run, st)
         r = res.read()
         if ref is None:
             ref = (r.count, r.sum)
         elif (r.count, r.sum) != ref:
-            raise RuntimeError(f"c2 candidate fanout={f} block={blk} coop={coop} changed the result")
-        table.append({"fanout": f, "block": blk, "coop": coop, "ms": ms})
-        log(f"c2 fanout={f} block={blk} coop={coop}: {ms:.3f} ms")
+            raise RuntimeError(f"c2 candidate {mode} fanout={f} block={blk} coop={coop} changed the result")
+        table.append({"mode": mode, "fanout": f, "block": blk, "coop": coop, "scatter": shape, "ms": ms})
+        log(f"c2 {mode} fanout={f} block={blk} coop={coop} scatter={shape}: {ms:.3f} ms")
     for b in (sk, ht.buf, res, ws, *cols.values()):
         b.close()
     best = min(table, key=lambda x: x["ms"])
-    return {k: best[k] for k in ("fanout", "block", "coop")} | {"ms": best["ms"], "candidates": table}
+    return {k: best[k] for k in ("mode", "fanout", "block", "coop", "scatter")} | {"ms": best["ms"],
+                                                                                  "candidates": table}
 
 
 def tune_c1(st, quick=False):
