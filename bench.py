@@ -322,6 +322,98 @@ def leg_c1(args, st):
     return out
 
 
+# ------------------------------------------------------------------ C3 skew
+
+def leg_c3(args, st):
+    """Fused select -> join -> GROUP BY with skew: R has 2^28 rows {c uniform
+    in [0, 2^31), k = Zipf(1.0) rank over 2^22 build keys mapped through a
+    seeded permutation, v in [0, 2^16)}; S has 2^22 unique keys (a seeded
+    permutation) with g in [0, 2^10).  GROUP BY S.g: COUNT, SUM(R.v) WHERE
+    R.c <= t - 1 for selectivities 1/10/50/90 %.  Direct and partitioned
+    paths are timed; checks: the group arrays of both paths agree, their
+    totals agree with a probe-sum kernel (COUNT, and SUM(R.v) = SUM(R.v +
+    S.g) - sum_g g * COUNT_g), and the first 2^22 rows agree with the oracle."""
+    nr, ns, G = 1 << 28, 1 << 22, 1 << 10
+    rc = gen_bits(nr, mgv.seed(3, 0), 0, 31, st.vp)
+    T = C.c_void_p()
+    mgv.call("mgv_zipf_table_create", C.byref(T), ns, 1.0)
+    rk = mgv.DeviceBuffer(4 * nr)
+    mgv.call("mgv_gen_zipf", rk.vp, nr, mgv.seed(3, 1), 0, T, ns, mgv.seed(3, 2), 0, st.vp)
+    rv = gen_bits(nr, mgv.seed(3, 3), 0, 16, st.vp)
+    sk = gen_perm(ns, ns, mgv.seed(3, 4), 0, st.vp)
+    sg = gen_bits(ns, mgv.seed(3, 5), 0, 10, st.vp)
+    ht = mgv.HashTable(ns, 0.5)
+    res = mgv.ResultBuffer(2)
+    mgv.call("mgv_hashjoin_build", ht.buf.vp, ht.nbuckets, sk.vp, sg.vp, mgv.MGV_PAYLOAD_COLUMN, None, 0, ns, 0,
+             res.vp, mgv.launch(0, 256), st.vp)
+    st.sync()
+    assert res.read(0).errors == 0
+    gbuf = mgv.DeviceBuffer(16 * G)
+    ws = mgv.partition_workspace(nr, 64, 2)
+    L = mgv.launch(0, 256, 1)
+    out = {"probe_rows": nr, "build_rows": ns, "groups": G, "zipf_s": 1.0, "selectivity": {}}
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py
+    orc = oracle_py.Oracle()
+    M = 1 << 22
+    h = {k: b.download(np.int32, M if k.startswith("r") else ns) for k, b in
+         (("rc", rc), ("rk", rk), ("rv", rv), ("sk", sk), ("sg", sg))}
+    otab = orc.table(h["sk"], h["sg"])
+    for sel, t in ((0.01, 21474836), (0.10, 214748365), (0.50, 1073741824), (0.90, 1932735283)):
+        bound = t - 1
+        paths = {}
+        for name in ("direct", "part64"):
+            def run():
+                mgv.call("mgv_fill_u64", gbuf.vp, 2 * G, 0, st.vp)
+                if name == "direct":
+                    mgv.call("mgv_hashjoin_probe_groupby", ht.buf.vp, ht.nbuckets, rc.vp, bound, rk.vp, rv.vp, nr,
+                             gbuf.vp, gbuf.at(8 * G), G, L, st.vp)
+                else:
+                    mgv.call("mgv_hashjoin_probe_groupby_partitioned", ht.buf.vp, ht.nbuckets, rc.vp, bound, rk.vp,
+                             rv.vp, nr, gbuf.vp, gbuf.at(8 * G), G, 64, ws.vp, ws.nbytes, L, st.vp)
+            times = []
+            for rep in range(3):
+                tm = Timer(st.vp)
+                tm.around(run)
+                st.sync()
+                times.append(tm.durations_ms()[0])
+            paths[name] = (min(times[1:]), gbuf.download(np.uint64, G), gbuf.download(np.int64, G, 8 * G))
+        cnt, gs = paths["direct"][1], paths["direct"][2]
+        same = bool(np.array_equal(cnt, paths["part64"][1]) and np.array_equal(gs, paths["part64"][2]))
+        # probe-sum totals: COUNT and SUM(R.v + S.g)
+        res.clear(st.vp)
+        mgv.call("mgv_hashjoin_probe", ht.buf.vp, ht.nbuckets, rc.vp, bound, rk.vp, rv.vp, nr, 0, 0, res.slot(1), L,
+                 st.vp)
+        st.sync()
+        ps = res.read(1)
+        gsum_g = int((np.arange(G, dtype=np.int64) * cnt.astype(np.int64)).sum())
+        totals_ok = int(cnt.sum()) == ps.count and int(gs.sum()) == ps.sum - gsum_g
+        # oracle on the first M rows
+        wc, wsum = orc.probe_groupby(otab, h["rk"], h["rv"], h["rc"], bound, G)
+        mgv.call("mgv_fill_u64", gbuf.vp, 2 * G, 0, st.vp)
+        mgv.call("mgv_hashjoin_probe_groupby", ht.buf.vp, ht.nbuckets, rc.vp, bound, rk.vp, rv.vp, M, gbuf.vp,
+                 gbuf.at(8 * G), G, L, st.vp)
+        st.sync()
+        oracle_ok = bool(np.array_equal(gbuf.download(np.uint64, G), wc) and
+                         np.array_equal(gbuf.download(np.int64, G, 8 * G), wsum))
+        g8 = 1.0 - (1.0 - sel) ** 8
+        algo = nr * (4 + 8 * g8 + 8 * sel)
+        best = min(("direct", "part64"), key=lambda n: paths[n][0])
+        ms = paths[best][0]
+        out["selectivity"][f"{int(sel * 100)}%"] = {
+            "bound": bound, "best_path": best, "ms": ms, "grows_per_s": nr / (ms / 1e3) / 1e9,
+            "algorithmic_gbps": algo / (ms / 1e3) / 1e9, "frac": algo / (ms / 1e3) / 1e9 / HBM_PEAK_GBPS,
+            "ms_by_path": {n: paths[n][0] for n in paths}, "matches": int(cnt.sum()), "sum_v": int(gs.sum()),
+            "paths_equal": same, "totals_equal_probe_sum": bool(totals_ok), "oracle_equal_first_2^22": oracle_ok}
+        log(f"C3 s={sel:.0%}: {best} {ms:.3f} ms ({nr / ms / 1e6:.1f} Grows/s), paths equal {same}, "
+            f"totals {totals_ok}, oracle {oracle_ok}")
+    orc.free(otab)
+    for b in (rc, rk, rv, sk, sg, ht.buf, res, gbuf, ws):
+        b.close()
+    mgv.load().mgv_free(T)
+    return out
+
+
 # ------------------------------------------------------------------ C4 batch
 
 def tuned(workload: str):
@@ -500,7 +592,7 @@ def main():
                     help="probe path (auto = time the candidates, keep the fastest)")
     ap.add_argument("--fanout", type=int, default=-1, help="fan-out of the partitioned paths (with --probe)")
     ap.add_argument("--scatter", type=int, default=-1, help="scatter workgroup shape MGV_SCATTER_* (-1 = tuned)")
-    ap.add_argument("--legs", default="c1,c4,c5")
+    ap.add_argument("--legs", default="c1,c3,c4,c5")
     ap.add_argument("--c5-requests", type=int, default=300)
     ap.add_argument("--c4-reps", type=int, default=5)
     ap.add_argument("--cpu", type=int, default=1)
@@ -523,6 +615,8 @@ def main():
     legs = [x for x in args.legs.split(",") if x]
     if "c1" in legs:
         extra["c1"] = leg_c1(args, st)
+    if "c3" in legs:
+        extra["c3"] = leg_c3(args, st)
     if "c4" in legs:
         mgv.load_ops()
         extra["c4"] = leg_c4(args, st)

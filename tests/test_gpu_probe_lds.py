@@ -133,3 +133,51 @@ def test_lds_probe_few_blocks_and_rejects_unsplittable_tables(gpu, orc):
         mgv.call("mgv_probe_lds_workspace_size", 1000, 25, 2, C.byref(need))     # odd bucket count
     with pytest.raises(mgv.MgvError):
         mgv.call("mgv_probe_lds_workspace_size", 1000, 1 << 24, 64, C.byref(need))  # slices above 2048 buckets
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+def test_scatter_shapes_give_identical_results(gpu, orc, shape):
+    """Every scatter workgroup shape (MGV_LAUNCH_SCATTER) partitions the same
+    rows: the LDS probe, the partitioned probe and the partitioned group-by
+    give the oracle's results with each."""
+    nb_keys, n = 1 << 16, (1 << 20) + 333
+    sk = orc.gen_perm(nb_keys, nb_keys, orc.seed(2, 0), 11)
+    sw = orc.gen_bits(nb_keys, orc.seed(2, 1), 11, 16)
+    rk = orc.gen_range(n, orc.seed(2, 2), 11, 2 * nb_keys)
+    rv = orc.gen_bits(n, orc.seed(2, 3), 11, 16)
+    c = orc.gen_bits(n, orc.seed(3, 0), 11, 31)
+    bound = (1 << 30) - 1
+    ht = build(sk, sw, 0.5)
+    t = orc.table(sk, sw)
+    want = orc.probe(t, rk, rv, pred=c, bound=bound)
+    orc.free(t)
+    flags = shape << 8
+    # LDS probe
+    need = C.c_uint64()
+    mgv.call("mgv_probe_lds_workspace_size", n, ht.nbuckets, 16, C.byref(need))
+    ws = mgv.DeviceBuffer(need.value)
+    dk, dv, dc = dev(rk), dev(rv), dev(c)
+    res = mgv.ResultBuffer(2)
+    mgv.call("mgv_hashjoin_probe_lds", ht.buf.vp, ht.nbuckets, dc.vp, bound, dk.vp, dv.vp, n, 16, ws.vp, ws.nbytes,
+             res.slot(0), mgv.Launch(0, 1024, -1, flags), None)
+    # partitioned probe
+    ws2 = mgv.partition_workspace(n, 128, 2)
+    mgv.call("mgv_hashjoin_probe_partitioned", ht.buf.vp, ht.nbuckets, dc.vp, bound, dk.vp, dv.vp, n, 0, 0, 128,
+             ws2.vp, ws2.nbytes, res.slot(1), mgv.Launch(0, 256, -1, flags), None)
+    mgv.sync()
+    for i in range(2):
+        r = res.read(i)
+        assert (r.count, r.sum) == (want["count"], want["sum"]), (i, shape)
+    # partitioned group-by
+    G = 20000
+    gk = orc.gen_range(n, orc.seed(5, 0), 11, G)
+    dg = dev(gk)
+    g = mgv.DeviceBuffer(16 * G)
+    mgv.call("mgv_fill_u64", g.vp, 2 * G, 0, None)
+    mgv.call("mgv_groupby_workspace_size", n, G, C.byref(need))
+    ws3 = mgv.DeviceBuffer(need.value)
+    mgv.call("mgv_groupby_partitioned", dg.vp, dv.vp, n, g.vp, g.at(8 * G), G, ws3.vp, ws3.nbytes, None,
+             mgv.Launch(0, 1024, -1, flags), None)
+    mgv.sync()
+    wc, wsum = orc.groupby(gk, rv, G)
+    assert np.array_equal(g.download(np.uint64, G), wc) and np.array_equal(g.download(np.int64, G, 8 * G), wsum)
