@@ -20,17 +20,11 @@
 
 namespace mgv {
 
-template <class Op>
+// NC: the chain reads columns [0, NC) (see load_rows_full).
+template <int NC, class Op>
 __device__ __forceinline__ void run_rows(Op &op, const Cols &cols, uint64_t lo, uint64_t hi, int64_t row_base) {
     op.consume_open();
-    Rows cur;
-    if (lo < hi) load_rows(cur, cols, lo, hi, row_base);
-    for (uint64_t base = lo; base < hi; base += kChunk) {
-        Rows nxt;
-        if (base + kChunk < hi) load_rows(nxt, cols, base + kChunk, hi, row_base);
-        op.consume(cur);
-        cur = nxt;
-    }
+    drive_chunks<NC>(op, cols, lo, kChunk, hi, row_base);
     op.consume_close();
 }
 
@@ -45,14 +39,14 @@ __device__ __forceinline__ void run_probe_sum(const mgv_query &q, uint64_t lo, u
         op.parent.nbuckets = q.nbuckets;
         op.parent.parent.res = q.res;
         Cols cols = {{q.pred, q.key, q.val, nullptr}, DIGEST ? 3 : -1};
-        run_rows(op, cols, lo, hi, q.row_base);
+        run_rows<3>(op, cols, lo, hi, q.row_base);
     } else {
         OpProbe<Agg> op;
         op.slots = q.slots;
         op.nbuckets = q.nbuckets;
         op.parent.res = q.res;
         Cols cols = {{q.key, q.val, nullptr, nullptr}, DIGEST ? 2 : -1};
-        run_rows(op, cols, lo, hi, q.row_base);
+        run_rows<2>(op, cols, lo, hi, q.row_base);
     }
 }
 
@@ -70,7 +64,7 @@ __device__ __forceinline__ void run_probe_group(const mgv_query &q, uint64_t lo,
         g.ngroups = q.ngroups;
         g.res = q.res;
         Cols cols = {{q.pred, q.key, q.val, nullptr}, -1};
-        run_rows(op, cols, lo, hi, q.row_base);
+        run_rows<3>(op, cols, lo, hi, q.row_base);
     } else {
         OpProbe<OpGroupWave> op;
         op.slots = q.slots;
@@ -80,7 +74,7 @@ __device__ __forceinline__ void run_probe_group(const mgv_query &q, uint64_t lo,
         op.parent.ngroups = q.ngroups;
         op.parent.res = q.res;
         Cols cols = {{q.key, q.val, nullptr, nullptr}, -1};
-        run_rows(op, cols, lo, hi, q.row_base);
+        run_rows<2>(op, cols, lo, hi, q.row_base);
     }
 }
 
@@ -96,14 +90,14 @@ __device__ __forceinline__ void run_build(const mgv_query &q, uint64_t lo, uint6
         op.parent.nbuckets = q.nbuckets;
         op.parent.res = q.res;
         Cols cols = {{q.pred, q.key, pay, nullptr}, rowid ? 2 : -1};
-        run_rows(op, cols, lo, hi, q.row_base);
+        run_rows<3>(op, cols, lo, hi, q.row_base);
     } else {
         OpBuild<W> op;
         op.slots = q.slots;
         op.nbuckets = q.nbuckets;
         op.res = q.res;
         Cols cols = {{q.key, pay, nullptr, nullptr}, rowid ? 1 : -1};
-        run_rows(op, cols, lo, hi, q.row_base);
+        run_rows<2>(op, cols, lo, hi, q.row_base);
     }
 }
 
@@ -114,7 +108,7 @@ __device__ __forceinline__ void run_select_sum(const mgv_query &q, uint64_t lo, 
     op.bound = q.pred ? q.bound : 0x7fffffff;
     op.parent.res = q.res;
     Cols cols = {{q.pred ? q.pred : q.val, q.val, nullptr, nullptr}, -1};
-    run_rows(op, cols, lo, hi, q.row_base);
+    run_rows<2>(op, cols, lo, hi, q.row_base);
 }
 
 __device__ __forceinline__ void run_groupby(const mgv_query &q, uint64_t lo, uint64_t hi) {
@@ -124,7 +118,7 @@ __device__ __forceinline__ void run_groupby(const mgv_query &q, uint64_t lo, uin
     op.ngroups = q.ngroups;
     op.res = q.res;
     Cols cols = {{q.key, q.val, nullptr, nullptr}, -1};
-    run_rows(op, cols, lo, hi, q.row_base);
+    run_rows<2>(op, cols, lo, hi, q.row_base);
 }
 
 template <int BLOCK, bool COMPACT>

@@ -111,11 +111,11 @@ struct OpAggregate {
                 D ^= sm->part[3][i];
             }
             if (A) {
-                atomicAdd((unsigned long long *)&res->count, (unsigned long long)A);
-                atomicAdd((unsigned long long *)&res->sum, (unsigned long long)B);
+                atomic_add_global((unsigned long long *)&res->count, (unsigned long long)A);
+                atomic_add_global((unsigned long long *)&res->sum, (unsigned long long)B);
                 if (DIGEST) {
-                    atomicAdd((unsigned long long *)&res->digest_sum, (unsigned long long)C);
-                    atomicXor((unsigned long long *)&res->digest_xor, (unsigned long long)D);
+                    atomic_add_global((unsigned long long *)&res->digest_sum, (unsigned long long)C);
+                    atomic_xor_global((unsigned long long *)&res->digest_xor, (unsigned long long)D);
                 }
             }
         }
@@ -164,8 +164,8 @@ struct OpGroupBy {
                         atomicAdd(&sm->cnt[g], 1u);
                         atomicAdd(&sm->sum[g], (unsigned long long)v);
                     } else {
-                        atomicAdd((unsigned long long *)&gcnt[g], 1ull);
-                        atomicAdd((unsigned long long *)&gsum[g], (unsigned long long)v);
+                        atomic_add_global((unsigned long long *)&gcnt[g], 1ull);
+                        atomic_add_global((unsigned long long *)&gsum[g], (unsigned long long)v);
                     }
                 } else {
                     ++errs;
@@ -183,8 +183,8 @@ struct OpGroupBy {
                     atomicAdd(&sm->cnt[g], 1u);
                     atomicAdd(&sm->sum[g], (unsigned long long)(int64_t)c1);
                 } else {
-                    atomicAdd((unsigned long long *)&gcnt[g], 1ull);
-                    atomicAdd((unsigned long long *)&gsum[g], (unsigned long long)(int64_t)c1);
+                    atomic_add_global((unsigned long long *)&gcnt[g], 1ull);
+                    atomic_add_global((unsigned long long *)&gsum[g], (unsigned long long)(int64_t)c1);
                 }
             } else {
                 ++errs;
@@ -198,13 +198,13 @@ struct OpGroupBy {
             for (uint32_t g = threadIdx.x; g < ngroups; g += blockDim.x) {
                 uint32_t c = sm->cnt[g];
                 if (c) {
-                    atomicAdd((unsigned long long *)&gcnt[g], (unsigned long long)c);
-                    atomicAdd((unsigned long long *)&gsum[g], sm->sum[g]);
+                    atomic_add_global((unsigned long long *)&gcnt[g], (unsigned long long)c);
+                    atomic_add_global((unsigned long long *)&gsum[g], sm->sum[g]);
                 }
             }
         }
         uint64_t e = wave_sum_u64(errs);
-        if (res && lane_id() == 0 && e) atomicAdd((unsigned long long *)&res->errors, (unsigned long long)e);
+        if (res && lane_id() == 0 && e) atomic_add_global((unsigned long long *)&res->errors, (unsigned long long)e);
         __syncthreads();
     }
 };
@@ -261,11 +261,11 @@ struct OpAggWave {
         const uint64_t a = wave_sum_u64(cnt), b = wave_sum_u64(sum);
         const uint64_t c = DIGEST ? wave_sum_u64(dsum) : 0, d = DIGEST ? wave_xor_u64(dxor) : 0;
         if (lane_id() == 0 && a) {
-            atomicAdd((unsigned long long *)&res->count, (unsigned long long)a);
-            atomicAdd((unsigned long long *)&res->sum, (unsigned long long)b);
+            atomic_add_global((unsigned long long *)&res->count, (unsigned long long)a);
+            atomic_add_global((unsigned long long *)&res->sum, (unsigned long long)b);
             if (DIGEST) {
-                atomicAdd((unsigned long long *)&res->digest_sum, (unsigned long long)c);
-                atomicXor((unsigned long long *)&res->digest_xor, (unsigned long long)d);
+                atomic_add_global((unsigned long long *)&res->digest_sum, (unsigned long long)c);
+                atomic_xor_global((unsigned long long *)&res->digest_xor, (unsigned long long)d);
             }
         }
     }
@@ -290,8 +290,8 @@ struct OpGroupWave {
             if ((r.m >> j) & 1u) {
                 const uint32_t g = (uint32_t)q_get(r.c[0], j);
                 if (g < ngroups) {
-                    atomicAdd((unsigned long long *)&gcnt[g], 1ull);
-                    atomicAdd((unsigned long long *)&gsum[g], (unsigned long long)(int64_t)q_get(r.c[1], j));
+                    atomic_add_global((unsigned long long *)&gcnt[g], 1ull);
+                    atomic_add_global((unsigned long long *)&gsum[g], (unsigned long long)(int64_t)q_get(r.c[1], j));
                 } else {
                     ++errs;
                 }
@@ -304,8 +304,8 @@ struct OpGroupWave {
         const uint32_t g = (uint32_t)c0;
         if (valid) {
             if (g < ngroups) {
-                atomicAdd((unsigned long long *)&gcnt[g], 1ull);
-                atomicAdd((unsigned long long *)&gsum[g], (unsigned long long)(int64_t)c1);
+                atomic_add_global((unsigned long long *)&gcnt[g], 1ull);
+                atomic_add_global((unsigned long long *)&gsum[g], (unsigned long long)(int64_t)c1);
             } else {
                 ++errs;
             }
@@ -314,7 +314,7 @@ struct OpGroupWave {
 
     __device__ void consume_close() {
         const uint64_t e = wave_sum_u64(errs);
-        if (res && lane_id() == 0 && e) atomicAdd((unsigned long long *)&res->errors, (unsigned long long)e);
+        if (res && lane_id() == 0 && e) atomic_add_global((unsigned long long *)&res->errors, (unsigned long long)e);
     }
 };
 
@@ -350,11 +350,11 @@ struct OpMaterialize {
         if (tot == 0) return;
         unsigned long long base = 0;
         if (lane_id() == 0) {
-            base = atomicAdd((unsigned long long *)&res->count, (unsigned long long)tot);
+            base = atomic_add_global((unsigned long long *)&res->count, (unsigned long long)tot);
             unsigned long long end = base + tot;
             if (end > cap) {
                 unsigned long long lost = end - (base > cap ? base : cap);
-                atomicAdd((unsigned long long *)&res->overflow, lost);
+                atomic_add_global((unsigned long long *)&res->overflow, lost);
             }
         }
         base = __builtin_amdgcn_readlane((uint32_t)base, 0) |
@@ -378,9 +378,9 @@ struct OpMaterialize {
         if (tot == 0) return;
         unsigned long long base = 0;
         if (lane_id() == 0) {
-            base = atomicAdd((unsigned long long *)&res->count, (unsigned long long)tot);
+            base = atomic_add_global((unsigned long long *)&res->count, (unsigned long long)tot);
             unsigned long long end = base + tot;
-            if (end > cap) atomicAdd((unsigned long long *)&res->overflow, end - (base > cap ? base : cap));
+            if (end > cap) atomic_add_global((unsigned long long *)&res->overflow, end - (base > cap ? base : cap));
         }
         base = __builtin_amdgcn_readlane((uint32_t)base, 0) |
                ((unsigned long long)__builtin_amdgcn_readlane((uint32_t)(base >> 32), 0) << 32);
@@ -455,7 +455,7 @@ struct OpProbe {
         if (pend) {
             const v4u *p = reinterpret_cast<const v4u *>(slots + (uint64_t)b * 8);
 #pragma unroll
-            for (int t = 0; t < 4; ++t) q[t] = p[t];
+            for (int t = 0; t < 4; ++t) q[t] = load_global(p + t);
         } else {
 #pragma unroll
             for (int t = 0; t < 4; ++t) q[t] = v4u{~0u, ~0u, ~0u, ~0u};
@@ -572,7 +572,7 @@ struct OpProbeCoop {
             v4u q[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                if ((mv >> j) & 1u) q[j] = *reinterpret_cast<const v4u *>(slots + (uint64_t)b[j] * 8 + 2 * sub);
+                if ((mv >> j) & 1u) q[j] = load_global(reinterpret_cast<const v4u *>(slots + (uint64_t)b[j] * 8 + 2 * sub));
                 else q[j] = v4u{~0u, ~0u, ~0u, ~0u};
             }
             uint32_t cont = 0;
@@ -587,7 +587,7 @@ struct OpProbeCoop {
                     for (uint32_t pass = 1; ballot(pend) && pass < nbuckets; ++pass) {
                         bb = (bb + 1 == nbuckets) ? 0 : bb + 1;
                         v4u qq = v4u{~0u, ~0u, ~0u, ~0u};
-                        if (pend) qq = *reinterpret_cast<const v4u *>(slots + (uint64_t)bb * 8 + 2 * sub);
+                        if (pend) qq = load_global(reinterpret_cast<const v4u *>(slots + (uint64_t)bb * 8 + 2 * sub));
                         pend = round<CAREFUL>(qq, pend ? key[j] : 0u, val[j], rid[j], grp, sub) && pend;
                     }
                 }
@@ -638,7 +638,7 @@ struct OpBuild {
         uint32_t s = 8;
 #pragma unroll
         for (int t = 3; t >= 0; --t) {
-            v4u w = p[t];
+            v4u w = load_global(p + t);
             if (w.z == ~0u && w.w == ~0u) s = 2 * t + 1;
             if (w.x == ~0u && w.y == ~0u) s = 2 * t;
         }
@@ -653,8 +653,8 @@ struct OpBuild {
                 s = 0;
             }
             unsigned long long old =
-                atomicCAS((unsigned long long *)(slots + (uint64_t)b * 8 + s), (unsigned long long)kEmptySlot,
-                          (unsigned long long)val);
+                atomic_cas_global((unsigned long long *)(slots + (uint64_t)b * 8 + s), (unsigned long long)kEmptySlot,
+                                  (unsigned long long)val);
             if (old == kEmptySlot) return;
             ++s;
         }
@@ -668,7 +668,7 @@ struct OpBuild {
 
     __device__ void consume_close() {
         uint64_t e = wave_sum_u64(errs);
-        if (res && lane_id() == 0 && e) atomicAdd((unsigned long long *)&res->errors, (unsigned long long)e);
+        if (res && lane_id() == 0 && e) atomic_add_global((unsigned long long *)&res->errors, (unsigned long long)e);
     }
 };
 

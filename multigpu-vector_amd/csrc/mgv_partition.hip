@@ -386,14 +386,7 @@ __global__ __launch_bounds__(BLOCK) void k_part_probe(typename Pipe::Params prm,
             const uint64_t ilo = s_start[p] + (uint64_t)(item - first) * kItemRows;
             const uint64_t pend = s_start[p] + s_size[p];
             const uint64_t ihi = (ilo + kItemRows < pend) ? ilo + kItemRows : pend;
-            Rows cur;
-            load_rows(cur, cols, ilo, ihi, 0);
-            for (uint64_t base = ilo; base < ihi; base += kChunk) {
-                Rows nxt;
-                if (base + kChunk < ihi) load_rows(nxt, cols, base + kChunk, ihi, 0);
-                op.consume(cur);
-                cur = nxt;
-            }
+            drive_chunks<Pipe::kCols>(op, cols, ilo, kChunk, ihi, 0);
         }
     }
     op.consume_close();

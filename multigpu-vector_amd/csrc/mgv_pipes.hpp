@@ -62,6 +62,90 @@ __device__ __forceinline__ void load_rows(Rows &r, const Cols &cols, uint64_t ba
     }
 }
 
+// Whole chunks (base + kChunk <= n) of a pipeline that reads columns
+// [0, NC) are loaded in two steps.  load_full issues one dwordx4 per lane and
+// column, unconditionally (a null column loads column 0 instead); finish_full,
+// called only when the chunk is about to be consumed, zeroes the null
+// columns and fills in row ids.  Nothing touches the loaded registers between
+// the two, so the next chunk's loads stay in flight while the current chunk
+// is consumed (a branch or a select right after the loads makes the compiler
+// wait for them at once).  Columns from NC on are row ids or zeros.
+template <int NC>
+__device__ __forceinline__ void load_full(Rows &r, const Cols &cols, uint64_t base) {
+    const uint64_t i0 = base + 4ull * lane_id();
+#pragma unroll
+    for (int c = 0; c < NC; ++c) r.c[c] = load_stream4((cols.p[c] ? cols.p[c] : cols.p[0]) + i0);
+}
+
+template <int NC>
+__device__ __forceinline__ void finish_full(Rows &r, const Cols &cols, uint64_t base, int64_t row_base) {
+    const int32_t b = (int32_t)(row_base + (int64_t)(base + 4ull * lane_id()));
+    const v4i ids = v4i{b, b + 1, b + 2, b + 3};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        if (c < NC) {
+            const int32_t keep = cols.p[c] ? -1 : 0;
+            const v4i x = r.c[c];
+            r.c[c] = (c == cols.rowid_col) ? ids : v4i{x.x & keep, x.y & keep, x.z & keep, x.w & keep};
+        } else {
+            r.c[c] = (c == cols.rowid_col) ? ids : v4i{0, 0, 0, 0};
+        }
+    }
+    r.m = 0xFu;
+}
+
+// Marks a whole chunk's loaded registers as read at this point.  Called right
+// after the chunk is consumed: a consumer that skips a column (no lane passed
+// the filter) would otherwise leave its load pending until the registers are
+// next written, and the wait there would also cover the loads issued since.
+template <int NC>
+__device__ __forceinline__ void settle(const Rows &r) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) asm volatile("" ::"v"(r.c[c].x), "v"(r.c[c].y), "v"(r.c[c].z), "v"(r.c[c].w));
+}
+
+// Walks the wave's chunks first, first + stride, ... below n (first, stride
+// and n wave-uniform, stride >= kChunk), consuming chunk i while chunk i+1 is
+// loaded.  Two row buffers alternate, so a buffer is never copied while its
+// loads are in flight; the loop handles a pair of whole chunks per iteration
+// and has a single exit, and its prefetches are unconditional (the last one
+// of an even count reloads the current chunk, in bounds, result unused): a
+// conditional load or a second loop exit makes the compiler wait for every
+// load before each consume.  Only the last chunk can be partial; it uses the
+// bounds-checked load_rows.
+template <int NC, class Op>
+__device__ __forceinline__ void drive_chunks(Op &op, const Cols &cols, uint64_t first, uint64_t stride, uint64_t n,
+                                             int64_t row_base) {
+    const uint64_t whole = first + kChunk <= n ? (n - kChunk - first) / stride + 1 : 0;
+    uint64_t base = first;
+    if (whole) {
+        Rows A, B;
+        load_full<NC>(A, cols, base);
+        for (uint64_t k = 2; k <= whole; k += 2) {
+            const uint64_t b1 = base + stride, b2 = b1 + stride;
+            load_full<NC>(B, cols, b1);
+            finish_full<NC>(A, cols, base, row_base);
+            op.consume(A);
+            settle<NC>(A);
+            load_full<NC>(A, cols, k < whole ? b2 : b1);
+            finish_full<NC>(B, cols, b1, row_base);
+            op.consume(B);
+            settle<NC>(B);
+            base = b2;
+        }
+        if (whole & 1) {
+            finish_full<NC>(A, cols, base, row_base);
+            op.consume(A);
+        }
+        base = first + whole * stride;
+    }
+    if (base < n) {
+        Rows T;
+        load_rows(T, cols, base, n, row_base);
+        op.consume(T);
+    }
+}
+
 // Each wave walks its chunks with a grid stride, loading chunk i+1 while it
 // consumes chunk i.
 template <class Pipe, int BLOCK>
@@ -73,19 +157,11 @@ __global__ __launch_bounds__(BLOCK) void k_pipeline(typename Pipe::Params prm, C
     Op op;
     Pipe::setup(op, prm, &sm);
     op.consume_open();
-    const uint64_t w = (uint64_t)blockIdx.x * WAVES + wave_id();
+    const uint64_t w = (uint64_t)blockIdx.x * WAVES + (uint64_t)__builtin_amdgcn_readfirstlane((int)wave_id());
     const uint64_t stride = (uint64_t)gridDim.x * WAVES * kChunk;
     // the next chunk's rows are loaded before the current chunk is consumed
     uint64_t base = w * kChunk;
-    Rows cur;
-    if (base < n) load_rows(cur, cols, base, n, row_base);
-    for (; base < n; base += stride) {
-        Rows nxt;
-        const uint64_t nb = base + stride;
-        if (nb < n) load_rows(nxt, cols, nb, n, row_base);
-        op.consume(cur);
-        cur = nxt;
-    }
+    drive_chunks<Pipe::kCols>(op, cols, base, stride, n, row_base);
     op.consume_close();
 }
 
@@ -94,6 +170,7 @@ __global__ __launch_bounds__(BLOCK) void k_pipeline(typename Pipe::Params prm, C
 // SELECT COUNT(*), SUM(c1) WHERE c0 <= bound.
 template <bool COMPACT>
 struct PSelectSum {
+    static constexpr int kCols = 2;   // pred, val
     struct Params {
         int32_t bound;
         mgv_result *res;
@@ -111,6 +188,7 @@ struct PSelectSum {
 
 // SELECT c1 WHERE c0 <= bound, materialised.
 struct PSelectMat {
+    static constexpr int kCols = 2;   // pred, val
     struct Params {
         int32_t bound;
         int32_t *out;
@@ -134,6 +212,7 @@ struct PSelectMat {
 // Build: (c0 key, c1 payload) or, with SEL, (c0 pred, c1 key, c2 payload).
 template <bool SEL>
 struct PBuild {
+    static constexpr int kCols = SEL ? 3 : 2;   // [pred], key, payload
     struct Params {
         int32_t bound;
         uint64_t *slots;
@@ -164,6 +243,7 @@ struct PBuild {
 // with SEL, (c0 pred, c1 key, c2 val, c3 rowid).
 template <bool SEL, bool COMPACT, bool DIGEST, template <class> class PROBE = OpProbe>
 struct PProbeSum {
+    static constexpr int kCols = (SEL ? 3 : 2) + (DIGEST ? 1 : 0);   // [pred], key, val, [row id]
     struct Params {
         int32_t bound;
         const uint64_t *slots;
@@ -196,6 +276,7 @@ struct PProbeSum {
 // Probe + GROUP BY payload: (c0 key, c1 val) or (c0 pred, c1 key, c2 val).
 template <bool SEL, bool COMPACT, int LDSG>
 struct PProbeGroup {
+    static constexpr int kCols = SEL ? 3 : 2;   // [pred], key, val
     struct Params {
         int32_t bound;
         const uint64_t *slots;
@@ -236,6 +317,7 @@ struct PProbeGroup {
 
 // Probe + materialise (r_row, payload): (c0 key, c1 rowid).
 struct PProbeMat {
+    static constexpr int kCols = 1;   // key (row ids computed)
     struct Params {
         const uint64_t *slots;
         uint32_t nbuckets;
@@ -261,6 +343,7 @@ struct PProbeMat {
 // GROUP BY c0: COUNT, SUM(c1).
 template <int LDSG>
 struct PGroupBy {
+    static constexpr int kCols = 2;   // key, val
     struct Params {
         uint64_t *gcnt;
         int64_t *gsum;

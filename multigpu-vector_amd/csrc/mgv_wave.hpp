@@ -103,13 +103,47 @@ __device__ __forceinline__ uint64_t wave_xor_u64(uint64_t v) {
     return r;
 }
 
-// Streaming loads that should not displace hash tables from L2 / MALL.
+// Streaming loads that should not displace hash tables from L2 / MALL.  The
+// pointers are cast to the global address space: columns are device or pinned
+// host memory, never LDS or scratch, and a pointer the compiler cannot place
+// (one read from a query descriptor) would otherwise become a flat load,
+// which also counts against the LDS counter, so every LDS wait would drain
+// all column loads in flight.
+template <class T>
+using global_ptr = __attribute__((address_space(1))) const T *;
+
 __device__ __forceinline__ v4i load_stream4(const int32_t *p) {
-    return __builtin_nontemporal_load(reinterpret_cast<const v4i *>(p));
+    return __builtin_nontemporal_load((global_ptr<v4i>)p);
 }
 
 __device__ __forceinline__ int32_t load_stream1(const int32_t *p) {
-    return __builtin_nontemporal_load(p);
+    return __builtin_nontemporal_load((global_ptr<int32_t>)p);
+}
+
+// Relaxed device-scope atomic add on global memory (atomicAdd's ordering),
+// issued as a global atomic for the reason given above.
+__device__ __forceinline__ unsigned long long atomic_add_global(unsigned long long *p, unsigned long long v) {
+    return __hip_atomic_fetch_add((__attribute__((address_space(1))) unsigned long long *)p, v, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned long long atomic_xor_global(unsigned long long *p, unsigned long long v) {
+    return __hip_atomic_fetch_xor((__attribute__((address_space(1))) unsigned long long *)p, v, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns the value found at p (atomicCAS's contract).
+__device__ __forceinline__ unsigned long long atomic_cas_global(unsigned long long *p, unsigned long long expect,
+                                                                unsigned long long v) {
+    __hip_atomic_compare_exchange_strong((__attribute__((address_space(1))) unsigned long long *)p, &expect, v,
+                                         __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return expect;
+}
+
+// Cached load of device memory (hash-table buckets) as a global load.
+template <class T>
+__device__ __forceinline__ T load_global(const T *p) {
+    return *(global_ptr<T>)p;
 }
 
 // ------------------------------------------------------------------ hashing
