@@ -129,6 +129,7 @@ typedef struct mgv_launch {
 #define MGV_SCATTER_256x4096 0u
 #define MGV_SCATTER_512x8192 1u
 #define MGV_SCATTER_1024x8192 2u
+#define MGV_SCATTER_1024x4096 3u
 #define MGV_LAUNCH_SCATTER(shape) ((uint32_t)(shape) << 8)
 
 /* ------------------------------------------------------------ datagen */

@@ -103,9 +103,9 @@ def tune_c2(st, quick=False):
     cands = []
     for blk, coop in itertools.product((256,) if quick else (64, 256, 1024), (0, 1)):
         cands.append(("direct", 1, blk, coop, 0))
-    for f, shape in itertools.product((64,) if quick else (32, 64, 128, 256), (0, 2) if quick else (0, 1, 2)):
+    for f, shape in itertools.product((64,) if quick else (32, 64, 128, 256), (0, 2) if quick else (0, 1, 2, 3)):
         cands.append(("part", f, 256, 0, shape))
-    for f, shape in itertools.product((32, 64) if quick else (16, 32, 64, 128), (0, 2) if quick else (0, 1, 2)):
+    for f, shape in itertools.product((32, 64) if quick else (16, 32, 64, 128), (0, 2) if quick else (0, 1, 2, 3)):
         cands.append(("lds", f, 1024, 0, shape))
     need = 0
     for mode, f, _, _, _ in cands:

@@ -56,7 +56,7 @@ struct OpAggregate {
     mgv_result *res;
     uint64_t cnt, sum, dsum, dxor;
 
-    __device__ void consume_open() { cnt = sum = dsum = dxor = 0; }
+    __device__ __forceinline__ void consume_open() { cnt = sum = dsum = dxor = 0; }
 
     __device__ __forceinline__ void consume(const Rows &r) {
         cnt += (uint64_t)__popc(r.m & 0xFu);
@@ -91,7 +91,7 @@ struct OpAggregate {
         }
     }
 
-    __device__ void consume_close() {
+    __device__ __forceinline__ void consume_close() {
         const uint32_t w = wave_id(), l = lane_id();
         uint64_t a = wave_sum_u64(cnt), b = wave_sum_u64(sum);
         uint64_t c = DIGEST ? wave_sum_u64(dsum) : 0, d = DIGEST ? wave_xor_u64(dxor) : 0;
@@ -142,7 +142,7 @@ struct OpGroupBy {
     mgv_result *res;
     uint64_t errs;
 
-    __device__ void consume_open() {
+    __device__ __forceinline__ void consume_open() {
         errs = 0;
         if (LDS_GROUPS > 0) {
             for (uint32_t g = threadIdx.x; g < ngroups; g += blockDim.x) {
@@ -161,8 +161,8 @@ struct OpGroupBy {
                 int64_t v = (int64_t)q_get(r.c[1], j);
                 if (g < ngroups) {
                     if (LDS_GROUPS > 0) {
-                        atomicAdd(&sm->cnt[g], 1u);
-                        atomicAdd(&sm->sum[g], (unsigned long long)v);
+                        atomic_add_lds(&sm->cnt[g], 1u);
+                        atomic_add_lds(&sm->sum[g], (unsigned long long)v);
                     } else {
                         atomic_add_global((unsigned long long *)&gcnt[g], 1ull);
                         atomic_add_global((unsigned long long *)&gsum[g], (unsigned long long)v);
@@ -180,8 +180,8 @@ struct OpGroupBy {
         if (valid) {
             if (g < ngroups) {
                 if (LDS_GROUPS > 0) {
-                    atomicAdd(&sm->cnt[g], 1u);
-                    atomicAdd(&sm->sum[g], (unsigned long long)(int64_t)c1);
+                    atomic_add_lds(&sm->cnt[g], 1u);
+                    atomic_add_lds(&sm->sum[g], (unsigned long long)(int64_t)c1);
                 } else {
                     atomic_add_global((unsigned long long *)&gcnt[g], 1ull);
                     atomic_add_global((unsigned long long *)&gsum[g], (unsigned long long)(int64_t)c1);
@@ -192,7 +192,7 @@ struct OpGroupBy {
         }
     }
 
-    __device__ void consume_close() {
+    __device__ __forceinline__ void consume_close() {
         if (LDS_GROUPS > 0) {
             __syncthreads();
             for (uint32_t g = threadIdx.x; g < ngroups; g += blockDim.x) {
@@ -222,7 +222,7 @@ struct OpAggWave {
     mgv_result *res;
     uint64_t cnt, sum, dsum, dxor;
 
-    __device__ void consume_open() { cnt = sum = dsum = dxor = 0; }
+    __device__ __forceinline__ void consume_open() { cnt = sum = dsum = dxor = 0; }
 
     __device__ __forceinline__ void consume(const Rows &r) {
         cnt += (uint64_t)__popc(r.m & 0xFu);
@@ -257,7 +257,7 @@ struct OpAggWave {
         }
     }
 
-    __device__ void consume_close() {
+    __device__ __forceinline__ void consume_close() {
         const uint64_t a = wave_sum_u64(cnt), b = wave_sum_u64(sum);
         const uint64_t c = DIGEST ? wave_sum_u64(dsum) : 0, d = DIGEST ? wave_xor_u64(dxor) : 0;
         if (lane_id() == 0 && a) {
@@ -282,7 +282,7 @@ struct OpGroupWave {
     mgv_result *res;
     uint64_t errs;
 
-    __device__ void consume_open() { errs = 0; }
+    __device__ __forceinline__ void consume_open() { errs = 0; }
 
     __device__ __forceinline__ void consume(const Rows &r) {
 #pragma unroll
@@ -312,7 +312,7 @@ struct OpGroupWave {
         }
     }
 
-    __device__ void consume_close() {
+    __device__ __forceinline__ void consume_close() {
         const uint64_t e = wave_sum_u64(errs);
         if (res && lane_id() == 0 && e) atomic_add_global((unsigned long long *)&res->errors, (unsigned long long)e);
     }
@@ -336,7 +336,7 @@ struct OpMaterialize {
     uint64_t cap;
     mgv_result *res;
 
-    __device__ void consume_open() {}
+    __device__ __forceinline__ void consume_open() {}
 
     __device__ __forceinline__ void consume(const Rows &r) {
         uint32_t off[4];
@@ -392,7 +392,7 @@ struct OpMaterialize {
         }
     }
 
-    __device__ void consume_close() {}
+    __device__ __forceinline__ void consume_close() {}
 };
 
 // ============================================================== hash join
@@ -414,8 +414,8 @@ struct OpProbe {
     const uint64_t *slots;
     uint32_t nbuckets;
 
-    __device__ void consume_open() { parent.consume_open(); }
-    __device__ void consume_close() { parent.consume_close(); }
+    __device__ __forceinline__ void consume_open() { parent.consume_open(); }
+    __device__ __forceinline__ void consume_close() { parent.consume_close(); }
 
     // Scan one bucket (8 slots in q[0..3]) for `key`; emit the hits.
     // Returns whether the bucket is full (slot 7 used).  Rows that are not
@@ -524,8 +524,8 @@ struct OpProbeCoop {
     const uint64_t *slots;
     uint32_t nbuckets;
 
-    __device__ void consume_open() { parent.consume_open(); }
-    __device__ void consume_close() { parent.consume_close(); }
+    __device__ __forceinline__ void consume_open() { parent.consume_open(); }
+    __device__ __forceinline__ void consume_close() { parent.consume_close(); }
 
     __device__ __forceinline__ static int32_t bperm(uint32_t src_lane, int32_t v) {
         return __builtin_amdgcn_ds_bpermute((int)(src_lane << 2), v);
@@ -624,7 +624,7 @@ struct OpBuild {
     mgv_result *res;
     uint64_t errs;
 
-    __device__ void consume_open() { errs = 0; }
+    __device__ __forceinline__ void consume_open() { errs = 0; }
 
     __device__ __forceinline__ void insert(uint32_t key, uint32_t pay) {
         const uint64_t val = ((uint64_t)pay << 32) | key;
@@ -666,7 +666,7 @@ struct OpBuild {
             if ((r.m >> j) & 1u) insert((uint32_t)q_get(r.c[0], j), (uint32_t)q_get(r.c[1], j));
     }
 
-    __device__ void consume_close() {
+    __device__ __forceinline__ void consume_close() {
         uint64_t e = wave_sum_u64(errs);
         if (res && lane_id() == 0 && e) atomic_add_global((unsigned long long *)&res->errors, (unsigned long long)e);
     }
@@ -700,7 +700,7 @@ struct OpSelect {
     int32_t bound;
     uint32_t fill;  // wave-uniform count of staged rows
 
-    __device__ void consume_open() {
+    __device__ __forceinline__ void consume_open() {
         fill = 0;
         parent.consume_open();
     }
@@ -763,7 +763,7 @@ struct OpSelect {
         }
     }
 
-    __device__ void consume_close() {
+    __device__ __forceinline__ void consume_close() {
         if (COMPACT && fill) {
             wave_sync();
             emit_stage(fill);

@@ -219,47 +219,85 @@ __device__ __forceinline__ uint32_t scan_block_excl(uint32_t v, uint32_t *wtot) 
 
 // LDS-staged scatter: each tile of TILE rows is counting-sorted by partition
 // in LDS, then written out so that consecutive lanes write consecutive slots
-// of a partition's run.  The next tile's rows are loaded before the current
-// tile is staged, so loads overlap the LDS work.
+// of a partition's run.
+//
+// Per tile: (1) rank every row within its partition with an LDS atomic;
+// (2) the thread owning partitions [t*per, (t+1)*per) scans the tile counts,
+// publishes base[p] = (block's next output row of p) - (p's first staged
+// slot), and advances its cursors, kept in registers; (3) stage the rows in
+// partition order; (4) write staged slot i to base[p] + i.
+//
+// Overlap: the next tile's columns are loaded before the current tile is
+// ranked, and nothing reads them until the next iteration (masks, partition
+// ids and row ids are computed at consume time).  The write step issues a
+// fixed number of stores per thread (lanes past the tile's last staged row
+// repeat that row's store), so the compiler can wait for the prefetched
+// loads without also waiting for the stores, which share the same counter.
 template <bool ROWID, uint32_t MAXF, int BLOCK, int TILE>
 __global__ __launch_bounds__(BLOCK) void k_part_scatter(PartLayout L, PartIn in) {
     constexpr int Q = TILE / (4 * BLOCK);   // quads per thread per tile
+    constexpr int R = TILE / BLOCK;         // write rounds per tile
+    constexpr int PER = MAXF > (uint32_t)BLOCK ? (int)(MAXF / BLOCK) : 1;   // partitions per owner thread
     static_assert(Q >= 1 && TILE % (4 * BLOCK) == 0 && kTileMax % TILE == 0, "scatter shape");
-    __shared__ unsigned long long cursor[MAXF];
+    __shared__ unsigned long long base[MAXF];
     __shared__ uint32_t tcnt[MAXF];
     __shared__ uint32_t toff[MAXF];
     __shared__ uint32_t wtot[BLOCK / kWave];
+    __shared__ uint32_t s_total;
     __shared__ int32_t sk[TILE];
     __shared__ int32_t sv[TILE];
     __shared__ int32_t sr[ROWID ? TILE : 1];
     const uint32_t F = L.F;
     const uint32_t t = threadIdx.x;
-    for (uint32_t p = t; p < F; p += BLOCK) {
-        uint32_t c = L.hist[(uint64_t)blockIdx.x * F + p];
-        cursor[p] = c ? atomicAdd(&L.pcursor[p], (unsigned long long)c) : 0;
-        tcnt[p] = 0;
+    const uint32_t per = (F + BLOCK - 1) / BLOCK;   // <= PER
+    unsigned long long cur[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        cur[e] = 0;
+        const uint32_t p = t * per + e;
+        if (e < (int)per && p < F) {
+            const uint32_t c = L.hist[(uint64_t)blockIdx.x * F + p];
+            cur[e] = c ? atomic_add_global(&L.pcursor[p], (unsigned long long)c) : 0;
+            tcnt[p] = 0;
+        }
     }
     const uint64_t lo = (uint64_t)blockIdx.x * L.rows_per_blk;
     uint64_t hi = min(L.n, lo + L.rows_per_blk);
     if (in.nlimit) hi = min(hi, (uint64_t)*in.nlimit);
-    v4i nk[Q], nv[Q];
-    uint32_t nm[Q];
+    v4i nk[Q], nv[Q], nc[Q];
+    uint32_t nb[Q];   // in-bounds rows of each quad
     auto load_tile = [&](uint64_t tb) {
+        if (tb + TILE <= hi) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const uint64_t i0 = tb + 4ull * t + (uint64_t)q * 4 * BLOCK;
-            v4i c;
-            nm[q] = (i0 < hi) ? quad_mask(in, i0, hi, &c) : 0u;
-            nk[q] = v4i{0, 0, 0, 0};
-            nv[q] = v4i{0, 0, 0, 0};
-            if (nm[q]) {
+            for (int q = 0; q < Q; ++q) {
+                const uint64_t i0 = tb + 4ull * t + (uint64_t)q * 4 * BLOCK;
+                nb[q] = 0xfu;
+                nk[q] = load_stream4(in.key + i0);
+                nv[q] = in.val ? load_stream4(in.val + i0) : v4i{0, 0, 0, 0};
+                nc[q] = in.pred ? load_stream4(in.pred + i0) : v4i{0, 0, 0, 0};
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const uint64_t i0 = tb + 4ull * t + (uint64_t)q * 4 * BLOCK;
+                uint32_t m = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) m |= (uint32_t)(i0 + j < hi) << j;
+                nb[q] = m;
                 nk[q] = load_quad(in.key, i0, hi);
-                nm[q] &= key_mask(in, nk[q]);
-                if (in.val) nv[q] = load_quad(in.val, i0, hi);
+                nv[q] = in.val ? load_quad(in.val, i0, hi) : v4i{0, 0, 0, 0};
+                nc[q] = in.pred ? load_quad(in.pred, i0, hi) : v4i{0, 0, 0, 0};
             }
         }
     };
     if (lo < hi) load_tile(lo);
+    // The first tile is waited for here: entering the loop with its loads
+    // pending would make the compiler wait for the previous iteration's
+    // stores as well at the top of every iteration.
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        asm volatile("" ::"v"(nk[q].x), "v"(nk[q].y), "v"(nk[q].z), "v"(nk[q].w), "v"(nv[q].x), "v"(nv[q].y),
+                     "v"(nv[q].z), "v"(nv[q].w), "v"(nc[q].x), "v"(nc[q].y), "v"(nc[q].z), "v"(nc[q].w));
     __syncthreads();
     for (uint64_t tb = lo; tb < hi; tb += TILE) {
         v4i ck[Q], cv[Q];
@@ -268,64 +306,95 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(PartLayout L, PartIn in)
         for (int q = 0; q < Q; ++q) {
             ck[q] = nk[q];
             cv[q] = nv[q];
-            cm[q] = nm[q];
+            uint32_t m = nb[q];
+            if (in.pred) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) m &= ~((uint32_t)(q_get(nc[q], j) > in.bound) << j);
+            }
+            cm[q] = m & key_mask(in, ck[q]);
         }
         if (tb + TILE < hi) load_tile(tb + TILE);
         // 1. rank every row within its partition
-        uint32_t rank[Q][4];
+        uint32_t part[Q][4], rank[Q][4];
 #pragma unroll
         for (int q = 0; q < Q; ++q)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                rank[q][j] = ((cm[q] >> j) & 1u) ? atomicAdd(&tcnt[part_key(in, q_get(ck[q], j), F)], 1u) : 0u;
+            for (int j = 0; j < 4; ++j) {
+                // masked rows get partition 0 (a dropped key may map past F)
+                const bool ok = (cm[q] >> j) & 1u;
+                part[q][j] = ok ? part_key(in, q_get(ck[q], j), F) : 0u;
+                rank[q][j] = ok ? atomicAdd(&tcnt[part[q][j]], 1u) : 0u;
+            }
         __syncthreads();
-        // 2. exclusive scan of the tile counts (<= 2 partitions per thread)
-        const uint32_t per = (F + BLOCK - 1) / BLOCK;
+        // 2. owners scan the tile counts, publish toff / base, advance cursors
+        uint32_t c[PER];
         uint32_t mine = 0;
-        for (uint32_t e = 0; e < per; ++e) {
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
             const uint32_t p = t * per + e;
-            mine += (p < F) ? tcnt[p] : 0;
+            c[e] = (e < (int)per && p < F) ? tcnt[p] : 0u;
+            mine += c[e];
         }
-        uint32_t base = scan_block_excl<BLOCK>(mine, wtot);
-        for (uint32_t e = 0; e < per; ++e) {
+        uint32_t start = scan_block_excl<BLOCK>(mine, wtot);
+        if (t == BLOCK - 1) s_total = start + mine;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
             const uint32_t p = t * per + e;
-            if (p < F) {
-                toff[p] = base;
-                base += tcnt[p];
+            if (e < (int)per && p < F) {
+                toff[p] = start;
+                base[p] = cur[e] - start;
+                cur[e] += c[e];
+                tcnt[p] = 0;
+                start += c[e];
             }
         }
         __syncthreads();
-        const uint32_t total = toff[F - 1] + tcnt[F - 1];
-        // 3. stage rows in partition order
+        // 3. stage rows in partition order (all slot reads first, then writes)
+        uint32_t pos[Q][4];
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) pos[q][j] = toff[part[q][j]] + rank[q][j];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if ((cm[q] >> j) & 1u) {
-                    const int32_t k = q_get(ck[q], j);
-                    const uint32_t pos = toff[part_key(in, k, F)] + rank[q][j];
-                    sk[pos] = k;
-                    sv[pos] = q_get(cv[q], j);
+                    sk[pos[q][j]] = q_get(ck[q], j);
+                    sv[pos[q][j]] = q_get(cv[q], j);
                     if (ROWID)
-                        sr[pos] = (int32_t)(in.row_base +
-                                            (int64_t)(tb + 4ull * t + (uint64_t)q * 4 * BLOCK + j));
+                        sr[pos[q][j]] = (int32_t)(in.row_base +
+                                                  (int64_t)(tb + 4ull * t + (uint64_t)q * 4 * BLOCK + j));
                 }
             }
         }
         __syncthreads();
-        // 4. write the runs
-        for (uint32_t i = t; i < total; i += BLOCK) {
-            const int32_t k = sk[i];
-            const uint32_t p = part_key(in, k, F);
-            const unsigned long long dst = cursor[p] + (i - toff[p]);
-            L.col[0][dst] = k;
-            if (L.ncols > 1) L.col[1][dst] = sv[i];
-            if (ROWID) L.col[2][dst] = sr[i];
+        // 4. write the runs: exactly R stores per column and thread.  An
+        // empty tile stores to slot cap - 1, which no partition reaches
+        // (partitions end at pstart[F] <= n + F * (align - 1) < cap - 1); a
+        // keys-only layout stores the key twice instead of a second column.
+        const uint32_t total = s_total;
+        int32_t k[R], v[R], rid[R];
+        uint32_t ii[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            ii[r] = total ? min(t + (uint32_t)r * BLOCK, total - 1) : 0u;
+            k[r] = sk[ii[r]];
+            v[r] = sv[ii[r]];
+            rid[r] = ROWID ? sr[ii[r]] : 0;
         }
-        __syncthreads();
-        for (uint32_t p = t; p < F; p += BLOCK) {
-            cursor[p] += tcnt[p];
-            tcnt[p] = 0;
+        unsigned long long dst[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t p = total ? part_key(in, k[r], F) : 0u;
+            dst[r] = total ? base[p] + ii[r] : L.cap - 1;
+        }
+        int32_t *const c1 = L.ncols > 1 ? L.col[1] : L.col[0];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            L.col[0][dst[r]] = k[r];
+            c1[dst[r]] = L.ncols > 1 ? v[r] : k[r];
+            if (ROWID) L.col[2][dst[r]] = rid[r];
         }
         __syncthreads();
     }
@@ -427,6 +496,9 @@ static void launch_scatter(const PartLayout &L, const PartIn &in, uint32_t shape
             break;
         case MGV_SCATTER_1024x8192:
             hipLaunchKernelGGL((k_part_scatter<ROWID, MAXF, 1024, 8192>), dim3(L.nblk), dim3(1024), 0, s, L, in);
+            break;
+        case MGV_SCATTER_1024x4096:
+            hipLaunchKernelGGL((k_part_scatter<ROWID, MAXF, 1024, 4096>), dim3(L.nblk), dim3(1024), 0, s, L, in);
             break;
         default:
             hipLaunchKernelGGL((k_part_scatter<ROWID, MAXF, 256, 4096>), dim3(L.nblk), dim3(256), 0, s, L, in);

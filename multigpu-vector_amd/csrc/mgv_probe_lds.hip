@@ -30,6 +30,12 @@ namespace mgv {
 constexpr uint32_t kSliceMax = 2048;     // buckets of a slice in LDS (128 KiB)
 constexpr uint32_t kSliceExtra = 8;      // buckets loaded past the slice
 constexpr int kLdsBlock = 1024;
+// Quads (4 rows) each lane probes per step; the next step's quads are loaded
+// while these are probed.  Measured on the 2^28-row probe: 2 per step ran
+// within 1 % of 1 and 3 per step was 8 % slower (register spills), so the
+// kernel is not bound by the row loads' latency.
+constexpr int kAhead = 1;
+constexpr uint64_t kQuadStride = 4ull * kLdsBlock;   // rows between a lane's quads
 
 struct LdsProbeArgs {
     const uint64_t *slots;
@@ -103,6 +109,12 @@ __device__ __forceinline__ uint32_t queue_push(WaveQueue &Q, uint32_t tail, bool
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)(tail + (uint32_t)popc64(bal)));
 }
 
+// LDS position of 16-byte quad w of slice bucket idx.  Buckets are 64 bytes,
+// so without a swizzle quad w of every bucket falls in one of 4 bank groups
+// and the 16 lanes a ds_read_b128 serves per cycle collide; xor-ing w with
+// bucket bits 2-3 spreads quad w over 16 groups.
+__device__ __forceinline__ uint32_t tab_at(uint32_t idx, uint32_t w) { return idx * 4 + (w ^ ((idx >> 2) & 3u)); }
+
 // Probes up to 64 queued entries (one per lane) and re-queues the chains
 // that go on; returns the new head and, through tail, the new tail.
 __device__ __forceinline__ uint32_t queue_round(WaveQueue &Q, uint32_t head, uint32_t &tail, const v4u *tab,
@@ -125,7 +137,7 @@ __device__ __forceinline__ uint32_t queue_round(WaveQueue &Q, uint32_t head, uin
     const uint32_t idx = bkt - lo;
     if (mine && idx < nload) {
 #pragma unroll
-        for (int w = 0; w < 4; ++w) q[w] = tab[idx * 4 + w];
+        for (int w = 0; w < 4; ++w) q[w] = tab[tab_at(idx, w)];
     } else if (mine) {
         const v4u *g = reinterpret_cast<const v4u *>(slots + (uint64_t)bkt * 8);
 #pragma unroll
@@ -172,7 +184,7 @@ __device__ __forceinline__ void lds_probe_quad(const v4i &k, const v4i &v, uint3
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int w = 0; w < 4; ++w) q[j][w] = tab[ii[j] * 4 + w];
+        for (int w = 0; w < 4; ++w) q[j][w] = tab[tab_at(ii[j], w)];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         uint32_t hits = 0;
@@ -228,11 +240,17 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a) {
         const uint32_t lo = p * a.slice;
         const uint32_t nload = min(a.slice + kSliceExtra, nb - lo);
         // the first rows' loads overlap the slice copy
-        v4i k, v;
+        v4i k[kAhead], v[kAhead];
+        uint32_t m[kAhead];
         uint64_t i0 = r0 + 4ull * t;
-        uint32_t m = load_rows4(K, V, i0 < r1 ? i0 : r0, i0 < r1 ? r1 : r0, k, v);
+#pragma unroll
+        for (int x = 0; x < kAhead; ++x) {
+            const uint64_t ix = i0 + (uint64_t)x * kQuadStride;
+            m[x] = load_rows4(K, V, ix < r1 ? ix : r0, ix < r1 ? r1 : r0, k[x], v[x]);
+        }
         const v4u *src = reinterpret_cast<const v4u *>(a.slots + (uint64_t)lo * 8);
-        for (uint32_t i = t; i < nload * 4; i += kLdsBlock) tab[i] = src[i];
+        // physical quad i holds quad (i & 3) ^ ((i >> 4) & 3) of bucket i >> 2 (tab_at)
+        for (uint32_t i = t; i < nload * 4; i += kLdsBlock) tab[i] = src[(i & ~3u) | ((i & 3u) ^ ((i >> 4) & 3u))];
         __syncthreads();
         // r0 is a multiple of 64 rows, so quads are 16-byte aligned
         uint32_t head = 0, tail = 0;   // the wave's queue (wave-uniform counters)
@@ -240,17 +258,31 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a) {
         // the end carry an empty mask), so the queue counters and ballots stay
         // wave-uniform
         while (ballot(i0 < r1)) {
-            const uint64_t i1 = i0 + 4ull * kLdsBlock;
-            v4i kn, vn;
-            const uint32_t mn = load_rows4(K, V, i1 < r1 ? i1 : r0, i1 < r1 ? r1 : r0, kn, vn);
-            bool neg = false;
+            // the next step's quads load while this step's are probed
+            const uint64_t i1 = i0 + (uint64_t)kAhead * kQuadStride;
+            v4i kn[kAhead], vn[kAhead];
+            uint32_t mn[kAhead];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) neg |= ((m >> j) & 1u) && q_get(k, j) == -1;
-            if (ballot(neg)) lds_probe_quad<true>(k, v, m, tab, a.slots, nb, lo, nload, cnt, sum, Q, head, tail);
-            else lds_probe_quad<false>(k, v, m, tab, a.slots, nb, lo, nload, cnt, sum, Q, head, tail);
-            k = kn;
-            v = vn;
-            m = mn;
+            for (int x = 0; x < kAhead; ++x) {
+                const uint64_t ix = i1 + (uint64_t)x * kQuadStride;
+                mn[x] = load_rows4(K, V, ix < r1 ? ix : r0, ix < r1 ? r1 : r0, kn[x], vn[x]);
+            }
+#pragma unroll
+            for (int x = 0; x < kAhead; ++x) {
+                bool neg = false;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) neg |= ((m[x] >> j) & 1u) && q_get(k[x], j) == -1;
+                if (ballot(neg))
+                    lds_probe_quad<true>(k[x], v[x], m[x], tab, a.slots, nb, lo, nload, cnt, sum, Q, head, tail);
+                else
+                    lds_probe_quad<false>(k[x], v[x], m[x], tab, a.slots, nb, lo, nload, cnt, sum, Q, head, tail);
+            }
+#pragma unroll
+            for (int x = 0; x < kAhead; ++x) {
+                k[x] = kn[x];
+                v[x] = vn[x];
+                m[x] = mn[x];
+            }
             i0 = i1;
         }
         // the rest of the queue, before the slice is replaced

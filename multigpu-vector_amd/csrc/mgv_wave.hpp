@@ -140,6 +140,19 @@ __device__ __forceinline__ unsigned long long atomic_cas_global(unsigned long lo
     return expect;
 }
 
+// Atomic adds on LDS through a pointer the compiler cannot place (an
+// operator's shared-memory block is reached through a pointer member): the
+// cast makes them ds_add instead of flat atomics.
+__device__ __forceinline__ void atomic_add_lds(uint32_t *p, uint32_t v) {
+    __hip_atomic_fetch_add((__attribute__((address_space(3))) uint32_t *)p, v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void atomic_add_lds(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_fetch_add((__attribute__((address_space(3))) unsigned long long *)p, v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Cached load of device memory (hash-table buckets) as a global load.
 template <class T>
 __device__ __forceinline__ T load_global(const T *p) {

@@ -135,7 +135,7 @@ def test_lds_probe_few_blocks_and_rejects_unsplittable_tables(gpu, orc):
         mgv.call("mgv_probe_lds_workspace_size", 1000, 1 << 24, 64, C.byref(need))  # slices above 2048 buckets
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
 def test_scatter_shapes_give_identical_results(gpu, orc, shape):
     """Every scatter workgroup shape (MGV_LAUNCH_SCATTER) partitions the same
     rows: the LDS probe, the partitioned probe and the partitioned group-by
