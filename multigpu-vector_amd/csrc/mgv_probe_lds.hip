@@ -46,42 +46,69 @@ struct LdsProbeArgs {
     mgv_result *res;
 };
 
-// A table slice in LDS keeps the global layout: 64-byte buckets of 8
-// {key, payload} slots, empty = (-1, -1).
-
-// Scans one bucket for key: number of hitting slots and the sum of their
-// payloads.  CAREFUL (key == -1 somewhere in the wave) excludes the empty
-// pattern (-1, -1) from the compare.  The common case (at most one hit)
-// selects the hit's payload instead of summing per slot.
-template <bool CAREFUL>
+// Scans one global-memory bucket (64 bytes: 8 {key, payload} slots, empty =
+// (-1, -1)) for key: number of hitting slots and the sum of their payloads.
+// The empty pattern is excluded from the compare, so key -1 is handled.
 __device__ __forceinline__ void bucket_scan(const v4u (&q)[4], uint32_t key, uint32_t &hits, int64_t &psum) {
     uint32_t h = 0;
-    int32_t sel = 0;
+    int64_t t = 0;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
         const uint32_t sk = (s & 1) ? q[s >> 1].z : q[s >> 1].x;
         const uint32_t sp = (s & 1) ? q[s >> 1].w : q[s >> 1].y;
-        const bool hit = CAREFUL ? (sk == key && !(sk == ~0u && sp == ~0u)) : (sk == key);
+        const bool hit = sk == key && !(sk == ~0u && sp == ~0u);
         h += hit ? 1u : 0u;
-        sel = hit ? (int32_t)sp : sel;
+        t += hit ? (int64_t)(int32_t)sp : 0;
     }
-    if (ballot(h > 1) != 0) {   // duplicate build keys: sum every hit
-        int64_t t = 0;
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const uint32_t sk = (s & 1) ? q[s >> 1].z : q[s >> 1].x;
-            const uint32_t sp = (s & 1) ? q[s >> 1].w : q[s >> 1].y;
-            const bool hit = CAREFUL ? (sk == key && !(sk == ~0u && sp == ~0u)) : (sk == key);
-            t += hit ? (int64_t)(int32_t)sp : 0;
-        }
-        psum += t;
-    } else {
-        psum += h ? (int64_t)sel : 0;
-    }
+    psum += t;
     hits += h;
 }
 
 __device__ __forceinline__ bool bucket_full(const v4u (&q)[4]) { return !(q[3].z == ~0u && q[3].w == ~0u); }
+
+// A table slice in LDS is stored keys first: per bucket its 8 keys (two
+// 16-byte quads), its 8 payloads, and its fill count c.  The build fills a
+// bucket's slots strictly in order (a slot is claimed only after every lower
+// slot was seen occupied, and slots are never freed), so the occupied slots
+// are exactly [0, c): slot s hits key iff s < c and its key equals key (an
+// empty slot never hits, whatever the key), and the bucket is full iff
+// c == 8.  A probe row then reads 32 bytes of keys and 1 byte of count, and a
+// payload only when it hits, instead of the 64-byte bucket.
+struct Slice {
+    v4u key[(kSliceMax + kSliceExtra) * 2];
+    uint32_t pay[(kSliceMax + kSliceExtra) * 8];
+    uint8_t cnt[kSliceMax + kSliceExtra];
+};
+
+// LDS quad of key quad w (slots 4w..4w+3) of slice bucket idx.  Buckets' key
+// quads are 32 bytes apart, so quad w of every bucket would fall in one of 8
+// bank groups; xor-ing w with bucket bit 3 spreads it over 16.
+__device__ __forceinline__ uint32_t key_at(uint32_t idx, uint32_t w) { return idx * 2 + (w ^ ((idx >> 3) & 1u)); }
+
+// Slot mask of the hits of key in a slice bucket with keys k0, k1 and fill
+// count c.
+__device__ __forceinline__ uint32_t slice_hits(const v4u &k0, const v4u &k1, uint32_t c, uint32_t key) {
+    const uint32_t m = (uint32_t)(k0.x == key) | (uint32_t)(k0.y == key) << 1 | (uint32_t)(k0.z == key) << 2 |
+                       (uint32_t)(k0.w == key) << 3 | (uint32_t)(k1.x == key) << 4 | (uint32_t)(k1.y == key) << 5 |
+                       (uint32_t)(k1.z == key) << 6 | (uint32_t)(k1.w == key) << 7;
+    return m & ((1u << c) - 1u);
+}
+
+// Payload sum of the hit slots hm of slice bucket idx, given the payload of
+// the lowest hit slot (p1, read by the caller).  Duplicate build keys (two or
+// more hits in one bucket) read the other payloads here.
+__device__ __forceinline__ void slice_payloads(const Slice &S, uint32_t idx, uint32_t hm, int32_t p1, uint32_t &hits,
+                                               int64_t &psum) {
+    const uint32_t h = (uint32_t)__builtin_popcount(hm);
+    int64_t t = h ? (int64_t)p1 : 0;
+    if (ballot(h > 1)) {
+        for (uint32_t m = hm & (hm - 1); m; m &= m - 1) t += (int64_t)(int32_t)S.pay[idx * 8 + __builtin_ctz(m)];
+    }
+    psum += t;
+    hits += h;
+}
+
+__device__ __forceinline__ uint32_t lowest_slot(uint32_t hm) { return hm ? (uint32_t)__builtin_ctz(hm) : 0u; }
 
 // Rows whose chain continues past their home bucket (about 5 % at load
 // factor 0.5, but some row of nearly every wave) are not followed on the
@@ -109,15 +136,10 @@ __device__ __forceinline__ uint32_t queue_push(WaveQueue &Q, uint32_t tail, bool
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)(tail + (uint32_t)popc64(bal)));
 }
 
-// LDS position of 16-byte quad w of slice bucket idx.  Buckets are 64 bytes,
-// so without a swizzle quad w of every bucket falls in one of 4 bank groups
-// and the 16 lanes a ds_read_b128 serves per cycle collide; xor-ing w with
-// bucket bits 2-3 spreads quad w over 16 groups.
-__device__ __forceinline__ uint32_t tab_at(uint32_t idx, uint32_t w) { return idx * 4 + (w ^ ((idx >> 2) & 3u)); }
-
 // Probes up to 64 queued entries (one per lane) and re-queues the chains
-// that go on; returns the new head and, through tail, the new tail.
-__device__ __forceinline__ uint32_t queue_round(WaveQueue &Q, uint32_t head, uint32_t &tail, const v4u *tab,
+// that go on; returns the new head and, through tail, the new tail.  A
+// bucket inside the loaded range is read from LDS, any other from the table.
+__device__ __forceinline__ uint32_t queue_round(WaveQueue &Q, uint32_t head, uint32_t &tail, const Slice &S,
                                                 const uint64_t *slots, uint32_t nb, uint32_t lo, uint32_t nload,
                                                 uint64_t &cnt, uint64_t &sum) {
     const uint32_t avail = tail - head;
@@ -133,22 +155,24 @@ __device__ __forceinline__ uint32_t queue_round(WaveQueue &Q, uint32_t head, uin
         bkt = Q.bkt[pos];
     }
     wave_sync();   // every lane has read its entry before slots are reused
-    v4u q[4];
     const uint32_t idx = bkt - lo;
-    if (mine && idx < nload) {
-#pragma unroll
-        for (int w = 0; w < 4; ++w) q[w] = tab[tab_at(idx, w)];
-    } else if (mine) {
-        const v4u *g = reinterpret_cast<const v4u *>(slots + (uint64_t)bkt * 8);
-#pragma unroll
-        for (int w = 0; w < 4; ++w) q[w] = g[w];
-    } else {
-#pragma unroll
-        for (int w = 0; w < 4; ++w) q[w] = v4u{~0u, ~0u, ~0u, ~0u};
-    }
     uint32_t hits = 0;
     int64_t psum = 0;
-    bucket_scan<true>(q, key, hits, psum);
+    bool full = false;
+    if (mine && idx < nload) {
+        const v4u k0 = S.key[key_at(idx, 0)], k1 = S.key[key_at(idx, 1)];
+        const uint32_t c = S.cnt[idx];
+        const uint32_t hm = slice_hits(k0, k1, c, key);
+        slice_payloads(S, idx, hm, (int32_t)S.pay[idx * 8 + lowest_slot(hm)], hits, psum);
+        full = c == 8;
+    } else if (mine) {
+        const v4u *g = reinterpret_cast<const v4u *>(slots + (uint64_t)bkt * 8);
+        v4u q[4];
+#pragma unroll
+        for (int w = 0; w < 4; ++w) q[w] = load_global(g + w);
+        bucket_scan(q, key, hits, psum);
+        full = bucket_full(q);
+    }
     if (mine && hits) {
         cnt += hits;
         sum += (uint64_t)(psum + (int64_t)hits * (int64_t)val);
@@ -156,18 +180,18 @@ __device__ __forceinline__ uint32_t queue_round(WaveQueue &Q, uint32_t head, uin
     // a chain ends at a bucket with an empty slot, or after visiting all nb
     // buckets once (when its next bucket is its home bucket again)
     const uint32_t nxt = (bkt + 1 == nb) ? 0 : bkt + 1;
-    const bool more = mine && bucket_full(q) && nxt != bucket_of((int32_t)key, nb);
+    const bool more = mine && full && nxt != bucket_of((int32_t)key, nb);
     head += take;
     tail = queue_push(Q, tail, more, key, val, nxt);
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)head);
 }
 
-// A lane's four rows: the four home buckets are read from LDS together (16
-// reads in flight) and scanned; rows whose bucket is full continue at the
-// next bucket, and rows whose home bucket lies outside the slice start at
-// their home bucket, through the wave's queue.
-template <bool CAREFUL>
-__device__ __forceinline__ void lds_probe_quad(const v4i &k, const v4i &v, uint32_t m, const v4u *tab,
+// A lane's four rows: the four home buckets' keys and counts are read from
+// LDS together (12 reads in flight) and compared, then the four lowest-hit
+// payloads are read together; rows whose bucket is full continue at the next
+// bucket, and rows whose home bucket lies outside the slice start at their
+// home bucket, through the wave's queue.
+__device__ __forceinline__ void lds_probe_quad(const v4i &k, const v4i &v, uint32_t m, const Slice &S,
                                                const uint64_t *slots, uint32_t nb, uint32_t lo, uint32_t nload,
                                                uint64_t &cnt, uint64_t &sum, WaveQueue &Q, uint32_t &head,
                                                uint32_t &tail) {
@@ -180,27 +204,36 @@ __device__ __forceinline__ void lds_probe_quad(const v4i &k, const v4i &v, uint3
         ins[j] = ((m >> j) & 1u) && idx < nload;
         ii[j] = ins[j] ? idx : 0u;
     }
-    v4u q[4][4];
+    v4u k0[4], k1[4];
+    uint32_t c[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < 4; ++j) {
+        k0[j] = S.key[key_at(ii[j], 0)];
+        k1[j] = S.key[key_at(ii[j], 1)];
+        c[j] = S.cnt[ii[j]];
+    }
+    uint32_t hm[4];
+    int32_t p1[4];
 #pragma unroll
-        for (int w = 0; w < 4; ++w) q[j][w] = tab[tab_at(ii[j], w)];
+    for (int j = 0; j < 4; ++j) hm[j] = ins[j] ? slice_hits(k0[j], k1[j], c[j], (uint32_t)q_get(k, j)) : 0u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) p1[j] = (int32_t)S.pay[ii[j] * 8 + lowest_slot(hm[j])];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         uint32_t hits = 0;
         int64_t psum = 0;
-        bucket_scan<CAREFUL>(q[j], ins[j] ? (uint32_t)q_get(k, j) : 0u, hits, psum);
-        if (ins[j] && hits) {   // rows outside the slice scanned bucket `lo`: discarded
+        slice_payloads(S, ii[j], hm[j], p1[j], hits, psum);
+        if (hits) {
             cnt += hits;
             sum += (uint64_t)(psum + (int64_t)hits * (int64_t)q_get(v, j));
         }
         const bool live = (m >> j) & 1u;
         const uint32_t from = ins[j] ? ((b[j] + 1 == nb) ? 0 : b[j] + 1) : b[j];
-        const bool go = live && (!ins[j] || (bucket_full(q[j]) && from != b[j]));
+        const bool go = live && (!ins[j] || (c[j] == 8 && from != b[j]));
         tail = queue_push(Q, tail, go, (uint32_t)q_get(k, j), q_get(v, j), from);
         // fewer than kWave entries were queued before this push and at most
         // kWave were added, so the queue never holds more than kQueue - 1
-        while (tail - head >= kWave) head = queue_round(Q, head, tail, tab, slots, nb, lo, nload, cnt, sum);
+        while (tail - head >= kWave) head = queue_round(Q, head, tail, S, slots, nb, lo, nload, cnt, sum);
     }
 }
 
@@ -217,8 +250,31 @@ __device__ __forceinline__ uint32_t load_rows4(const int32_t *K, const int32_t *
     return left >= 4 ? 0xfu : (1u << left) - 1u;
 }
 
+// Copies slice buckets [lo, lo + nload) of the table into S, one bucket per
+// thread: keys, payloads and the fill count (the first empty slot, or 8).
+__device__ __forceinline__ void load_slice(Slice &S, const uint64_t *slots, uint32_t lo, uint32_t nload) {
+    for (uint32_t i = threadIdx.x; i < nload; i += kLdsBlock) {
+        const v4u *g = reinterpret_cast<const v4u *>(slots + (uint64_t)(lo + i) * 8);
+        const v4u q0 = load_global(g), q1 = load_global(g + 1), q2 = load_global(g + 2), q3 = load_global(g + 3);
+        const v4u qs[4] = {q0, q1, q2, q3};
+        uint32_t c = 8;
+#pragma unroll
+        for (int s = 7; s >= 0; --s) {
+            const v4u &q = qs[s >> 1];
+            const bool empty = (s & 1) ? (q.z == ~0u && q.w == ~0u) : (q.x == ~0u && q.y == ~0u);
+            c = empty ? (uint32_t)s : c;
+        }
+        S.key[key_at(i, 0)] = v4u{q0.x, q0.z, q1.x, q1.z};
+        S.key[key_at(i, 1)] = v4u{q2.x, q2.z, q3.x, q3.z};
+        v4u *pay4 = reinterpret_cast<v4u *>(S.pay);
+        pay4[i * 2] = v4u{q0.y, q0.w, q1.y, q1.w};
+        pay4[i * 2 + 1] = v4u{q2.y, q2.w, q3.y, q3.w};
+        S.cnt[i] = (uint8_t)c;
+    }
+}
+
 __global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a) {
-    __shared__ v4u tab[(kSliceMax + kSliceExtra) * 4];
+    __shared__ __align__(16) Slice S;
     __shared__ WaveQueue s_queue[kLdsBlock / kWave];
     __shared__ uint32_t s_part;
     const uint32_t t = threadIdx.x;
@@ -248,9 +304,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a) {
             const uint64_t ix = i0 + (uint64_t)x * kQuadStride;
             m[x] = load_rows4(K, V, ix < r1 ? ix : r0, ix < r1 ? r1 : r0, k[x], v[x]);
         }
-        const v4u *src = reinterpret_cast<const v4u *>(a.slots + (uint64_t)lo * 8);
-        // physical quad i holds quad (i & 3) ^ ((i >> 4) & 3) of bucket i >> 2 (tab_at)
-        for (uint32_t i = t; i < nload * 4; i += kLdsBlock) tab[i] = src[(i & ~3u) | ((i & 3u) ^ ((i >> 4) & 3u))];
+        load_slice(S, a.slots, lo, nload);
         __syncthreads();
         // r0 is a multiple of 64 rows, so quads are 16-byte aligned
         uint32_t head = 0, tail = 0;   // the wave's queue (wave-uniform counters)
@@ -268,15 +322,8 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a) {
                 mn[x] = load_rows4(K, V, ix < r1 ? ix : r0, ix < r1 ? r1 : r0, kn[x], vn[x]);
             }
 #pragma unroll
-            for (int x = 0; x < kAhead; ++x) {
-                bool neg = false;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) neg |= ((m[x] >> j) & 1u) && q_get(k[x], j) == -1;
-                if (ballot(neg))
-                    lds_probe_quad<true>(k[x], v[x], m[x], tab, a.slots, nb, lo, nload, cnt, sum, Q, head, tail);
-                else
-                    lds_probe_quad<false>(k[x], v[x], m[x], tab, a.slots, nb, lo, nload, cnt, sum, Q, head, tail);
-            }
+            for (int x = 0; x < kAhead; ++x)
+                lds_probe_quad(k[x], v[x], m[x], S, a.slots, nb, lo, nload, cnt, sum, Q, head, tail);
 #pragma unroll
             for (int x = 0; x < kAhead; ++x) {
                 k[x] = kn[x];
@@ -286,14 +333,14 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a) {
             i0 = i1;
         }
         // the rest of the queue, before the slice is replaced
-        while (tail != head) head = queue_round(Q, head, tail, tab, a.slots, nb, lo, nload, cnt, sum);
+        while (tail != head) head = queue_round(Q, head, tail, S, a.slots, nb, lo, nload, cnt, sum);
         __syncthreads();
     }
     cnt = wave_sum_u64(cnt);
     sum = wave_sum_u64(sum);
     if (lane_id() == 0) {
-        if (cnt) atomicAdd((unsigned long long *)&a.res->count, (unsigned long long)cnt);
-        if (sum) atomicAdd((unsigned long long *)&a.res->sum, (unsigned long long)sum);
+        if (cnt) atomic_add_global((unsigned long long *)&a.res->count, (unsigned long long)cnt);
+        if (sum) atomic_add_global((unsigned long long *)&a.res->sum, (unsigned long long)sum);
     }
 }
 
