@@ -181,3 +181,46 @@ def test_scatter_shapes_give_identical_results(gpu, orc, shape):
     mgv.sync()
     wc, wsum = orc.groupby(gk, rv, G)
     assert np.array_equal(g.download(np.uint64, G), wc) and np.array_equal(g.download(np.int64, G, 8 * G), wsum)
+
+
+def _assert_prefix_filled(slots: np.ndarray, nbuckets: int, nkeys: int):
+    """Every bucket's occupied slots (not the empty pattern (-1, -1)) are a
+    prefix of its 8 slots, and every inserted row is in the table once."""
+    occ = (slots != np.uint64(0xFFFFFFFFFFFFFFFF)).reshape(nbuckets, 8)
+    c = occ.sum(axis=1)
+    prefix = np.arange(8)[None, :] < c[:, None]
+    assert np.array_equal(occ, prefix)
+    assert int(c.sum()) == nkeys
+
+
+@pytest.mark.parametrize("lf,dup", [(0.25, False), (0.5, False), (0.9, False), (0.9, True), (0.5, True)])
+def test_build_fills_bucket_slots_in_order(gpu, orc, lf, dup):
+    """The LDS probe stores a slice as keys plus a per-bucket fill count and
+    treats slots [0, count) as the occupied ones: that holds only if the
+    build claims a bucket's slots strictly in order.  Checked on tables built
+    concurrently (the build kernel and the fused batch build), with heavy
+    duplicates and chains that run over many buckets."""
+    n = 200000
+    keys = orc.gen_range(n, orc.seed(6, 0), 1, 5000) if dup else orc.gen_perm(n, n, orc.seed(6, 0), 1)
+    pay = orc.gen_bits(n, orc.seed(6, 1), 1, 16)
+    ht = build(keys, pay, lf)
+    _assert_prefix_filled(ht.slots(), ht.nbuckets, n)
+    # the same rows through the fused batch kernel's build
+    ht2 = mgv.HashTable(n, lf)
+    dk, dp = dev(keys), dev(pay)
+    res = mgv.ResultBuffer(1)
+    q = (mgv.Query * 1)()
+    q[0] = mgv.Query(mgv.MGV_Q_BUILD, 0, None, dk.ptr, dp.ptr, n, ht2.buf.ptr, ht2.nbuckets, 0, None, None,
+                     res.ptr, mgv.MGV_PAYLOAD_COLUMN, 0, 0)
+    cnt = C.c_uint64()
+    mgv.call("mgv_batch_plan", q, 1, 4096, None, 0, C.byref(cnt))
+    items = (mgv.WorkItem * cnt.value)()
+    mgv.call("mgv_batch_plan", q, 1, 4096, items, cnt.value, C.byref(cnt))
+    dq = mgv.DeviceBuffer(C.sizeof(q)).upload(np.frombuffer(bytes(q), dtype=np.uint8))
+    di = mgv.DeviceBuffer(C.sizeof(items)).upload(np.frombuffer(bytes(items), dtype=np.uint8))
+    ctr = mgv.DeviceBuffer(256)
+    mgv.call("mgv_fill_u32", ctr.vp, 64, 0, None)
+    mgv.call("mgv_batch_run", dq.vp, di.vp, cnt.value, ctr.vp, mgv.launch(0, 256, 0), None)
+    mgv.sync()
+    assert res.read(0).errors == 0
+    _assert_prefix_filled(ht2.slots(), ht2.nbuckets, n)
