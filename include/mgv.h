@@ -357,9 +357,12 @@ int mgv_debug_xcc_histogram(uint32_t *out8, uint32_t grid, mgv_stream_t s);
  * nbuckets must split into power-of-two slices of at most 2048 buckets (a
  * power-of-two multiple of 2 * fanout slices, at most 2048 of them);
  * n < 2^31.  The workspace (size from mgv_probe_lds_workspace_size) holds
- * both partitionings (16 bytes per row).  Replaces hashjoin::consume_warp
- * (src/operators/hashjoin.cu:135-182) fused with aggregation
- * (src/operators/aggregation.cu:27-66). */
+ * both partitionings (16 bytes per row).  The table must come from
+ * mgv_hashjoin_build or a batch build (mgv_batch_run), which fill each
+ * bucket's slots in order: the LDS copy of a slice keeps per bucket only
+ * the keys, the payloads and the number of filled slots.  Replaces
+ * hashjoin::consume_warp (src/operators/hashjoin.cu:135-182) fused with
+ * aggregation (src/operators/aggregation.cu:27-66). */
 int mgv_probe_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes);
 int mgv_hashjoin_probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
                            const int32_t *keys, const int32_t *vals, uint64_t n, uint32_t fanout, void *ws,
