@@ -224,3 +224,29 @@ def test_build_fills_bucket_slots_in_order(gpu, orc, lf, dup):
     mgv.sync()
     assert res.read(0).errors == 0
     _assert_prefix_filled(ht2.slots(), ht2.nbuckets, n)
+
+
+@pytest.mark.parametrize("n", [1, 3, 4, 255, 257, 4097, 9000])
+def test_tiny_probes_through_both_partitioned_paths(gpu, orc, n):
+    """Inputs smaller than one scatter tile, or not a multiple of 4 rows:
+    the LDS probe and the one-level partitioned probe give the direct
+    probe's COUNT and SUM, with and without a filter."""
+    nb_keys = 1 << 14
+    sk = orc.gen_perm(nb_keys, nb_keys, orc.seed(2, 0), 12)
+    sw = orc.gen_bits(nb_keys, orc.seed(2, 1), 12, 16)
+    rk = orc.gen_range(n, orc.seed(2, 2), 12, 2 * nb_keys)
+    rv = orc.gen_bits(n, orc.seed(2, 3), 12, 16)
+    c = orc.gen_bits(n, orc.seed(3, 0), 12, 31)
+    ht = build(sk, sw, 0.5)
+    for pred, bound in ((None, 0), (c, (1 << 30) - 1)):
+        want = probe_direct(ht, rk, rv, pred, bound)
+        assert probe_lds(ht, rk, rv, pred=pred, bound=bound, fanout=4) == want
+        ws = mgv.partition_workspace(n, 16, 2)
+        dk, dv = dev(rk), dev(rv)
+        dp = dev(pred) if pred is not None else None
+        res = mgv.ResultBuffer()
+        mgv.call("mgv_hashjoin_probe_partitioned", ht.buf.vp, ht.nbuckets, dp.vp if dp else None, bound, dk.vp,
+                 dv.vp, n, 0, 0, 16, ws.vp, ws.nbytes, res.vp, mgv.launch(0, 256), None)
+        mgv.sync()
+        r = res.read()
+        assert (r.count, r.sum) == want
