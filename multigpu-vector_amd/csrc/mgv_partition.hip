@@ -131,6 +131,10 @@ __device__ __forceinline__ uint32_t quad_mask(const PartIn &in, uint64_t i0, uin
     return m;
 }
 
+// Per-block partition histograms.  The block's rows are read in steps of
+// 4 * kPartBlock rows: whole steps with unconditional quad loads, the next
+// step's loads issued before the current step is counted, then the last,
+// partial step with bounds-checked loads.
 template <uint32_t MAXF>
 __global__ __launch_bounds__(kPartBlock) void k_part_count(PartLayout L, PartIn in) {
     __shared__ uint32_t h[MAXF];
@@ -142,26 +146,52 @@ __global__ __launch_bounds__(kPartBlock) void k_part_count(PartLayout L, PartIn 
     uint64_t hi = min(L.n, lo + L.rows_per_blk);
     if (in.nlimit) hi = min(hi, (uint64_t)*in.nlimit);
     uint32_t errs = 0;
-    for (uint64_t i0 = lo + 4ull * threadIdx.x; i0 < hi; i0 += 4ull * kPartBlock) {
-        v4i c;
-        uint32_t m = quad_mask(in, i0, hi, &c);
-        if (m) {
-            v4i k = load_quad(in.key, i0, hi);
-            const uint32_t ok = m & key_mask(in, k);
-            errs += (uint32_t)__builtin_popcount(m & ~ok);
+    auto count_quad = [&](const v4i &k, uint32_t m) {
+        const uint32_t ok = m & key_mask(in, k);
+        errs += (uint32_t)__builtin_popcount(m & ~ok);
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if ((ok >> j) & 1u) atomicAdd(&h[part_key(in, q_get(k, j), L.F)], 1u);
+        for (int j = 0; j < 4; ++j)
+            if ((ok >> j) & 1u) atomicAdd(&h[part_key(in, q_get(k, j), L.F)], 1u);
+    };
+    auto pred_mask = [&](const v4i &c) {
+        uint32_t m = 0xfu;
+        if (in.pred) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) m &= ~((uint32_t)(q_get(c, j) > in.bound) << j);
         }
+        return m;
+    };
+    constexpr uint64_t kStep = 4ull * kPartBlock;
+    const uint64_t whole = hi > lo ? (hi - lo) / kStep : 0;
+    uint64_t i0 = lo + 4ull * threadIdx.x;
+    if (whole) {
+        v4i k = load_stream4(in.key + i0);
+        v4i c = in.pred ? load_stream4(in.pred + i0) : v4i{0, 0, 0, 0};
+        for (uint64_t j = 0; j < whole; ++j) {
+            // the last whole step reloads itself (in bounds, unused) so the
+            // prefetch is unconditional
+            const uint64_t i1 = j + 1 < whole ? i0 + kStep : i0;
+            const v4i kn = load_stream4(in.key + i1);
+            const v4i cn = in.pred ? load_stream4(in.pred + i1) : v4i{0, 0, 0, 0};
+            count_quad(k, pred_mask(c));
+            k = kn;
+            c = cn;
+            i0 += kStep;
+        }
+    }
+    if (i0 < hi) {
+        v4i c;
+        const uint32_t m = quad_mask(in, i0, hi, &c);
+        if (m) count_quad(load_quad(in.key, i0, hi), m);
     }
     if (errs) atomicAdd(&s_err, errs);
     __syncthreads();
     if (threadIdx.x == 0 && s_err && in.res)
-        atomicAdd((unsigned long long *)&in.res->errors, (unsigned long long)s_err);
+        atomic_add_global((unsigned long long *)&in.res->errors, (unsigned long long)s_err);
     for (uint32_t p = threadIdx.x; p < L.F; p += kPartBlock) {
         uint32_t c = h[p];
         L.hist[(uint64_t)blockIdx.x * L.F + p] = c;
-        if (c) atomicAdd(&L.ptotal[p], (unsigned long long)c);
+        if (c) atomic_add_global(&L.ptotal[p], (unsigned long long)c);
     }
 }
 
@@ -239,9 +269,10 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(PartLayout L, PartIn in)
     constexpr int R = TILE / BLOCK;         // write rounds per tile
     constexpr int PER = MAXF > (uint32_t)BLOCK ? (int)(MAXF / BLOCK) : 1;   // partitions per owner thread
     static_assert(Q >= 1 && TILE % (4 * BLOCK) == 0 && kTileMax % TILE == 0, "scatter shape");
+    static_assert(TILE <= 65536, "tile offsets are 16-bit");
     __shared__ unsigned long long base[MAXF];
     __shared__ uint32_t tcnt[MAXF];
-    __shared__ uint32_t toff[MAXF];
+    __shared__ uint16_t toff[MAXF];   // first staged slot of each partition (< TILE)
     __shared__ uint32_t wtot[BLOCK / kWave];
     __shared__ uint32_t s_total;
     __shared__ int32_t sk[TILE];
@@ -341,7 +372,7 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(PartLayout L, PartIn in)
         for (int e = 0; e < PER; ++e) {
             const uint32_t p = t * per + e;
             if (e < (int)per && p < F) {
-                toff[p] = start;
+                toff[p] = (uint16_t)start;
                 base[p] = cur[e] - start;
                 cur[e] += c[e];
                 tcnt[p] = 0;
