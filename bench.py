@@ -102,19 +102,39 @@ def leg_c2(args, st):
     L = mgv.Launch(0, block, -1, coop | (shape << 8))
     Llds = mgv.Launch(0, 1024, -1, shape << 8)
 
-    # build (timed: clear excluded)
-    build_ms = []
-    for rep in range(3):
-        ht.clear(st.vp)
-        res.clear(st.vp)
-        t = Timer(st.vp)
-        t.around(lambda: mgv.call("mgv_hashjoin_build", ht.buf.vp, ht.nbuckets, sk.vp, sw.vp,
-                                  mgv.MGV_PAYLOAD_COLUMN, None, 0, ns, 0, res.vp, L, st.vp))
-        st.sync()
-        build_ms.append(t.durations_ms()[0])
+    # build (timed: clear excluded): the direct build and the build through
+    # LDS table slices; the faster one's table is kept for the probe
+    bneed = C.c_uint64()
+    mgv.call("mgv_build_lds_workspace_size", ns, ht.nbuckets, 32, C.byref(bneed))
+    bws = mgv.DeviceBuffer(bneed.value)
+    Lb = mgv.Launch(0, 1024, -1, 2 << 8)
+    builds = {
+        "direct": lambda: mgv.call("mgv_hashjoin_build", ht.buf.vp, ht.nbuckets, sk.vp, sw.vp,
+                                   mgv.MGV_PAYLOAD_COLUMN, None, 0, ns, 0, res.vp, L, st.vp),
+        "lds": lambda: mgv.call("mgv_hashjoin_build_lds", ht.buf.vp, ht.nbuckets, sk.vp, sw.vp,
+                                mgv.MGV_PAYLOAD_COLUMN, None, 0, ns, 0, 32, bws.vp, bws.nbytes, res.vp, Lb, st.vp),
+    }
+    build_by = {}
+    for name, fn in builds.items():
+        bms = []
+        for rep in range(3):
+            ht.clear(st.vp)
+            res.clear(st.vp)
+            t = Timer(st.vp)
+            t.around(fn)
+            st.sync()
+            bms.append(t.durations_ms()[0])
+        assert res.read().errors == 0
+        build_by[name] = min(bms[1:])
+        log(f"C2: build {name} {build_by[name]:.3f} ms ({gb(ns) / (build_by[name] / 1e3):.2f} Grows/s)")
+    build_path = min(build_by, key=build_by.get)
+    build_t = build_by[build_path]
+    ht.clear(st.vp)
+    res.clear(st.vp)
+    builds[build_path]()
+    st.sync()
     assert res.read().errors == 0
-    build_t = min(build_ms[1:])
-    log(f"C2: build {build_t:.3f} ms ({gb(ns) / (build_t / 1e3):.2f} Grows/s)")
+    bws.close()
 
     ws = None
 
@@ -218,7 +238,8 @@ def leg_c2(args, st):
             "coop_probe": coop,
             "algorithmic_bytes_per_launch": algo_bytes,
         },
-        "build": {"ms": build_t, "grows_per_s": gb(ns) / (build_t / 1e3),
+        "build": {"ms": build_t, "path": build_path, "ms_by_path": build_by,
+                  "grows_per_s": gb(ns) / (build_t / 1e3),
                   "gbps_algorithmic": (ns * 8 + ht.nbuckets * 64) / (build_t / 1e3) / 1e9},
         "result": {"count": r.count, "sum": r.sum, "count_closed_form_ok": ok_count},
         "probe_path_sweep_ms": {f"{m}{f}": v for (m, f), v in sweep.items()},

@@ -349,6 +349,25 @@ int mgv_hashjoin_probe_groupby_partitioned(const uint64_t *slots, uint32_t nbuck
 /* Diagnostic: out8[x] += number of `grid` one-wave blocks that ran on XCD x. */
 int mgv_debug_xcc_histogram(uint32_t *out8, uint32_t grid, mgv_stream_t s);
 
+/* Build through LDS-resident table slices (same table contents, as sets
+ * of {key, payload} per chain, as mgv_hashjoin_build): the rows (optionally
+ * only those with pred <= bound) are partitioned by table slice as in
+ * mgv_hashjoin_probe_lds, then one block per slice copies the slice into
+ * LDS, inserts the slice's rows with LDS compare-and-swap (each bucket's
+ * slots claimed in order), and writes it back; rows whose chain runs past
+ * their slice are inserted into the global table afterwards.  Existing
+ * contents of the table are kept.  No other build may write the same table
+ * concurrently.  MGV_PAYLOAD_ROWID runs mgv_hashjoin_build.  Same bucket
+ * and fan-out limits as mgv_hashjoin_probe_lds; n < 2^31; the workspace
+ * (mgv_build_lds_workspace_size) holds both partitionings and the overflow
+ * list (24 bytes per row).  Replaces hashjoin_builder::consume_warp
+ * (src/operators/hashjoin.cu:41-60). */
+int mgv_build_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes);
+int mgv_hashjoin_build_lds(uint64_t *slots, uint32_t nbuckets, const int32_t *keys, const int32_t *payload,
+                           int payload_mode, const int32_t *pred, int32_t bound, uint64_t n, int64_t row_base,
+                           uint32_t fanout, void *workspace, uint64_t workspace_bytes, mgv_result *res,
+                           mgv_launch launch, mgv_stream_t s);
+
 /* Probe + aggregate through LDS-resident table slices (same results as
  * mgv_hashjoin_probe without flags): the probe rows (optionally only those
  * with pred <= bound) are hash-partitioned twice -- `fanout` partitions,

@@ -344,6 +344,117 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a) {
     }
 }
 
+// ------------------------------------------------------------------ build
+
+struct LdsBuildArgs {
+    uint64_t *slots;
+    uint32_t nbuckets;
+    uint32_t slice;          // buckets per level-2 partition
+    PartLayout L;            // level-2 layout (key, payload)
+    uint32_t *next;          // partition counter (zeroed before the launch)
+    unsigned long long *ovf_count;
+    int32_t *ovf_key;        // rows whose chain leaves their slice
+    int32_t *ovf_pay;
+    mgv_result *res;
+};
+
+// Inserts {key, pay} into slice bucket idx of an LDS slice of `slice`
+// buckets, the way OpBuild::insert does in global memory: the first guess is
+// the bucket's first empty slot as seen, a failed compare-and-swap moves to
+// the next slot, a full bucket to the next bucket, so each bucket's slots are
+// claimed in order.  Returns false when the chain would leave the slice (the
+// caller finishes the row in the global table).
+__device__ __forceinline__ bool lds_insert(unsigned long long *tab, uint32_t slice, uint32_t idx, uint64_t val) {
+    const v4u *q = reinterpret_cast<const v4u *>(tab + (uint64_t)idx * 8);
+    uint32_t s = 8;
+#pragma unroll
+    for (int t = 3; t >= 0; --t) {
+        const v4u w = q[t];
+        if (w.z == ~0u && w.w == ~0u) s = 2 * t + 1;
+        if (w.x == ~0u && w.y == ~0u) s = 2 * t;
+    }
+    for (;;) {
+        if (s == 8) {
+            if (++idx == slice) return false;
+            s = 0;
+        }
+        if (atomicCAS(&tab[idx * 8 + s], (unsigned long long)kEmptySlot, (unsigned long long)val) == kEmptySlot)
+            return true;
+        ++s;
+    }
+}
+
+// One block per CU claims level-2 partitions: it copies the partition's
+// table slice into LDS, inserts the partition's rows (all of which have
+// their home bucket in the slice), and writes the slice back.  Rows whose
+// chain runs past the slice go to the overflow list.
+__global__ __launch_bounds__(kLdsBlock) void k_lds_build(LdsBuildArgs a) {
+    __shared__ __align__(16) unsigned long long tab[kSliceMax * 8];
+    __shared__ uint32_t s_part;
+    const uint32_t t = threadIdx.x;
+    const int32_t *K = a.L.col[0];
+    const int32_t *V = a.L.col[1];
+    uint64_t errs = 0;
+    for (;;) {
+        if (t == 0) s_part = atomicAdd(a.next, 1u);
+        __syncthreads();
+        const uint32_t p = s_part;
+        __syncthreads();
+        if (p >= a.L.F) break;
+        const uint64_t r0 = a.L.pstart[p];
+        const uint64_t r1 = r0 + a.L.ptotal[p];
+        if (r0 == r1) continue;   // an untouched slice is not copied
+        const uint32_t lo = p * a.slice;
+        v4u *g = reinterpret_cast<v4u *>(a.slots + (uint64_t)lo * 8);
+        v4u *l = reinterpret_cast<v4u *>(tab);
+        for (uint32_t i = t; i < a.slice * 4; i += kLdsBlock) l[i] = load_global(g + i);
+        __syncthreads();
+        // r0 is a multiple of 64 rows; columns are padded past the partition end
+        for (uint64_t i0 = r0 + 4ull * t; i0 < r1; i0 += 4ull * kLdsBlock) {
+            const v4i k = load_stream4(K + i0);
+            const v4i v = load_stream4(V + i0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (i0 + j >= r1) break;
+                const uint32_t key = (uint32_t)q_get(k, j), pay = (uint32_t)q_get(v, j);
+                const uint64_t val = ((uint64_t)pay << 32) | key;
+                if (val == kEmptySlot) {
+                    ++errs;
+                    continue;
+                }
+                // home buckets lie in the slice (bucket / slice = level-2
+                // partition); anything else is finished in the global table
+                const uint32_t idx = bucket_of((int32_t)key, a.nbuckets) - lo;
+                if (idx >= a.slice || !lds_insert(tab, a.slice, idx, val)) {
+                    const unsigned long long o = atomic_add_global(a.ovf_count, 1ull);
+                    a.ovf_key[o] = (int32_t)key;
+                    a.ovf_pay[o] = (int32_t)pay;
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = t; i < a.slice * 4; i += kLdsBlock) g[i] = l[i];
+        __syncthreads();
+    }
+    errs = wave_sum_u64(errs);
+    if (lane_id() == 0 && errs && a.res) atomic_add_global((unsigned long long *)&a.res->errors, errs);
+}
+
+// The overflow rows, inserted into the global table after every slice is
+// written back (their chains continue into the next slices, or wrap around
+// to bucket 0).
+__global__ __launch_bounds__(256) void k_lds_build_overflow(LdsBuildArgs a) {
+    OpBuild<256 / kWave> op;
+    op.slots = a.slots;
+    op.nbuckets = a.nbuckets;
+    op.res = a.res;
+    op.consume_open();
+    const uint64_t n = *a.ovf_count;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        op.insert((uint32_t)a.ovf_key[i], (uint32_t)a.ovf_pay[i]);
+    op.consume_close();
+}
+
 // Level-2 fan-out for a table of nb buckets: the smallest power of two
 // F2 >= 2 * F1 that divides nb into slices of at most kSliceMax buckets.
 static int lds_fanout(uint32_t nb, uint32_t F1, uint32_t *F2) {
@@ -371,6 +482,24 @@ static int lds_plan(uint64_t n, uint32_t nb, uint32_t F1, void *ws, uint64_t ws_
     if (!ws || ws_bytes < o2 + b2) return fail(MGV_E_CAPACITY, "LDS probe: workspace too small");
     if ((rc = plan_layout(n, F1, 2, (char *)ws + o1, b1, L1, nullptr, 1))) return rc;
     return plan_layout(n, *F2, 2, (char *)ws + o2, b2, L2, nullptr, kPartAlign);
+}
+
+// Build workspace: the probe's (counters + both partitionings of n rows)
+// followed by the overflow list (n keys, n payloads).
+static int build_plan(uint64_t n, uint32_t nb, uint32_t F1, void *ws, uint64_t ws_bytes, PartLayout *L1,
+                      PartLayout *L2, uint32_t *F2, int32_t **okey, int32_t **opay, uint64_t *need) {
+    uint64_t base = 0;
+    int rc = lds_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, F2, &base);
+    if (rc) return rc;
+    base = (base + 255) / 256 * 256;
+    const uint64_t ob = (n * 4 + 255) / 256 * 256;
+    if (need) *need = base + 2 * ob;
+    if (!L1) return MGV_OK;
+    if (!ws || ws_bytes < base + 2 * ob) return fail(MGV_E_CAPACITY, "LDS build: workspace too small");
+    if ((rc = lds_plan(n, nb, F1, ws, base, L1, L2, F2, nullptr))) return rc;
+    *okey = (int32_t *)((char *)ws + base);
+    *opay = (int32_t *)((char *)ws + base + ob);
+    return MGV_OK;
 }
 
 }  // namespace mgv
@@ -411,6 +540,49 @@ int mgv_hashjoin_probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32
     LdsProbeArgs a = {slots, nbuckets, nbuckets / F2, L2, next, res};
     hipLaunchKernelGGL(k_lds_probe, dim3(grid), dim3(kLdsBlock), 0, s, a);
     return check_launch("k_lds_probe");
+}
+
+int mgv_build_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes) {
+    if (!bytes || nbuckets == 0) return fail(MGV_E_INVALID, "mgv_build_lds_workspace_size: bad argument");
+    uint32_t F2 = 0;
+    return build_plan(n, nbuckets, fanout, nullptr, 0, nullptr, nullptr, &F2, nullptr, nullptr, bytes);
+}
+
+int mgv_hashjoin_build_lds(uint64_t *slots, uint32_t nbuckets, const int32_t *keys, const int32_t *payload,
+                           int payload_mode, const int32_t *pred, int32_t bound, uint64_t n, int64_t row_base,
+                           uint32_t fanout, void *ws, uint64_t ws_bytes, mgv_result *res, mgv_launch LC,
+                           mgv_stream_t st) {
+    if (!current_state()) return fail(MGV_E_NOT_INIT, "mgv_init was not called");
+    if (!slots || !keys || nbuckets == 0) return fail(MGV_E_INVALID, "build_lds: null argument");
+    if (n == 0) return MGV_OK;
+    if (n >= (uint64_t)nbuckets * MGV_SLOTS_PER_BUCKET)
+        return fail(MGV_E_CAPACITY, "build_lds: %llu rows do not fit %u buckets", (unsigned long long)n, nbuckets);
+    if (n >= (1ull << 31)) return fail(MGV_E_INVALID, "build_lds: n must be < 2^31");
+    if (payload_mode == MGV_PAYLOAD_ROWID)   // row-id payloads: the direct build
+        return mgv_hashjoin_build(slots, nbuckets, keys, payload, payload_mode, pred, bound, n, row_base, res, LC, st);
+    if (payload_mode == MGV_PAYLOAD_COLUMN && !payload) return fail(MGV_E_INVALID, "build_lds: payload column missing");
+    if (payload_mode != MGV_PAYLOAD_COLUMN && payload_mode != MGV_PAYLOAD_NONE)
+        return fail(MGV_E_INVALID, "build_lds: bad payload mode %d", payload_mode);
+    PartLayout L1, L2;
+    uint32_t F2 = 0;
+    int32_t *okey = nullptr, *opay = nullptr;
+    int rc = build_plan(n, nbuckets, fanout, ws, ws_bytes, &L1, &L2, &F2, &okey, &opay, nullptr);
+    if (rc) return rc;
+    hipStream_t s = stream_of(st);
+    const int32_t *pcol = payload_mode == MGV_PAYLOAD_COLUMN ? payload : nullptr;   // NONE: zeros
+    PartIn in1 = {pred, bound, keys, pcol, 0, 0u, 0u, 0u, nullptr, nullptr};
+    if ((rc = partition_rows(L1, in1, s, scatter_shape(LC)))) return rc;
+    PartIn in2 = {nullptr, 0, L1.col[0], L1.col[1], 0, 0u, 0u, 0u, nullptr, L1.pstart + L1.F};
+    if ((rc = partition_rows(L2, in2, s, scatter_shape(LC)))) return rc;
+    uint32_t *ctr = (uint32_t *)ws;   // [0]: partition counter, [2..3]: overflow count
+    if ((rc = mgv_fill_u32(ctr, 4, 0, (mgv_stream_t)s))) return rc;
+    LdsBuildArgs a = {slots, nbuckets, nbuckets / F2, L2, ctr, (unsigned long long *)(ctr + 2), okey, opay, res};
+    static thread_local int cached = 0;
+    if (!cached) cached = persistent_blocks(k_lds_build, kLdsBlock);
+    hipLaunchKernelGGL(k_lds_build, dim3(LC.grid ? LC.grid : (uint32_t)cached), dim3(kLdsBlock), 0, s, a);
+    if ((rc = check_launch("k_lds_build"))) return rc;
+    hipLaunchKernelGGL(k_lds_build_overflow, dim3(256), dim3(256), 0, s, a);
+    return check_launch("k_lds_build_overflow");
 }
 
 }  // extern "C"
