@@ -20,7 +20,7 @@
 
 namespace mgv {
 
-// NC: the chain reads columns [0, NC) (see load_rows_full).
+// NC: the chain reads columns [0, NC) (see load_full in mgv_pipes.hpp).
 template <int NC, class Op>
 __device__ __forceinline__ void run_rows(Op &op, const Cols &cols, uint64_t lo, uint64_t hi, int64_t row_base) {
     op.consume_open();
