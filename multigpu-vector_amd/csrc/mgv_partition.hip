@@ -346,15 +346,15 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(PartLayout L, PartIn in)
         }
         if (tb + TILE < hi) load_tile(tb + TILE);
         // 1. rank every row within its partition
-        uint32_t part[Q][4], rank[Q][4];
+        // masked rows get partition 0 (a dropped key may map past F); the
+        // partition is recomputed from the key when the row is staged
+        uint32_t rank[Q][4];
 #pragma unroll
         for (int q = 0; q < Q; ++q)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                // masked rows get partition 0 (a dropped key may map past F)
                 const bool ok = (cm[q] >> j) & 1u;
-                part[q][j] = ok ? part_key(in, q_get(ck[q], j), F) : 0u;
-                rank[q][j] = ok ? atomicAdd(&tcnt[part[q][j]], 1u) : 0u;
+                rank[q][j] = ok ? atomicAdd(&tcnt[ok ? part_key(in, q_get(ck[q], j), F) : 0u], 1u) : 0u;
             }
         __syncthreads();
         // 2. owners scan the tile counts, publish toff / base, advance cursors
@@ -385,7 +385,10 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(PartLayout L, PartIn in)
 #pragma unroll
         for (int q = 0; q < Q; ++q)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) pos[q][j] = toff[part[q][j]] + rank[q][j];
+            for (int j = 0; j < 4; ++j) {
+                const bool ok = (cm[q] >> j) & 1u;
+                pos[q][j] = toff[ok ? part_key(in, q_get(ck[q], j), F) : 0u] + rank[q][j];
+            }
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
 #pragma unroll
@@ -405,27 +408,31 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(PartLayout L, PartIn in)
         // (partitions end at pstart[F] <= n + F * (align - 1) < cap - 1); a
         // keys-only layout stores the key twice instead of a second column.
         const uint32_t total = s_total;
-        int32_t k[R], v[R], rid[R];
-        uint32_t ii[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            ii[r] = total ? min(t + (uint32_t)r * BLOCK, total - 1) : 0u;
-            k[r] = sk[ii[r]];
-            v[r] = sv[ii[r]];
-            rid[r] = ROWID ? sr[ii[r]] : 0;
-        }
-        unsigned long long dst[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const uint32_t p = total ? part_key(in, k[r], F) : 0u;
-            dst[r] = total ? base[p] + ii[r] : L.cap - 1;
-        }
         int32_t *const c1 = L.ncols > 1 ? L.col[1] : L.col[0];
+        constexpr int G = R < 8 ? R : 8;   // rounds per group: bounds the registers held
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            L.col[0][dst[r]] = k[r];
-            c1[dst[r]] = L.ncols > 1 ? v[r] : k[r];
-            if (ROWID) L.col[2][dst[r]] = rid[r];
+        for (int r0 = 0; r0 < R; r0 += G) {
+            int32_t k[G], v[G], rid[G];
+            uint32_t ii[G];
+#pragma unroll
+            for (int r = 0; r < G; ++r) {
+                ii[r] = total ? min(t + (uint32_t)(r0 + r) * BLOCK, total - 1) : 0u;
+                k[r] = sk[ii[r]];
+                v[r] = sv[ii[r]];
+                rid[r] = ROWID ? sr[ii[r]] : 0;
+            }
+            unsigned long long dst[G];
+#pragma unroll
+            for (int r = 0; r < G; ++r) {
+                const uint32_t p = total ? part_key(in, k[r], F) : 0u;
+                dst[r] = total ? base[p] + ii[r] : L.cap - 1;
+            }
+#pragma unroll
+            for (int r = 0; r < G; ++r) {
+                L.col[0][dst[r]] = k[r];
+                c1[dst[r]] = L.ncols > 1 ? v[r] : k[r];
+                if (ROWID) L.col[2][dst[r]] = rid[r];
+            }
         }
         __syncthreads();
     }
