@@ -82,8 +82,16 @@ struct Admitted {
     uint64_t off, bytes;     // arena reservation: table, or group arrays [+ group-by workspace]
     uint32_t nbuckets;
     uint64_t group_stage;    // offset (in groups) into the slot's pinned group staging
-    uint64_t ws;             // group-by workspace bytes (0: group-by runs in the fused launch)
+    uint64_t ws;             // workspace bytes at the end of the reservation: the partitioned
+                             // group-by's, or a large join build's (LDS build); 0: fused launch
 };
+
+// Joins whose build side has at least this many rows build their table
+// through LDS slices (mgv_hashjoin_build_lds, about twice as fast as the
+// fused launch's CAS build for tables far larger than the L2) instead of in
+// the batch's fused build launch.
+constexpr uint64_t kLdsBuildRows = 1ull << 21;
+constexpr uint32_t kLdsBuildFanout = 32;
 
 struct Slot {
     mgv_stream_t stream = nullptr;
@@ -137,12 +145,14 @@ struct Server {
     // arrays plus, for many groups, the partitioned group-by's workspace.
     uint64_t footprint(const mgv_serve_request &q, uint32_t *nbuckets, uint64_t *ws = nullptr) const {
         if (q.kind == MGV_SERVE_JOIN_SUM) {
-            uint64_t bytes = 0;
+            uint64_t bytes = 0, w = 0;
             uint32_t nb = 0;
             gpu(mgv_ht_plan(q.bn, cfg.load_factor, &nb, &bytes));
             if (nbuckets) *nbuckets = nb;
-            if (ws) *ws = 0;
-            return bytes;
+            // a table whose buckets do not split into LDS slices builds in the fused launch
+            if (q.bn < kLdsBuildRows || mgv_build_lds_workspace_size(q.bn, nb, kLdsBuildFanout, &w) != MGV_OK) w = 0;
+            if (ws) *ws = w;
+            return w ? align_up(bytes, 256) + w : bytes;
         }
         uint64_t w = 0;
         gpu(mgv_groupby_workspace_size(q.n, q.ngroups, &w));
@@ -211,6 +221,7 @@ struct Server {
         const size_t items_at = align_up(2 * (size_t)cfg.max_batch * sizeof(mgv_query), 256);
         mgv_work_item *items = (mgv_work_item *)(s.h_desc + items_at);
         s.nbuild = s.nmain = 0;
+        uint32_t join = 0;   // build result slot: max_batch + the join's index in the batch
         for (size_t i = 0; i < s.q.size(); ++i) {
             const Admitted &a = s.q[i];
             const mgv_serve_request &r = R[a.req];
@@ -225,9 +236,9 @@ struct Server {
                 b.n = r.bn;
                 b.slots = (uint64_t *)(d_arena + a.off);
                 b.nbuckets = a.nbuckets;
-                b.res = s.d_res + cfg.max_batch + s.nbuild;
+                b.res = s.d_res + cfg.max_batch + join++;
                 b.payload_mode = MGV_PAYLOAD_COLUMN;
-                bq[s.nbuild++] = b;
+                if (!a.ws) bq[s.nbuild++] = b;   // large builds run through LDS slices below
                 m.kind = MGV_Q_PROBE_SUM;
                 m.bound = r.bound;
                 m.pred = r.pred;
@@ -265,6 +276,18 @@ struct Server {
         const mgv_query *dbq = (const mgv_query *)s.d_desc;
         const mgv_query *dmq = dbq + cfg.max_batch;
         const mgv_work_item *di = (const mgv_work_item *)(s.d_desc + items_at);
+        join = 0;
+        for (size_t i = 0; i < s.q.size(); ++i) {
+            const Admitted &a = s.q[i];
+            const mgv_serve_request &r = R[a.req];
+            if (r.kind != MGV_SERVE_JOIN_SUM) continue;
+            const uint32_t j = join++;
+            if (!a.ws) continue;
+            gpu(mgv_hashjoin_build_lds((uint64_t *)(d_arena + a.off), a.nbuckets, r.bkey, r.bval, MGV_PAYLOAD_COLUMN,
+                                       nullptr, 0, r.bn, 0, kLdsBuildFanout, d_arena + a.off + (a.bytes - a.ws), a.ws,
+                                       s.d_res + cfg.max_batch + j, mgv_launch{0, 1024, -1, MGV_LAUNCH_SCATTER(2u)},
+                                       s.stream));
+        }
         if (nb) gpu(mgv_batch_run(dbq, di, nb, s.d_counter, cfg.launch, s.stream));
         if (nm) gpu(mgv_batch_run(dmq, di + nb, nm, s.d_counter + 32, cfg.launch, s.stream));
         for (size_t i = 0; i < s.q.size(); ++i) {

@@ -107,3 +107,30 @@ def test_serve_many_groups_use_partitioned_groupby(gpu, orc):
         assert big and all(mix[i].footprint > (1 << 14) * 16 for i in big)   # workspace counted
     finally:
         t.close()
+
+
+def test_serve_large_builds_through_lds_slices(gpu, orc):
+    """Joins whose build side has >= 2^21 rows build their table through LDS
+    slices (their footprint then includes the build workspace); results are
+    the same alone and in the mix, and equal the oracle."""
+    mgv.load_ops()
+    t = workloads.ServeTables(build_logs=(20, 21, 22), probe_cap=1 << 20, group_rows=(1 << 16,),
+                              ngroups=1 << 10, zipf_s=0.8)
+    mgv.sync()
+    try:
+        n = 12
+        reqs, desc = t.requests(n, rate_qps=5000.0, seed=5)
+        mix, _ = workloads.serve(reqs, n, 1 << 30, max_inflight=3, max_batch=4, vector_rows=4096)
+        alone, _ = workloads.serve(reqs, n, 1 << 30, max_inflight=1, max_batch=1, vector_rows=4096)
+        assert triples(mix, n) == triples(alone, n) == oracle_results(orc, t, desc)
+        big = [i for i, d in enumerate(desc) if d["kind"] == "join" and t.pairs[d["pair"]]["B"] >= 1 << 21]
+        assert big, "the request mix has no large build"
+        for i in big:
+            _, table_bytes = mgv.ht_plan(t.pairs[desc[i]["pair"]]["B"], 0.5)
+            assert mix[i].footprint > table_bytes   # table + LDS-build workspace
+        small = [i for i, d in enumerate(desc) if d["kind"] == "join" and t.pairs[d["pair"]]["B"] < 1 << 21]
+        for i in small:
+            _, table_bytes = mgv.ht_plan(t.pairs[desc[i]["pair"]]["B"], 0.5)
+            assert mix[i].footprint <= (table_bytes + 255) // 256 * 256
+    finally:
+        t.close()
