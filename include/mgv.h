@@ -7,8 +7,10 @@
  * success, otherwise a HIP error code or one of the MGV_E* codes below, and
  * records a message for mgv_last_error_string().  Nothing here calls exit().
  * Device pointers come from mgv_malloc; every launch takes an explicit stream
- * (NULL = the library's default stream of the current device).  Calls are
- * thread-safe for distinct streams.
+ * (NULL = the library's default stream of the current device).  All of the
+ * library's streams, the default one included, are non-blocking: work on two
+ * different streams is ordered only through events or synchronisation.
+ * Calls are thread-safe for distinct streams.
  *
  * Each entry point names the reference interface it replaces (path:line,
  * relative to the reference tree).  The reference has no C ABI of its own:
