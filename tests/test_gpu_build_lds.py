@@ -162,3 +162,32 @@ def test_lds_build_on_top_of_an_existing_table(gpu, orc):
     rk = orc.gen_range(n, orc.seed(7, 2), 7, n // 2)
     rv = orc.gen_bits(n, orc.seed(7, 3), 7, 16)
     assert probe(a, rk, rv) == probe(b, rk, rv)
+
+
+def test_lds_build_rejects_bad_arguments(gpu, orc):
+    n = 8192   # 2048 buckets at load factor 0.5: splits into slices
+    keys = orc.gen_perm(n, n, orc.seed(7, 0), 8)
+    pay = orc.gen_bits(n, orc.seed(7, 1), 8, 16)
+    ht = mgv.HashTable(n, 0.5)
+    dk, dp = dev(keys), dev(pay)
+    need = C.c_uint64()
+    mgv.call("mgv_build_lds_workspace_size", n, ht.nbuckets, 4, C.byref(need))
+    ws = mgv.DeviceBuffer(need.value)
+    L = mgv.Launch(0, 1024, -1, 2 << 8)
+    with pytest.raises(mgv.MgvError):   # workspace one byte short
+        mgv.call("mgv_hashjoin_build_lds", ht.buf.vp, ht.nbuckets, dk.vp, dp.vp, mgv.MGV_PAYLOAD_COLUMN, None, 0, n, 0,
+                 4, ws.vp, need.value - 1, None, L, None)
+    with pytest.raises(mgv.MgvError):   # unknown payload mode
+        mgv.call("mgv_hashjoin_build_lds", ht.buf.vp, ht.nbuckets, dk.vp, dp.vp, 7, None, 0, n, 0, 4, ws.vp,
+                 ws.nbytes, None, L, None)
+    with pytest.raises(mgv.MgvError):   # payload column missing
+        mgv.call("mgv_hashjoin_build_lds", ht.buf.vp, ht.nbuckets, dk.vp, None, mgv.MGV_PAYLOAD_COLUMN, None, 0, n, 0,
+                 4, ws.vp, ws.nbytes, None, L, None)
+    with pytest.raises(mgv.MgvError):   # bucket count that does not split into slices
+        mgv.call("mgv_build_lds_workspace_size", n, 4099, 4, C.byref(need))
+    with pytest.raises(mgv.MgvError):   # more rows than slots
+        mgv.call("mgv_hashjoin_build_lds", ht.buf.vp, 16, dk.vp, dp.vp, mgv.MGV_PAYLOAD_COLUMN, None, 0, n, 0, 4,
+                 ws.vp, ws.nbytes, None, L, None)
+    mgv.sync()
+    # the table was not touched by the rejected calls
+    assert not np.any(ht.slots() != EMPTY)
