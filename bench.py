@@ -226,6 +226,11 @@ def leg_c2(args, st):
             "unit": "GB/s",
             "frac": algo_bytes / (avg_kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBPS,
             "traffic": traffic_from_profiles(f"probe_c2_{best_mode}{fanout}"),
+            # the measured HBM traffic per call over the same average call time
+            "traffic_gbps": (traffic_from_profiles(f"probe_c2_{best_mode}{fanout}") or 0) / (avg_kernel_ms / 1e3) / 1e9
+            or None,
+            "traffic_frac": (traffic_from_profiles(f"probe_c2_{best_mode}{fanout}") or 0) / (avg_kernel_ms / 1e3) / 1e9
+            / HBM_PEAK_GBPS or None,
             "kernel": {"direct": "k_pipeline<PProbeSum<false,true,false>> (mgv_hashjoin_probe)",
                        "part": "mgv_hashjoin_probe_partitioned: k_part_prep + k_part_count + k_part_scan + "
                                "k_part_scatter + k_part_probe (events around the whole call)",
