@@ -424,6 +424,32 @@ int mgv_groupby_partitioned(const int32_t *keys, const int32_t *vals, uint64_t n
                             uint32_t ngroups, void *ws, uint64_t ws_bytes, mgv_result *res, mgv_launch launch,
                             mgv_stream_t s);
 
+/* ------------------------------------------------------------ split */
+
+/* The split operator (src/operators/split.cu:54-118, partition function
+ * hash3(x) = x & 1 at split.cu:49-51): every row of a vector whose first
+ * column x has (x & 1) == p, and which passes the optional filter
+ * pred <= bound (pred NULL: every row), is appended to parent p's output
+ * columns.  Rows whose destination has no parent are dropped, as in the
+ * reference: with nparents == 1 the odd rows, and parents 2.. receive
+ * nothing.
+ *   cols      host array of ncols device columns of n rows (the row's
+ *             values; column 0 decides the destination)
+ *   out       host array of nparents * ncols device columns; parent p's
+ *             column c is out[p * ncols + c], `cap` rows each
+ *   counts    device array of nparents append cursors (rows so far): a row
+ *             for parent p goes to position counts[p]++ of its columns;
+ *             rows at positions >= cap are not written but still counted,
+ *             so counts[p] > cap after the call reports an overflow
+ * The order of the rows inside a parent's output is unspecified.  The
+ * launch's grid (0 = one persistent grid) is honoured, its block is not
+ * (blocks of 1024 threads take tiles of 8192 rows). */
+#define MGV_SPLIT_MAX_COLS 4
+#define MGV_SPLIT_MAX_PARENTS 8
+int mgv_split(const int32_t *pred, int32_t bound, const int32_t *const *cols, uint32_t ncols, uint64_t n,
+              uint32_t nparents, int32_t *const *out, uint64_t cap, uint64_t *counts, mgv_launch launch,
+              mgv_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,26 @@ int mgv_ops_select(const int32_t *pred, const int32_t *val, uint64_t n, int32_t 
                    uint32_t vector_rows, int device, uint32_t grid, uint32_t block, int32_t *dst, uint64_t dst_cap,
                    uint64_t *count);
 
+/* generator -> exchange -> unstable_select(less_eq_than(thres)) ->
+ * split(x & 1) -> {union_all, union_all} (one union_all, both parents of the
+ * split) -> gpu_to_cpu -> exchange -> materializer(dst)
+ * (create_select_pipeline_exp, src/main9.cu:228-258, on one GPU).  The split
+ * routes each selected row by its parity and the union_all brings both
+ * halves back together, so the contract is mgv_ops_select's: dst (if not
+ * NULL, capacity dst_cap >= n) receives the selected rows of val in
+ * unspecified order, *count = selected rows. */
+int mgv_ops_select_split(const int32_t *pred, const int32_t *val, uint64_t n, int32_t thres, uint32_t vector_rows,
+                         int device, uint32_t grid, uint32_t block, int32_t *dst, uint64_t dst_cap, uint64_t *count);
+
+/* generator -> exchange -> unstable_select(less_eq_than(thres)) ->
+ * split(x & 1) -> {aggregation -> gpu_to_cpu -> exchange -> materializer} x 2:
+ * a split with two different parents (the shape of hash_exchange,
+ * src/main9.cu:104-148, with both sides on one GPU).  sums[p] = exact int64
+ * SUM(val WHERE pred <= thres AND (val & 1) == p), counts[p] = those rows;
+ * sums and counts are host arrays of 2. */
+int mgv_ops_split_sums(const int32_t *pred, const int32_t *val, uint64_t n, int32_t thres, uint32_t vector_rows,
+                       int device, uint32_t grid, uint32_t block, int64_t *sums, uint64_t *counts);
+
 /* Build: generator(R) -> exchange -> [select(pred_r <= bound) ->]
  * hashjoin_builder; probe: generator(L) -> exchange -> [select(pred_l <=
  * bound) ->] hashjoin -> gpu_to_cpu -> exchange -> materializer

@@ -7,6 +7,7 @@
 //     aggregation<WARPSIZE, 0>                                    aggregation.cuh:8-30
 //     hashjoin<WARPSIZE> / hashjoin_builder<WARPSIZE>             hashjoin.cuh:8-58
 //     gpu_to_cpu<WARPSIZE, 64, buffer_t *>                        gpu_to_cpu.cuh:14-64
+//     split<WARPSIZE, int32_t> / union_all<WARPSIZE, int32_t>     split.cuh:10-35, union_all.cuh:11-37
 //   host side:
 //     generator, exchange (+ consumer), materializer              generators.cuh, exchange.cuh, materializer.cuh
 //
@@ -27,7 +28,12 @@
 //    columns): select tests column 0 and passes the remaining columns (or
 //    column 0 when alone) to its parent; the probe joins on the first column
 //    it receives and sums the next one plus the build payload;
-//  - split / union_all / mem_move (multi-GPU routing) are not provided.
+//  - a split stages its parents' rows and hands them on as vectors when its
+//    staging fills and at close (the reference forwarded every 128 rows),
+//    partitioning on the first column it receives;
+//  - a union_all must be the first operator of its chain (the parent a
+//    split or an exchange hands vectors to);
+//  - mem_move and union_all_cpu (multi-GPU routing) are not provided.
 #pragma once
 
 #include <atomic>
@@ -135,15 +141,43 @@ public:
     const mgv_result &result() const { return last; }
 };
 
+// Routes each row to parent (x & 1) of its first column x (split.cu:49-118;
+// mgv_split): rows whose destination has no parent are dropped, parents
+// beyond the second receive nothing.  A parent may appear several times.
+template <size_t warp_size = WARPSIZE, typename T = int32_t>
+class split {
+public:
+    std::vector<d_operator_t *> parents;
+    split(std::vector<d_operator_t *> parents_, launch_conf conf, int dev = 0);
+};
+
+// Fan-in of num_of_children children into one parent: the first child's
+// open opens the parent and the last child's close closes it
+// (union_all.cu:19-30, 63-74); children may consume from several threads.
+template <size_t warp_size = WARPSIZE, typename T = int32_t>
+class union_all {
+public:
+    d_operator_t *parent;
+    int num_of_children;
+    int num_of_active_children = 0;
+    int num_of_closed_children = 0;
+    std::mutex host_lock;
+    union_all(d_operator_t *parent_, int num_of_children_, launch_conf conf, int dev = 0);
+};
+
 typedef unstable_select<WARPSIZE, less_eq_than<int32_t>, int32_t> select_t;
 typedef aggregation<WARPSIZE, 0> aggregation_t;
 typedef hashjoin<WARPSIZE> hashjoin_t;
 typedef hashjoin_builder<WARPSIZE> hashjoin_builder_t;
 typedef gpu_to_cpu<WARPSIZE, 64, buffer_t *> gpu_to_cpu_t;
+typedef split<WARPSIZE, int32_t> split_t;
+typedef union_all<WARPSIZE, int32_t> union_all_t;
 
 class d_operator_t {
 public:
-    typedef std::variant<select_t *, aggregation_t *, hashjoin_builder_t *, hashjoin_t *, gpu_to_cpu_t *> op_t;
+    typedef std::variant<select_t *, aggregation_t *, hashjoin_builder_t *, hashjoin_t *, gpu_to_cpu_t *, split_t *,
+                         union_all_t *>
+        op_t;
     op_t op;
     launch_conf conf;
 
@@ -169,10 +203,11 @@ public:
 private:
     d_operator_t(launch_conf lc, op_t o) : op(std::move(o)), conf(lc) {}
 
-    enum Kind { NONE, SELECT_SUM, SELECT_MAT, PROBE, SELECT_PROBE, BUILD, SELECT_BUILD };
+    enum Kind { NONE, SELECT_SUM, SELECT_MAT, PROBE, SELECT_PROBE, BUILD, SELECT_BUILD, SPLIT };
     Kind kind = NONE;
     int32_t bound = 0;
     bool sum = false;
+    bool filtered = false;               // a select heads the chain
     std::shared_ptr<mgv_table_state> table;
     gpu_to_cpu_t *sink = nullptr;
     mgv_result *res = nullptr;           // device
@@ -180,8 +215,18 @@ private:
     uint64_t out_cap = 0, out_upper = 0;
     mgv_stream_t last_stream = nullptr;
     bool opened = false;
+    // split: per-destination staging columns and their append cursors
+    split_t *splitter = nullptr;
+    int32_t *stage = nullptr;            // device, [dest][col][stage_cap]
+    uint64_t *cursors = nullptr;         // device, [dest]
+    uint64_t stage_cap = 0, stage_upper = 0;
+    int stage_cols = 0;
 
     void resolve();
+    void do_open();
+    void do_close();
+    void flush_split(mgv_stream_t s);
+    uint32_t split_dests() const;
 };
 
 // ------------------------------------------------------------ host operators

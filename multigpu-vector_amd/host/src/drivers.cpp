@@ -156,6 +156,80 @@ int mgv_ops_select(const int32_t *pred, const int32_t *val, uint64_t n, int32_t 
     });
 }
 
+int mgv_ops_select_split(const int32_t *pred, const int32_t *val, uint64_t n, int32_t thres, uint32_t vector_rows,
+                         int device, uint32_t grid, uint32_t block, int32_t *dst, uint64_t dst_cap, uint64_t *count) {
+    return guarded("mgv_ops_select_split", [&] {
+        require_manager();
+        check_vector_rows(vector_rows);
+        if (!pred || !val || !count) throw mgv::error(MGV_E_INVALID, "null argument");
+        if (dst && dst_cap < n) throw mgv::error(MGV_E_CAPACITY, "dst must hold n rows");
+        const launch_conf conf = make_conf(grid, block);
+        Sink sink(dst);
+        // create_select_pipeline_exp (main9.cu:239-249): both parents of the
+        // split are the same union_all
+        d_operator_t *g2c = d_operator_t::create<gpu_to_cpu_t>(conf, sink.xroot, conf.device);
+        d_operator_t *g2cu = d_operator_t::create<union_all_t>(conf, g2c, 2, conf, conf.device);
+        d_operator_t *spl = d_operator_t::create<split_t>(conf, std::vector<d_operator_t *>{g2cu, g2cu}, conf,
+                                                          conf.device);
+        d_operator_t *sel = d_operator_t::create<select_t>(conf, spl, less_eq_than<int32_t>(thres),
+                                                           conf.get_blocks_per_grid(), 0);
+        auto cleanup = [&] {
+            d_operator_t::destroy(sel);
+            d_operator_t::destroy(spl);
+            d_operator_t::destroy(g2cu);
+            d_operator_t::destroy(g2c);
+        };
+        try {
+            run_generator(sel, conf, {(int32_t *)pred, (int32_t *)val}, n, vector_rows, device);
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        *count = n ? g2c->get<gpu_to_cpu_t *>()->result().count : 0;
+        cleanup();
+    });
+}
+
+int mgv_ops_split_sums(const int32_t *pred, const int32_t *val, uint64_t n, int32_t thres, uint32_t vector_rows,
+                       int device, uint32_t grid, uint32_t block, int64_t *sums, uint64_t *counts) {
+    return guarded("mgv_ops_split_sums", [&] {
+        require_manager();
+        check_vector_rows(vector_rows);
+        if (!pred || !val || !sums || !counts) throw mgv::error(MGV_E_INVALID, "null argument");
+        const launch_conf conf = make_conf(grid, block);
+        Sink sink0(nullptr), sink1(nullptr);
+        d_operator_t *g2c[2], *aggr[2];
+        g2c[0] = d_operator_t::create<gpu_to_cpu_t>(conf, sink0.xroot, conf.device);
+        g2c[1] = d_operator_t::create<gpu_to_cpu_t>(conf, sink1.xroot, conf.device);
+        for (int p = 0; p < 2; ++p)
+            aggr[p] = d_operator_t::create<aggregation_t>(conf, g2c[p], 8000, conf.get_blocks_per_grid(), 0);
+        d_operator_t *spl = d_operator_t::create<split_t>(conf, std::vector<d_operator_t *>{aggr[0], aggr[1]}, conf,
+                                                          conf.device);
+        d_operator_t *sel = d_operator_t::create<select_t>(conf, spl, less_eq_than<int32_t>(thres),
+                                                           conf.get_blocks_per_grid(), 0);
+        auto cleanup = [&] {
+            d_operator_t::destroy(sel);
+            d_operator_t::destroy(spl);
+            for (int p = 0; p < 2; ++p) {
+                d_operator_t::destroy(aggr[p]);
+                d_operator_t::destroy(g2c[p]);
+            }
+        };
+        try {
+            run_generator(sel, conf, {(int32_t *)pred, (int32_t *)val}, n, vector_rows, device);
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        for (int p = 0; p < 2; ++p) {
+            const mgv_result &r = g2c[p]->get<gpu_to_cpu_t *>()->result();
+            sums[p] = n ? r.sum : 0;
+            counts[p] = n ? r.count : 0;
+        }
+        cleanup();
+    });
+}
+
 int mgv_ops_count_join(const int32_t *R, const int32_t *pred_r, uint64_t rn, const int32_t *L, const int32_t *pred_l,
                        uint64_t ln, int32_t bound, uint32_t vector_rows, int device, uint32_t grid, uint32_t block,
                        int32_t *dst, uint64_t *count) {

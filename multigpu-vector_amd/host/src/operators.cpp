@@ -4,6 +4,7 @@
 #include "mgv/operators.hpp"
 
 #include <chrono>
+#include <algorithm>
 #include <climits>
 #include <cstring>
 
@@ -44,6 +45,28 @@ gpu_to_cpu<WARPSIZE, 64, buffer_t *>::~gpu_to_cpu() {
     mgv_channel_destroy(channel);
 }
 
+// ------------------------------------------------------------------ split / union_all
+
+template <>
+split<WARPSIZE, int32_t>::split(std::vector<d_operator_t *> parents_, launch_conf conf, int dev)
+    : parents(std::move(parents_)) {
+    (void)conf;
+    (void)dev;
+    if (parents.empty() || parents.size() > MGV_SPLIT_MAX_PARENTS)
+        throw mgv::error(MGV_E_INVALID, "split: 1 to 8 parents");
+    for (d_operator_t *p : parents)
+        if (!p) throw mgv::error(MGV_E_INVALID, "split: null parent");
+}
+
+template <>
+union_all<WARPSIZE, int32_t>::union_all(d_operator_t *parent_, int num_of_children_, launch_conf conf, int dev)
+    : parent(parent_), num_of_children(num_of_children_) {
+    (void)conf;
+    (void)dev;
+    if (!parent) throw mgv::error(MGV_E_INVALID, "union_all: null parent");
+    if (num_of_children < 1) throw mgv::error(MGV_E_INVALID, "union_all: at least one child");
+}
+
 // ------------------------------------------------------------------ d_operator_t
 
 namespace {
@@ -59,8 +82,13 @@ d_operator_t *parent_of(const d_operator_t::op_t &op) {
 
 void d_operator_t::resolve() {
     // the chain from this operator up to its sink
+    // a union_all passes its children's vectors to its parent's chain
     std::vector<d_operator_t *> chain;
-    for (d_operator_t *c = this; c; c = parent_of(c->op)) {
+    d_operator_t *start = this;
+    if (auto u = std::get_if<union_all_t *>(&op)) start = (*u)->parent;
+    for (d_operator_t *c = start; c; c = parent_of(c->op)) {
+        if (std::holds_alternative<union_all_t *>(c->op))
+            throw mgv::error(MGV_E_INVALID, "a union_all must be the first operator of its chain");
         chain.push_back(c);
         if (chain.size() > 8) throw mgv::error(MGV_E_INVALID, "operator chain too long or cyclic");
     }
@@ -74,6 +102,13 @@ void d_operator_t::resolve() {
         sel = true;
         bound = std::get<select_t *>(chain[0]->op)->filt.bound;
         i = 1;
+    }
+    filtered = sel;
+    if (is(i, (split_t *)nullptr)) {
+        // [select ->] split: the split's parents are the heads of their own chains
+        kind = SPLIT;
+        splitter = std::get<split_t *>(chain[i]->op);
+        return;
     }
     if (is(i, (hashjoin_builder_t *)nullptr) && chain.size() == i + 1) {
         kind = sel ? SELECT_BUILD : BUILD;
@@ -105,9 +140,51 @@ void d_operator_t::resolve() {
     sink = std::get<gpu_to_cpu_t *>(chain[i]->op);
 }
 
+uint32_t d_operator_t::split_dests() const {
+    const size_t np = splitter->parents.size();
+    return np < 2 ? (uint32_t)np : 2u;   // x & 1 names parents 0 and 1 only
+}
+
 void d_operator_t::open() {
+    if (auto u = std::get_if<union_all_t *>(&op)) {
+        // the first child opens the parent (union_all.cu:19-30)
+        std::lock_guard<std::mutex> g((*u)->host_lock);
+        if ((*u)->num_of_active_children++ == 0) do_open();
+        return;
+    }
+    do_open();
+}
+
+void d_operator_t::close() {
+    if (auto u = std::get_if<union_all_t *>(&op)) {
+        // the last child closes the parent (union_all.cu:63-74)
+        std::lock_guard<std::mutex> g((*u)->host_lock);
+        if (++(*u)->num_of_closed_children < (*u)->num_of_children) return;
+        (*u)->num_of_active_children = 0;
+        (*u)->num_of_closed_children = 0;
+        do_close();
+        return;
+    }
+    do_close();
+}
+
+void d_operator_t::do_open() {
     if (opened) return;
     if (kind == NONE) resolve();
+    if (kind == SPLIT) {
+        if (!cursors) {
+            void *p = nullptr;
+            gpu(mgv_malloc(&p, MGV_SPLIT_MAX_PARENTS * sizeof(uint64_t)));
+            cursors = (uint64_t *)p;
+        }
+        gpu(mgv_fill_u64(cursors, MGV_SPLIT_MAX_PARENTS, 0, nullptr));
+        gpu(mgv_stream_synchronize(nullptr));
+        stage_upper = 0;
+        opened = true;
+        // before_open opens every parent (split.cu:22-34)
+        for (d_operator_t *p : splitter->parents) p->open();
+        return;
+    }
     if (sink) {
         // before_open: open the host parent and start the catcher
         sink->parent->open();
@@ -165,7 +242,9 @@ void d_operator_t::open() {
 }
 
 void d_operator_t::consume(buffer_t *v, mgv_stream_t s) {
-    if (!opened) open();
+    std::unique_lock<std::mutex> fan_in;
+    if (auto u = std::get_if<union_all_t *>(&op)) fan_in = std::unique_lock<std::mutex>((*u)->host_lock);
+    if (!opened) do_open();
     last_stream = s;
     const uint64_t n = v->count();
     if (n == 0) return;
@@ -178,7 +257,8 @@ void d_operator_t::consume(buffer_t *v, mgv_stream_t s) {
             break;
         case SELECT_MAT: {
             if (out_upper + n > out_cap) {   // grow the output (rare: doubling)
-                gpu(mgv_stream_synchronize(s));
+                // every stream: the children of a union_all write from several
+                gpu(mgv_device_synchronize());
                 uint64_t cap = out_cap ? out_cap : (1u << 20);
                 while (cap < out_upper + n) cap *= 2;
                 void *p = nullptr;
@@ -212,13 +292,77 @@ void d_operator_t::consume(buffer_t *v, mgv_stream_t s) {
                                    nc > 2 ? MGV_PAYLOAD_COLUMN : MGV_PAYLOAD_NONE, c[0], bound, n, 0, nullptr, L, s));
             break;
         }
+        case SPLIT: {
+            // the select's filter column, then the columns the split carries
+            // (the first decides the destination)
+            const int32_t *pred = filtered ? c[0] : nullptr;
+            int32_t *const *car = (filtered && nc > 1) ? c + 1 : c;
+            const int ncar = (filtered && nc > 1) ? nc - 1 : nc;
+            const uint32_t nd = split_dests();
+            if (!stage || stage_cols != ncar || n > stage_cap) {
+                flush_split(s);
+                gpu(mgv_stream_synchronize(s));
+                gpu(mgv_free(stage));
+                stage = nullptr;
+                const uint64_t cap = std::max<uint64_t>(2 * n, buffer_t::capacity());
+                void *p = nullptr;
+                gpu(mgv_malloc(&p, cap * nd * (uint64_t)ncar * sizeof(int32_t)));
+                stage = (int32_t *)p;
+                stage_cap = cap;
+                stage_cols = ncar;
+            }
+            if (stage_upper + n > stage_cap) flush_split(s);
+            int32_t *outs[MGV_SPLIT_MAX_PARENTS * MGV_SPLIT_MAX_COLS] = {};
+            for (uint32_t d = 0; d < nd; ++d)
+                for (int k = 0; k < ncar; ++k) outs[d * ncar + k] = stage + (d * (uint64_t)ncar + k) * stage_cap;
+            gpu(mgv_split(pred, bound, (const int32_t *const *)car, (uint32_t)ncar, n, nd, outs, stage_cap, cursors,
+                          L, s));
+            stage_upper += n;   // an upper bound of every destination's staged rows
+            break;
+        }
         default:
             throw mgv::error(MGV_E_INVALID, "d_operator_t::consume: operator is not the head of a chain");
     }
 }
 
-void d_operator_t::close() {
+// Hands the staged rows of every destination to its parent as vectors of at
+// most the buffer capacity, on stream s, then empties the staging (the
+// reference's split forwards each full 4*32-row chunk at once, split.cu:95-107,
+// and what is left at close, 131-150).
+void d_operator_t::flush_split(mgv_stream_t s) {
+    if (stage_upper == 0) return;
+    gpu(mgv_stream_synchronize(s));
+    uint64_t cnt[MGV_SPLIT_MAX_PARENTS] = {};
+    const uint32_t nd = split_dests();
+    gpu(mgv_memcpy(cnt, cursors, nd * sizeof(uint64_t)));
+    const uint64_t cap = buffer_t::capacity();
+    for (uint32_t d = 0; d < nd; ++d) {
+        if (cnt[d] > stage_cap) throw mgv::error(MGV_E_CAPACITY, "split: staging overflow");
+        for (uint64_t off = 0; off < cnt[d]; off += cap) {
+            int32_t *cols[buffer_t::kMaxCols] = {};
+            for (int k = 0; k < stage_cols; ++k) cols[k] = stage + (d * (uint64_t)stage_cols + k) * stage_cap + off;
+            buffer_t b(cols[0], conf.device);
+            b.set_columns(cols, stage_cols);
+            b.cnt = (uint32_t)std::min<uint64_t>(cap, cnt[d] - off);
+            splitter->parents[d]->consume(&b, s);
+        }
+    }
+    // the parents' launches on s read the staging before this clear and the
+    // next split launch
+    gpu(mgv_fill_u64(cursors, nd, 0, s));
+    stage_upper = 0;
+}
+
+void d_operator_t::do_close() {
     if (!opened) return;
+    if (kind == SPLIT) {
+        flush_split(last_stream);   // at_close (split.cu:131-150)
+        gpu(mgv_stream_synchronize(last_stream));
+        opened = false;
+        // after_close closes every parent (split.cu:152-163)
+        for (d_operator_t *p : splitter->parents) p->close();
+        return;
+    }
     gpu(mgv_stream_synchronize(last_stream));
     gpu(mgv_device_synchronize());
     if (sink) {
@@ -242,6 +386,8 @@ void d_operator_t::close() {
 d_operator_t::~d_operator_t() {
     mgv_free(res);
     mgv_free(out);
+    mgv_free(stage);
+    mgv_free(cursors);
 }
 
 void d_operator_t::destroy(d_operator_t *op) {

@@ -171,6 +171,7 @@ SIGNATURES = {
     "mgv_hashjoin_build_lds": [VP, U32, VP, VP, I32, VP, I32, U64, I64, U32, VP, U64, VP, Launch, VP],
     "mgv_hashjoin_probe_lds": [VP, U32, VP, I32, VP, VP, U64, U32, VP, U64, VP, Launch, VP],
     "mgv_groupby_partitioned": [VP, VP, U64, VP, VP, U32, VP, U64, VP, Launch, VP],
+    "mgv_split": [VP, I32, VP, U32, U64, U32, VP, U64, VP, Launch, VP],
 }
 POOL_END = -1
 POOL_RETRY = -2
@@ -244,6 +245,8 @@ OPS_SIGNATURES = {
     "mgv_ops_last_error": [],
     "mgv_ops_sum_select": [VP, VP, U64, I32, U32, C.c_int, U32, U32, VP, C.POINTER(I64)],
     "mgv_ops_select": [VP, VP, U64, I32, C.c_int, U32, C.c_int, U32, U32, VP, U64, C.POINTER(U64)],
+    "mgv_ops_select_split": [VP, VP, U64, I32, U32, C.c_int, U32, U32, VP, U64, C.POINTER(U64)],
+    "mgv_ops_split_sums": [VP, VP, U64, I32, U32, C.c_int, U32, U32, VP, VP],
     "mgv_ops_count_join": [VP, VP, U64, VP, VP, U64, I32, U32, C.c_int, U32, U32, VP, C.POINTER(U64)],
     "mgv_ops_run_plain": [VP, U32, U32, Launch, VP],
     "mgv_serve_run": [VP, U32, VP, VP, VP],

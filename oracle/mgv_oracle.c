@@ -395,3 +395,18 @@ int64_t orc_selfjoin_count(const int32_t *x, uint64_t n, int32_t bound, uint32_t
     free(hist);
     return s;
 }
+
+/* split.cu:54-118: the destination of a row is hash3 of its value (49-51);
+ * consume_warp only ballots destinations 0 .. num_of_parents-1, so a row
+ * whose destination has no parent is never written. */
+void orc_split(const int32_t *pred, int32_t bound, const int32_t *const *cols, uint32_t ncols, uint64_t n,
+               uint32_t nparents, int32_t *const *out, uint64_t *counts) {
+    for (uint32_t p = 0; p < nparents; ++p) counts[p] = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (pred && pred[i] > bound) continue;
+        const uint32_t d = (uint32_t)(cols[0][i] & 1);
+        if (d >= nparents) continue;
+        for (uint32_t c = 0; c < ncols; ++c) out[d * ncols + c][counts[d]] = cols[c][i];
+        ++counts[d];
+    }
+}

@@ -20,6 +20,10 @@
  *                       builds; nested-loop brute force at <= 2^12 rows;
  *                       self-join sum_v hist[v]^2 for duplicate keys.
  *   orc_groupby_*       numpy bincount; sum over groups == total.
+ *   orc_split           partition invariants: the parents' rows are a
+ *                       partition of the kept rows, parent p holds exactly
+ *                       the rows with x % 2 == p (Python modulo), one parent
+ *                       keeps only the even rows, parents >= 2 stay empty.
  * No function is "parity unpinned".
  */
 #ifndef MGV_ORACLE_H_
@@ -149,6 +153,14 @@ void     orc_groupby(const int32_t *key, const int32_t *val, uint64_t n,
 /* Self-join count, sum_selfjoin_select_cpu (main9.cu:63-69): sum over values v
  * with v <= bound of hist[v]^2, for values in [0, vmax].  Returns int64; the
  * reference's int32 result is its low 32 bits. */
+/* split (src/operators/split.cu:54-118; hash3(x) = x & 1 at 49-51): row i
+ * (kept when pred is NULL or pred[i] <= bound) goes to parent cols[0][i] & 1
+ * when that is < nparents, else it is dropped.  Parent p's rows are written
+ * in input order to out[p * ncols + c] (c < ncols); counts[p] = its rows.
+ * The outputs must hold n rows each. */
+void     orc_split(const int32_t *pred, int32_t bound, const int32_t *const *cols, uint32_t ncols, uint64_t n,
+                   uint32_t nparents, int32_t *const *out, uint64_t *counts);
+
 int64_t  orc_selfjoin_count(const int32_t *x, uint64_t n, int32_t bound, uint32_t vmax);
 
 #ifdef __cplusplus

@@ -56,6 +56,7 @@ class Oracle:
         L.orc_fmix32.restype = U32
         L.orc_mix64.argtypes = [U64]
         L.orc_mix64.restype = U64
+        L.orc_split.argtypes = [VP, I32, VP, U32, U64, U32, VP, VP]
         L.orc_select.argtypes = [VP, U64, I32, VP]
         L.orc_select.restype = U64
         L.orc_select_sum.argtypes = [VP, VP, U64, I32, C.POINTER(U64), C.POINTER(C.c_int64), C.POINTER(I32)]
@@ -165,6 +166,19 @@ class Oracle:
 
     def selfjoin_count(self, x, bound, vmax):
         return self.L.orc_selfjoin_count(_ptr(x), len(x), bound, vmax)
+
+    def split(self, cols, nparents, pred=None, bound=0):
+        """Rows of each parent, in input order: a list of nparents lists of
+        ncols int32 arrays."""
+        cols = [np.ascontiguousarray(c, dtype=np.int32) for c in cols]
+        n, nc = len(cols[0]), len(cols)
+        outs = [np.zeros(max(n, 1), np.int32) for _ in range(nparents * nc)]
+        cnt = np.zeros(nparents, np.uint64)
+        cp = (VP * nc)(*[c.ctypes.data for c in cols])
+        op = (VP * len(outs))(*[o.ctypes.data for o in outs])
+        pr = np.ascontiguousarray(pred, dtype=np.int32) if pred is not None else None
+        self.L.orc_split(_ptr(pr) if pr is not None else None, bound, cp, nc, n, nparents, op, _ptr(cnt))
+        return [[outs[p * nc + c][: int(cnt[p])].copy() for c in range(nc)] for p in range(nparents)]
 
 
 def col_check(x: np.ndarray) -> dict:

@@ -91,6 +91,49 @@ def test_select_plan_multiset(ops, orc, where, apply_select):
     del keep
 
 
+# host-resident vectors are copied into the manager's device buffers, which
+# this module's mgv_ops_init makes too small for 4 Mi-row vectors
+@pytest.mark.parametrize("n,vector_rows,where", [(200003, 8192, 0), (200003, 8192, -1), (9000000, 65536, 0),
+                                                 (9000000, 65536, -1), (5000000, 4 << 20, 0), (0, 4096, 0),
+                                                 (0, 4096, -1)])
+def test_select_split_plan_multiset(ops, orc, n, vector_rows, where):
+    """create_select_pipeline_exp: select -> split -> union_all (both split
+    parents) -> gpu_to_cpu gives the select's rows; 9e6 rows of 65536-row
+    vectors overflow the split's 4 Mi-row staging twice, so its parents get
+    flushed vectors mid-stream as well as at close."""
+    a = orc.gen_bits(n, orc.seed(1, 4), 2, 31) - (1 << 29)     # negative values too
+    b = orc.gen_bits(n, orc.seed(1, 5), 2, 31)
+    bound = (1 << 30) - 1
+    (pb, pa), keep = cols(where, b, a)
+    dst = np.full(n + 1, -7, np.int32)
+    cnt = C.c_uint64()
+    mgv.ops_call("mgv_ops_select_split", pb, pa, n, bound, vector_rows, where, 64, 256, host_ptr(dst), n + 1,
+                 C.byref(cnt))
+    want = a[b <= bound]
+    assert cnt.value == len(want)
+    assert np.array_equal(np.sort(dst[:cnt.value]), np.sort(want))
+    assert np.all(dst[cnt.value:] == -7)
+    del keep
+
+
+@pytest.mark.parametrize("n,vector_rows", [(300001, 4096), (6000000, 1 << 20)])
+def test_split_sums_plan(ops, orc, n, vector_rows):
+    """select -> split -> {aggregation -> gpu_to_cpu} x 2: each parent sums
+    the selected rows of its parity, as the oracle's split routes them."""
+    a = orc.gen_bits(n, orc.seed(1, 6), 2, 31) - (1 << 30)
+    b = orc.gen_bits(n, orc.seed(1, 7), 2, 31)
+    bound = (1 << 30) - 1
+    (pb, pa), keep = cols(0, b, a)
+    sums = np.zeros(2, np.int64)
+    counts = np.zeros(2, np.uint64)
+    mgv.ops_call("mgv_ops_split_sums", pb, pa, n, bound, vector_rows, 0, 64, 1024, host_ptr(sums),
+                 host_ptr(counts))
+    parts = orc.split([a], 2, pred=b, bound=bound)
+    assert counts.tolist() == [len(parts[0][0]), len(parts[1][0])]
+    assert sums.tolist() == [int(parts[p][0].astype(np.int64).sum()) for p in range(2)]
+    del keep
+
+
 def test_select_plan_rejects_small_dst(ops):
     x = dev(np.arange(1024, dtype=np.int32))
     dst = np.zeros(10, np.int32)

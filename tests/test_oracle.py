@@ -302,3 +302,40 @@ def test_groupby_golden(orc, golden):
     assert col_check(cnt.astype(np.int64)) == g["gcnt_hash"]
     assert col_check(s) == g["gsum_hash"]
     assert int(s.sum()) == g["total_sum"] == int(v.astype(np.int64).sum())
+
+
+# ----------------------------------------------------------------- split
+
+def _rows(cols):
+    """Rows of a column list as a sorted list of tuples (a multiset)."""
+    return sorted(zip(*[c.tolist() for c in cols])) if len(cols[0]) else []
+
+
+@pytest.mark.parametrize("nparents", [1, 2, 3, 8])
+@pytest.mark.parametrize("ncols,filtered", [(1, False), (2, True), (4, False), (3, True)])
+def test_split_partitions_the_kept_rows_by_parity(orc, nparents, ncols, filtered):
+    n = 5000
+    rng = np.random.default_rng(11 + ncols)
+    cols = [rng.integers(-2**31, 2**31, n, dtype=np.int64).astype(np.int32) for _ in range(ncols)]
+    cols[0][:7] = [0, -1, 1, -2, 2**31 - 1, -2**31, 3]     # signs and extremes of the parity test
+    pred = rng.integers(0, 100, n).astype(np.int32) if filtered else None
+    bound = 49
+    parts = orc.split(cols, nparents, pred=pred, bound=bound)
+    keep = np.ones(n, bool) if pred is None else pred <= bound
+    par = np.array([int(x) % 2 for x in cols[0]])           # Python modulo: independent of the C & 1
+    for p in range(nparents):
+        sel = keep & (par == p)
+        if p >= 2:
+            assert len(parts[p][0]) == 0                     # x & 1 never names parent 2 or more
+            continue
+        # parent p: exactly the kept rows of that parity, in input order
+        for c in range(ncols):
+            assert np.array_equal(parts[p][c], cols[c][sel])
+    got = sorted(_rows(parts[0]) + (_rows(parts[1]) if nparents > 1 else []))
+    want = _rows([c[keep & ((par == 0) | (nparents > 1))] for c in cols])
+    assert got == want                                       # nothing lost or duplicated
+
+
+def test_split_of_nothing(orc):
+    parts = orc.split([np.zeros(0, np.int32)], 2)
+    assert [len(p[0]) for p in parts] == [0, 0]
