@@ -407,6 +407,52 @@ def c1_fused_batch(args, st, bound):
     return best
 
 
+# ------------------------------------------------------------------ split
+
+def leg_f3(args, st):
+    """The split operator (mgv_split): 2^28 rows routed to two parents by
+    parity, (a) one column and no filter, 8 B per row (4 read, 4 written);
+    (b) pred <= 2^30-1 then the value column, selectivity 1/2, 10 B per row
+    (8 read, 2 written on average).  Cursors are cleared outside the timed
+    launch; the kept-row total is checked against n and mgv_select_sum."""
+    out = {}
+    n = 1 << 28
+    bound = (1 << 30) - 1
+    x = gen_bits(n, mgv.seed(6, 0), 0, 32, st.vp)
+    pred = gen_bits(n, mgv.seed(6, 1), 0, 31, st.vp)
+    outs = [mgv.DeviceBuffer(4 * n) for _ in range(2)]
+    cnt = mgv.DeviceBuffer(64)
+    res = mgv.ResultBuffer(1)
+    mgv.call("mgv_select_sum", pred.vp, x.vp, n, bound, res.vp, mgv.launch(0, 1024, 0), st.vp)
+    st.sync()
+    selected = res.read(0).count
+    cp = (C.c_void_p * 1)(x.ptr)
+    op = (C.c_void_p * 2)(outs[0].ptr, outs[1].ptr)
+    for name, p, bytes_per_row, want in (("split_2^28", None, 8.0, n), ("split_2^28_filtered", pred, 10.0, selected)):
+        times = []
+        for rep in range(5):
+            mgv.call("mgv_fill_u64", cnt.vp, 2, 0, st.vp)
+            t = Timer(st.vp)
+            t.around(lambda: mgv.call("mgv_split", p.vp if p else None, bound, cp, 1, n, 2, op, n, cnt.vp,
+                                      mgv.launch(0, 1024), st.vp))
+            st.sync()
+            times.append(t.durations_ms()[0])
+        c = cnt.download(np.uint64, 2)
+        ms = min(times[1:])
+        algo = n * bytes_per_row
+        gbps = algo / (ms / 1e3) / 1e9
+        traffic = traffic_from_profiles(name)
+        out[name] = {"ms": ms, "grows_per_s": n / (ms / 1e3) / 1e9, "gbps_algorithmic": gbps,
+                     "frac": gbps / HBM_PEAK_GBPS, "bytes_per_row": bytes_per_row, "traffic": traffic,
+                     "traffic_gbps": traffic / (ms / 1e3) / 1e9 if traffic else None,
+                     "rows_per_parent": [int(v) for v in c], "kept_rows_ok": int(c.sum()) == want}
+        log(f"F3 {name}: {ms:.4f} ms, {gbps:.0f} GB/s algorithmic ({gbps / HBM_PEAK_GBPS:.0%}), "
+            f"rows {c.tolist()} ok {int(c.sum()) == want}")
+    for b in (x, pred, cnt, res, *outs):
+        b.close()
+    return out
+
+
 # ------------------------------------------------------------------ C3 skew
 
 def leg_c3(args, st):
@@ -677,7 +723,7 @@ def main():
                     help="probe path (auto = time the candidates, keep the fastest)")
     ap.add_argument("--fanout", type=int, default=-1, help="fan-out of the partitioned paths (with --probe)")
     ap.add_argument("--scatter", type=int, default=-1, help="scatter workgroup shape MGV_SCATTER_* (-1 = tuned)")
-    ap.add_argument("--legs", default="c1,c3,c4,c5")
+    ap.add_argument("--legs", default="c1,c3,c4,c5,f3")
     ap.add_argument("--c5-requests", type=int, default=300)
     ap.add_argument("--c4-reps", type=int, default=5)
     ap.add_argument("--cpu", type=int, default=1)
@@ -705,6 +751,8 @@ def main():
     if "c4" in legs:
         mgv.load_ops()
         extra["c4"] = leg_c4(args, st)
+    if "f3" in legs:
+        extra["f3"] = leg_f3(args, st)
     if "c5" in legs:
         mgv.load_ops()
         extra["c5"] = leg_c5(args, st)

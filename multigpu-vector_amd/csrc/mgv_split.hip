@@ -10,11 +10,11 @@
 // parents beyond the second receive nothing.
 //
 // Here a split appends each destination's rows to that parent's output
-// columns: one block of 1024 threads takes a tile of 8192 rows, every wave
-// ballots its rows per destination, the block adds up its waves' counts in
-// LDS and reserves one contiguous run per destination with a single global
-// atomic, and each wave writes its rows into its part of the run (each
-// ballot's rows to consecutive positions).  A vector is read once and each
+// columns: one block of 1024 threads takes a tile of 32768 rows (8192 with
+// two to four columns), every wave ballots its rows per destination, the
+// block adds up its waves' counts in LDS and reserves one contiguous run per
+// destination with a single global atomic, and each wave writes its rows
+// into its part of the run (each ballot's rows to consecutive positions).  A vector is read once and each
 // kept row written once: 4 * ncols bytes read per row (plus 4 for a filter
 // column) and 4 * ncols written per kept row, HBM-bound.  The order of the
 // rows inside a destination is unspecified, as it is in the reference.
@@ -26,10 +26,16 @@
 namespace mgv {
 
 constexpr int kSplitBlock = 1024;
-constexpr int kSplitRows = 8;                                  // rows per lane per tile
 constexpr uint32_t kSplitWaves = kSplitBlock / kWave;
-constexpr uint64_t kSplitTile = (uint64_t)kSplitBlock * kSplitRows;   // 8192 rows
 constexpr int kSplitDests = 2;                                 // x & 1 has two values
+
+// Rows per lane per tile.  One cursor atomic per destination per tile: with
+// 8192-row tiles of one column the atomics on the two cursors, not HBM, set
+// the pace, so a one-column tile has 32768 rows (127 VGPRs of the 128 a
+// 1024-thread block allows); with two to four columns, 32 rows per lane
+// spill, and tiles keep 8192 rows.
+template <int NC>
+constexpr int split_rows() { return NC == 1 ? 32 : 8; }
 
 struct SplitArgs {
     const int32_t *pred;
@@ -44,37 +50,55 @@ struct SplitArgs {
 
 template <int NC>
 __global__ __launch_bounds__(kSplitBlock) void k_split(SplitArgs a) {
+    constexpr int R = split_rows<NC>();
+    constexpr uint64_t kTile = (uint64_t)kSplitBlock * R;
     __shared__ uint32_t wcnt[kSplitDests][kSplitWaves];
     __shared__ unsigned long long sbase[kSplitDests];
     const uint32_t lane = lane_id();
     const uint32_t w = wave_id();
-    const uint64_t ntiles = (a.n + kSplitTile - 1) / kSplitTile;
+    const uint64_t ntiles = (a.n + kTile - 1) / kTile;
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        // row of (lane, k): consecutive lanes read consecutive rows
-        const uint64_t r0 = t * kSplitTile + (uint64_t)w * (kWave * kSplitRows) + lane;
-        int32_t v[NC][kSplitRows];
-        int dest[kSplitRows];
+        // Tile-uniform base pointers and 32-bit row offsets inside the tile
+        // (one scalar base and one offset register per access, not a 64-bit
+        // address per row); row (lane, k) of the wave is w*64*R + k*64 + lane.
+        const uint64_t first = t * kTile;
+        const uint32_t rows = (uint32_t)(a.n - first < kTile ? a.n - first : kTile);
+        const uint32_t off0 = w * (kWave * R) + lane;
+        int32_t v[NC][R];
+        int32_t pv[R];
 #pragma unroll
-        for (int k = 0; k < kSplitRows; ++k) {
-            const uint64_t r = r0 + (uint64_t)k * kWave;
-            const bool ok = r < a.n;
+        for (int k = 0; k < R; ++k) {
+            const uint32_t off = off0 + (uint32_t)k * kWave;
+            const bool ok = off < rows;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) v[c][k] = ok ? a.col[c][r] : 0;
-            bool keep = ok;
-            if (a.pred) keep = keep && (ok ? a.pred[r] : 0) <= a.bound;
-            const int d = v[0][k] & 1;            // hash3 (split.cu:49-51)
-            dest[k] = (keep && (uint32_t)d < a.ndest) ? d : -1;
+            for (int c = 0; c < NC; ++c) v[c][k] = ok ? a.col[c][first + off] : 0;
         }
-        uint64_t b[kSplitDests][kSplitRows];
+        if (a.pred) {
 #pragma unroll
-        for (int d = 0; d < kSplitDests; ++d) {
-            uint32_t tot = 0;
-#pragma unroll
-            for (int k = 0; k < kSplitRows; ++k) {
-                b[d][k] = ballot(dest[k] == d);
-                tot += popc64(b[d][k]);
+            for (int k = 0; k < R; ++k) {
+                const uint32_t off = off0 + (uint32_t)k * kWave;
+                pv[k] = off < rows ? a.pred[first + off] : 0;
             }
-            if (lane == 0) wcnt[d][w] = tot;
+        }
+        // Destinations as per-lane bit masks (bit k = row k): hash3
+        // (split.cu:49-51) is the low bit of the row's first column; rows
+        // outside the vector, filtered out or without a parent are in
+        // neither mask.  Plain bit operations here, so no lane mask is shared
+        // between the count and the write phase (64 of them would spill).
+        uint32_t keep = 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const bool ok = off0 + (uint32_t)k * kWave < rows && (!a.pred || pv[k] <= a.bound);
+            keep |= (uint32_t)ok << k;
+        }
+        uint32_t odd = 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) odd |= ((uint32_t)v[0][k] & 1u) << k;
+        const uint32_t dmask[kSplitDests] = {keep & ~odd, a.ndest > 1 ? keep & odd : 0u};
+        const uint64_t tot = wave_sum_u64((uint64_t)__popc(dmask[0]) | ((uint64_t)__popc(dmask[1]) << 32));
+        if (lane == 0) {
+            wcnt[0][w] = (uint32_t)tot;
+            wcnt[1][w] = (uint32_t)(tot >> 32);
         }
         __syncthreads();
         if (threadIdx.x < kSplitDests) {
@@ -88,14 +112,20 @@ __global__ __launch_bounds__(kSplitBlock) void k_split(SplitArgs a) {
         for (int d = 0; d < kSplitDests; ++d) {
             uint64_t base = sbase[d];
             for (uint32_t i = 0; i < w; ++i) base += wcnt[d][i];
+            // this wave's run starts at `base`; positions >= cap are not written
+            const uint64_t room64 = base < a.cap ? a.cap - base : 0;
+            const uint32_t room = room64 < 0xffffffffull ? (uint32_t)room64 : 0xffffffffu;
+            uint32_t at = 0;                           // rows of this wave's run so far
 #pragma unroll
-            for (int k = 0; k < kSplitRows; ++k) {
-                const uint64_t pos = base + mbcnt(b[d][k]);
-                if (dest[k] == d && pos < a.cap) {
+            for (int k = 0; k < R; ++k) {
+                const bool mine = (dmask[d] >> k) & 1u;
+                const uint64_t b = ballot(mine);
+                const uint32_t pos = at + mbcnt(b);
+                if (mine && pos < room) {
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) a.out[d][c][pos] = v[c][k];
+                    for (int c = 0; c < NC; ++c) a.out[d][c][base + pos] = v[c][k];
                 }
-                base += popc64(b[d][k]);
+                at += popc64(b);
             }
         }
         __syncthreads();   // wcnt and sbase are rewritten by the next tile
@@ -133,7 +163,8 @@ int mgv_split(const int32_t *pred, int32_t bound, const int32_t *const *cols, ui
             a.out[d][c] = out[d * ncols + c];
         }
     if (n == 0) return MGV_OK;
-    const uint64_t ntiles = (n + kSplitTile - 1) / kSplitTile;
+    const uint64_t tile = (uint64_t)kSplitBlock * (ncols == 1 ? split_rows<1>() : split_rows<4>());
+    const uint64_t ntiles = (n + tile - 1) / tile;
     static thread_local int cached = 0;
     if (!cached) cached = persistent_blocks(k_split<4>, kSplitBlock);
     uint64_t grid = L.grid ? L.grid : (uint64_t)cached;

@@ -95,7 +95,7 @@ def test_split_appends_to_the_cursors(gpu, orc):
 
 @pytest.mark.parametrize("grid", [1, 3, 7])
 def test_split_small_grids_cover_every_tile(gpu, orc, grid):
-    n = 8192 * 9 + 5
+    n = 32768 * 9 + 5   # nine whole tiles of one column and a ragged one
     cols = seeded_cols(orc, n, 1, 40)
     want = orc.split(cols, 2)
     s = Split(2, 1, n)
