@@ -413,7 +413,8 @@ def leg_f3(args, st):
     """The split operator (mgv_split): 2^28 rows routed to two parents by
     parity, (a) one column and no filter, 8 B per row (4 read, 4 written);
     (b) pred <= 2^30-1 then the value column, selectivity 1/2, 10 B per row
-    (8 read, 2 written on average).  Cursors are cleared outside the timed
+    (8 read, 2 written on average); (c) two carried columns, no filter,
+    16 B per row.  Cursors are cleared outside the timed
     launch; the kept-row total is checked against n and mgv_select_sum."""
     out = {}
     n = 1 << 28
@@ -426,14 +427,21 @@ def leg_f3(args, st):
     mgv.call("mgv_select_sum", pred.vp, x.vp, n, bound, res.vp, mgv.launch(0, 1024, 0), st.vp)
     st.sync()
     selected = res.read(0).count
-    cp = (C.c_void_p * 1)(x.ptr)
-    op = (C.c_void_p * 2)(outs[0].ptr, outs[1].ptr)
-    for name, p, bytes_per_row, want in (("split_2^28", None, 8.0, n), ("split_2^28_filtered", pred, 10.0, selected)):
+    # one column; or (x, pred) carried as two columns into four outputs
+    cp1 = (C.c_void_p * 1)(x.ptr)
+    op1 = (C.c_void_p * 2)(outs[0].ptr, outs[1].ptr)
+    extra_outs = [mgv.DeviceBuffer(4 * n) for _ in range(2)]
+    cp2 = (C.c_void_p * 2)(x.ptr, pred.ptr)
+    op2 = (C.c_void_p * 4)(outs[0].ptr, extra_outs[0].ptr, outs[1].ptr, extra_outs[1].ptr)
+    for name, p, ncols, bytes_per_row, want in (("split_2^28", None, 1, 8.0, n),
+                                                ("split_2^28_filtered", pred, 1, 10.0, selected),
+                                                ("split_2^28_2cols", None, 2, 16.0, n)):
+        cp, op = (cp1, op1) if ncols == 1 else (cp2, op2)
         times = []
         for rep in range(5):
             mgv.call("mgv_fill_u64", cnt.vp, 2, 0, st.vp)
             t = Timer(st.vp)
-            t.around(lambda: mgv.call("mgv_split", p.vp if p else None, bound, cp, 1, n, 2, op, n, cnt.vp,
+            t.around(lambda: mgv.call("mgv_split", p.vp if p else None, bound, cp, ncols, n, 2, op, n, cnt.vp,
                                       mgv.launch(0, 1024), st.vp))
             st.sync()
             times.append(t.durations_ms()[0])
@@ -448,7 +456,7 @@ def leg_f3(args, st):
                      "rows_per_parent": [int(v) for v in c], "kept_rows_ok": int(c.sum()) == want}
         log(f"F3 {name}: {ms:.4f} ms, {gbps:.0f} GB/s algorithmic ({gbps / HBM_PEAK_GBPS:.0%}), "
             f"rows {c.tolist()} ok {int(c.sum()) == want}")
-    for b in (x, pred, cnt, res, *outs):
+    for b in (x, pred, cnt, res, *outs, *extra_outs):
         b.close()
     return out
 

@@ -29,11 +29,10 @@ constexpr int kSplitBlock = 1024;
 constexpr uint32_t kSplitWaves = kSplitBlock / kWave;
 constexpr int kSplitDests = 2;                                 // x & 1 has two values
 
-// Rows per lane per tile.  One cursor atomic per destination per tile: with
-// 8192-row tiles of one column the atomics on the two cursors, not HBM, set
-// the pace, so a one-column tile has 32768 rows (127 VGPRs of the 128 a
-// 1024-thread block allows); with two to four columns, 32 rows per lane
-// spill, and tiles keep 8192 rows.
+// Rows per lane per tile; each tile costs one cursor atomic per destination.
+// A one-column tile has 32768 rows (127 VGPRs of the 128 a 1024-thread block
+// allows); with two to four columns 16 or 32 rows per lane spill, and tiles
+// keep 8192 rows.
 template <int NC>
 constexpr int split_rows() { return NC == 1 ? 32 : 8; }
 
