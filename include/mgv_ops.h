@@ -67,6 +67,21 @@ int mgv_ops_select_split(const int32_t *pred, const int32_t *val, uint64_t n, in
 int mgv_ops_split_sums(const int32_t *pred, const int32_t *val, uint64_t n, int32_t thres, uint32_t vector_rows,
                        int device, uint32_t grid, uint32_t block, int64_t *sums, uint64_t *counts);
 
+/* Multi-stream fan-in: generator -> exchange with nstreams consumers (each
+ * with its own streams, vectors going to whichever is free) -> one
+ * unstable_select(less_eq_than(thres)) per consumer -> union_all(nstreams)
+ * -> [aggregation ->] gpu_to_cpu -> exchange -> materializer (the
+ * union_all fan-in of union_all.cu:36-57, src/main9.cu:104-148, on one
+ * GPU).  nstreams in [1, 16].  _sum: *sum = exact int64 SUM(val WHERE pred
+ * <= thres); _select: dst (if not NULL, capacity dst_cap >= n) receives the
+ * selected rows of val in unspecified order, *count = selected rows. */
+int mgv_ops_sum_select_streams(const int32_t *pred, const int32_t *val, uint64_t n, int32_t thres,
+                               uint32_t vector_rows, int device, uint32_t grid, uint32_t block, uint32_t nstreams,
+                               int64_t *sum);
+int mgv_ops_select_streams(const int32_t *pred, const int32_t *val, uint64_t n, int32_t thres, uint32_t vector_rows,
+                           int device, uint32_t grid, uint32_t block, uint32_t nstreams, int32_t *dst,
+                           uint64_t dst_cap, uint64_t *count);
+
 /* Build: generator(R) -> exchange -> [select(pred_r <= bound) ->]
  * hashjoin_builder; probe: generator(L) -> exchange -> [select(pred_l <=
  * bound) ->] hashjoin -> gpu_to_cpu -> exchange -> materializer

@@ -31,8 +31,8 @@
 //  - a split stages its parents' rows and hands them on as vectors when its
 //    staging fills and at close (the reference forwarded every 128 rows),
 //    partitioning on the first column it receives;
-//  - a union_all must be the first operator of its chain (the parent a
-//    split or an exchange hands vectors to);
+//  - the chains of a union_all's children share its accumulators (one result
+//    slot, one row buffer), and each child opens / closes it once;
 //  - mem_move and union_all_cpu (multi-GPU routing) are not provided.
 #pragma once
 
@@ -215,6 +215,9 @@ private:
     uint64_t out_cap = 0, out_upper = 0;
     mgv_stream_t last_stream = nullptr;
     bool opened = false;
+    // a union_all further up the chain: this chain shares its result slot
+    // and materialised rows, and opens / closes it as one of its children
+    d_operator_t *through = nullptr;
     // split: per-destination staging columns and their append cursors
     split_t *splitter = nullptr;
     int32_t *stage = nullptr;            // device, [dest][col][stage_cap]

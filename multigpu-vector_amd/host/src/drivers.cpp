@@ -62,11 +62,13 @@ struct Sink {
     materializer *m() { return mat->get<materializer *>(); }
 };
 
-// generator -> exchange -> head; run with open / close.
-void run_generator(d_operator_t *head, const launch_conf &conf, std::vector<int32_t *> cols, uint64_t n,
-                   uint32_t vector_rows, int device) {
-    h_operator_t *xgen = h_operator_t::create<exchange>(std::vector<p_operator_t>{p_operator_t(head)},
-                                                        std::vector<launch_conf>{conf});
+// generator -> exchange -> heads (one consumer, with its own streams, per
+// head); run with open / close.
+void run_generator(const std::vector<d_operator_t *> &heads, const launch_conf &conf, std::vector<int32_t *> cols,
+                   uint64_t n, uint32_t vector_rows, int device) {
+    std::vector<p_operator_t> parents;
+    for (d_operator_t *h : heads) parents.emplace_back(h);
+    h_operator_t *xgen = h_operator_t::create<exchange>(parents, std::vector<launch_conf>(heads.size(), conf));
     try {
         generator gen(xgen, std::move(cols), n, vector_rows, device);
         gen.open();
@@ -76,6 +78,42 @@ void run_generator(d_operator_t *head, const launch_conf &conf, std::vector<int3
         throw;
     }
     h_operator_t::destroy(xgen);
+}
+
+void run_generator(d_operator_t *head, const launch_conf &conf, std::vector<int32_t *> cols, uint64_t n,
+                   uint32_t vector_rows, int device) {
+    run_generator(std::vector<d_operator_t *>{head}, conf, std::move(cols), n, vector_rows, device);
+}
+
+// generator -> exchange -> {select(thres)} x nstreams -> union_all(nstreams)
+// -> [aggregation ->] gpu_to_cpu -> exchange -> materializer.
+void run_fan_in(const int32_t *pred, const int32_t *val, uint64_t n, int32_t thres, uint32_t vector_rows,
+                int device, const launch_conf &conf, uint32_t nstreams, bool aggregate, Sink &sink,
+                mgv_result *result) {
+    if (nstreams == 0 || nstreams > 16) throw mgv::error(MGV_E_INVALID, "nstreams must be in [1, 16]");
+    d_operator_t *g2c = d_operator_t::create<gpu_to_cpu_t>(conf, sink.xroot, conf.device);
+    d_operator_t *aggr =
+        aggregate ? d_operator_t::create<aggregation_t>(conf, g2c, 8000, conf.get_blocks_per_grid(), 0) : nullptr;
+    d_operator_t *u = d_operator_t::create<union_all_t>(conf, aggregate ? aggr : g2c, (int)nstreams, conf,
+                                                        conf.device);
+    std::vector<d_operator_t *> sels;
+    auto cleanup = [&] {
+        for (d_operator_t *x : sels) d_operator_t::destroy(x);
+        d_operator_t::destroy(u);
+        d_operator_t::destroy(aggr);
+        d_operator_t::destroy(g2c);
+    };
+    try {
+        for (uint32_t i = 0; i < nstreams; ++i)
+            sels.push_back(d_operator_t::create<select_t>(conf, u, less_eq_than<int32_t>(thres),
+                                                          conf.get_blocks_per_grid(), 0));
+        run_generator(sels, conf, {(int32_t *)pred, (int32_t *)val}, n, vector_rows, device);
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    *result = g2c->get<gpu_to_cpu_t *>()->result();
+    cleanup();
 }
 
 }  // namespace
@@ -227,6 +265,37 @@ int mgv_ops_split_sums(const int32_t *pred, const int32_t *val, uint64_t n, int3
             counts[p] = n ? r.count : 0;
         }
         cleanup();
+    });
+}
+
+int mgv_ops_sum_select_streams(const int32_t *pred, const int32_t *val, uint64_t n, int32_t thres,
+                               uint32_t vector_rows, int device, uint32_t grid, uint32_t block, uint32_t nstreams,
+                               int64_t *sum) {
+    return guarded("mgv_ops_sum_select_streams", [&] {
+        require_manager();
+        check_vector_rows(vector_rows);
+        if (!pred || !val || !sum) throw mgv::error(MGV_E_INVALID, "null argument");
+        const launch_conf conf = make_conf(grid, block);
+        Sink sink(nullptr);
+        mgv_result r{};
+        run_fan_in(pred, val, n, thres, vector_rows, device, conf, nstreams, true, sink, &r);
+        *sum = n ? sink.m()->result64 : 0;
+    });
+}
+
+int mgv_ops_select_streams(const int32_t *pred, const int32_t *val, uint64_t n, int32_t thres, uint32_t vector_rows,
+                           int device, uint32_t grid, uint32_t block, uint32_t nstreams, int32_t *dst,
+                           uint64_t dst_cap, uint64_t *count) {
+    return guarded("mgv_ops_select_streams", [&] {
+        require_manager();
+        check_vector_rows(vector_rows);
+        if (!pred || !val || !count) throw mgv::error(MGV_E_INVALID, "null argument");
+        if (dst && dst_cap < n) throw mgv::error(MGV_E_CAPACITY, "dst must hold n rows");
+        const launch_conf conf = make_conf(grid, block);
+        Sink sink(dst);
+        mgv_result r{};
+        run_fan_in(pred, val, n, thres, vector_rows, device, conf, nstreams, false, sink, &r);
+        *count = n ? r.count : 0;
     });
 }
 

@@ -75,22 +75,29 @@ d_operator_t *parent_of(const d_operator_t::op_t &op) {
     if (auto p = std::get_if<select_t *>(&op)) return (*p)->parent;
     if (auto p = std::get_if<aggregation_t *>(&op)) return (*p)->parent;
     if (auto p = std::get_if<hashjoin_t *>(&op)) return (*p)->parent;
+    if (auto p = std::get_if<union_all_t *>(&op)) return (*p)->parent;
     return nullptr;
 }
 
 }  // namespace
 
 void d_operator_t::resolve() {
-    // the chain from this operator up to its sink
-    // a union_all passes its children's vectors to its parent's chain
+    // the chain from this operator up to its sink; a union_all passes its
+    // children's vectors on to its parent (union_all.cu:45-48), so it is not
+    // a stage of the chain: a union_all at the head owns the chain's
+    // accumulators, one further up shares its own with this chain
     std::vector<d_operator_t *> chain;
     d_operator_t *start = this;
-    if (auto u = std::get_if<union_all_t *>(&op)) start = (*u)->parent;
+    if (std::holds_alternative<union_all_t *>(op)) start = parent_of(op);
+    size_t hops = 0;
     for (d_operator_t *c = start; c; c = parent_of(c->op)) {
-        if (std::holds_alternative<union_all_t *>(c->op))
-            throw mgv::error(MGV_E_INVALID, "a union_all must be the first operator of its chain");
+        if (++hops > 9) throw mgv::error(MGV_E_INVALID, "operator chain too long or cyclic");
+        if (std::holds_alternative<union_all_t *>(c->op)) {
+            if (through || c == this) throw mgv::error(MGV_E_INVALID, "at most one union_all per operator chain");
+            through = c;
+            continue;
+        }
         chain.push_back(c);
-        if (chain.size() > 8) throw mgv::error(MGV_E_INVALID, "operator chain too long or cyclic");
     }
     auto is = [&](size_t i, auto tag) {
         using P = decltype(tag);
@@ -106,6 +113,7 @@ void d_operator_t::resolve() {
     filtered = sel;
     if (is(i, (split_t *)nullptr)) {
         // [select ->] split: the split's parents are the heads of their own chains
+        if (through) throw mgv::error(MGV_E_INVALID, "a split after a union_all must be its first operator");
         kind = SPLIT;
         splitter = std::get<split_t *>(chain[i]->op);
         return;
@@ -171,6 +179,13 @@ void d_operator_t::close() {
 void d_operator_t::do_open() {
     if (opened) return;
     if (kind == NONE) resolve();
+    if (through) {
+        // one child of the union_all: the first child's open opens the
+        // shared chain (union_all.cu:19-30)
+        through->open();
+        opened = true;
+        return;
+    }
     if (kind == SPLIT) {
         if (!cursors) {
             void *p = nullptr;
@@ -242,9 +257,13 @@ void d_operator_t::do_open() {
 }
 
 void d_operator_t::consume(buffer_t *v, mgv_stream_t s) {
-    std::unique_lock<std::mutex> fan_in;
-    if (auto u = std::get_if<union_all_t *>(&op)) fan_in = std::unique_lock<std::mutex>((*u)->host_lock);
     if (!opened) do_open();
+    // the accumulators: this chain's own, or those of the union_all it
+    // feeds; the children of a union_all consume one at a time
+    d_operator_t *const acc = through ? through : this;
+    std::unique_lock<std::mutex> fan_in;
+    if (auto u = std::get_if<union_all_t *>(&acc->op)) fan_in = std::unique_lock<std::mutex>((*u)->host_lock);
+    mgv_result *const res = acc->res;
     last_stream = s;
     const uint64_t n = v->count();
     if (n == 0) return;
@@ -256,22 +275,22 @@ void d_operator_t::consume(buffer_t *v, mgv_stream_t s) {
             gpu(mgv_select_sum(c[0], nc > 1 ? c[1] : c[0], n, bound, res, L, s));
             break;
         case SELECT_MAT: {
-            if (out_upper + n > out_cap) {   // grow the output (rare: doubling)
+            if (acc->out_upper + n > acc->out_cap) {   // grow the output (rare: doubling)
                 // every stream: the children of a union_all write from several
                 gpu(mgv_device_synchronize());
-                uint64_t cap = out_cap ? out_cap : (1u << 20);
-                while (cap < out_upper + n) cap *= 2;
+                uint64_t cap = acc->out_cap ? acc->out_cap : (1u << 20);
+                while (cap < acc->out_upper + n) cap *= 2;
                 void *p = nullptr;
                 gpu(mgv_malloc(&p, cap * sizeof(int32_t)));
-                if (out) {
-                    gpu(mgv_memcpy(p, out, out_cap * sizeof(int32_t)));
-                    gpu(mgv_free(out));
+                if (acc->out) {
+                    gpu(mgv_memcpy(p, acc->out, acc->out_cap * sizeof(int32_t)));
+                    gpu(mgv_free(acc->out));
                 }
-                out = (int32_t *)p;
-                out_cap = cap;
+                acc->out = (int32_t *)p;
+                acc->out_cap = cap;
             }
-            out_upper += n;
-            gpu(mgv_select_materialize(c[0], nc > 1 ? c[1] : c[0], n, bound, out, out_cap, res, L, s));
+            acc->out_upper += n;
+            gpu(mgv_select_materialize(c[0], nc > 1 ? c[1] : c[0], n, bound, acc->out, acc->out_cap, res, L, s));
             break;
         }
         case PROBE:
@@ -355,6 +374,13 @@ void d_operator_t::flush_split(mgv_stream_t s) {
 
 void d_operator_t::do_close() {
     if (!opened) return;
+    if (through) {
+        // the last child's close closes the shared chain (union_all.cu:63-74)
+        gpu(mgv_stream_synchronize(last_stream));
+        opened = false;
+        through->close();
+        return;
+    }
     if (kind == SPLIT) {
         flush_split(last_stream);   // at_close (split.cu:131-150)
         gpu(mgv_stream_synchronize(last_stream));

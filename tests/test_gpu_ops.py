@@ -134,6 +134,50 @@ def test_split_sums_plan(ops, orc, n, vector_rows):
     del keep
 
 
+@pytest.mark.parametrize("nstreams,n,vector_rows", [(1, 1 << 20, 8192), (2, (1 << 21) + 77, 8192),
+                                                     (4, 3000000, 65536), (3, 0, 4096)])
+def test_fan_in_sum_plan(ops, orc, nstreams, n, vector_rows):
+    """Several exchange consumers (own streams), one select each, into one
+    union_all -> aggregation -> gpu_to_cpu: the union_all's children share
+    its result, so the sum is the whole input's."""
+    a = orc.gen_bits(n, orc.seed(1, 8), 3, 20)
+    b = orc.gen_bits(n, orc.seed(1, 9), 3, 31)
+    bound = (1 << 30) - 1
+    (pb, pa), keep = cols(0, b, a)
+    s = C.c_int64()
+    mgv.ops_call("mgv_ops_sum_select_streams", pb, pa, n, bound, vector_rows, 0, 64, 256, nstreams, C.byref(s))
+    assert s.value == (orc.select_sum(b, a, bound)[1] if n else 0)
+    del keep
+
+
+@pytest.mark.parametrize("nstreams,where", [(2, 0), (3, -1)])
+def test_fan_in_select_plan(ops, orc, nstreams, where):
+    """Selected rows of several consumers gathered through one union_all ->
+    gpu_to_cpu: the multiset of the select's rows."""
+    n = 1500001
+    a = orc.gen_bits(n, orc.seed(1, 10), 3, 31)
+    b = orc.gen_bits(n, orc.seed(1, 11), 3, 31)
+    bound = (1 << 29) - 1
+    (pb, pa), keep = cols(where, b, a)
+    dst = np.full(n, -7, np.int32)
+    cnt = C.c_uint64()
+    mgv.ops_call("mgv_ops_select_streams", pb, pa, n, bound, 16384, where, 64, 256, nstreams, host_ptr(dst), n,
+                 C.byref(cnt))
+    want = a[b <= bound]
+    assert cnt.value == len(want)
+    assert np.array_equal(np.sort(dst[:cnt.value]), np.sort(want))
+    assert np.all(dst[cnt.value:] == -7)
+    del keep
+
+
+def test_fan_in_rejects_bad_stream_count(ops):
+    x = dev(np.arange(1024, dtype=np.int32))
+    s = C.c_int64()
+    for k in (0, 17):
+        with pytest.raises(mgv.MgvError):
+            mgv.ops_call("mgv_ops_sum_select_streams", x.vp, x.vp, 1024, 5, 256, 0, 8, 256, k, C.byref(s))
+
+
 def test_select_plan_rejects_small_dst(ops):
     x = dev(np.arange(1024, dtype=np.int32))
     dst = np.zeros(10, np.int32)
