@@ -82,6 +82,20 @@ int mgv_ops_select_streams(const int32_t *pred, const int32_t *val, uint64_t n, 
                            int device, uint32_t grid, uint32_t block, uint32_t nstreams, int32_t *dst,
                            uint64_t dst_cap, uint64_t *count);
 
+/* create_select_pipeline (aggregate = 0) / create_sum_select_pipeline
+ * (aggregate = 1) with npipelines launch confs on one GPU
+ * (src/main9.cu:150-174, 202-226): generator -> exchange with one consumer
+ * per conf -> unstable_select(less_eq_than(thres)) -> [aggregation ->]
+ * gpu_to_cpu -> mem_move -> union_all_cpu(npipelines) -> exchange ->
+ * materializer(dst).  npipelines in [1, 16].  aggregate = 0: dst (if not
+ * NULL, dst_cap >= n) receives the selected rows of val, *value = their
+ * number; aggregate = 1: *value = exact int64 SUM(val WHERE pred <= thres),
+ * dst (if not NULL, dst_cap >= npipelines) the low 32 bits of each
+ * pipeline's partial sum, as the reference's materializer stores them. */
+int mgv_ops_select_pipelines(const int32_t *pred, const int32_t *val, uint64_t n, int32_t thres,
+                             uint32_t vector_rows, int device, uint32_t grid, uint32_t block, uint32_t npipelines,
+                             int aggregate, int32_t *dst, uint64_t dst_cap, int64_t *value);
+
 /* Build: generator(R) -> exchange -> [select(pred_r <= bound) ->]
  * hashjoin_builder; probe: generator(L) -> exchange -> [select(pred_l <=
  * bound) ->] hashjoin -> gpu_to_cpu -> exchange -> materializer

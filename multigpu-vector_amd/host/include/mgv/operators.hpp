@@ -10,6 +10,7 @@
 //     split<WARPSIZE, int32_t> / union_all<WARPSIZE, int32_t>     split.cuh:10-35, union_all.cuh:11-37
 //   host side:
 //     generator, exchange (+ consumer), materializer              generators.cuh, exchange.cuh, materializer.cuh
+//     union_all_cpu, mem_move                                     union_all_cpu.cuh, mem_move.cuh
 //
 // d_operator_t::create<Op>(conf, args...) builds a host-side operator
 // description; open() resolves the chain from an operator up to its
@@ -33,7 +34,8 @@
 //    partitioning on the first column it receives;
 //  - the chains of a union_all's children share its accumulators (one result
 //    slot, one row buffer), and each child opens / closes it once;
-//  - mem_move and union_all_cpu (multi-GPU routing) are not provided.
+//  - mem_move copies the first column of a device vector to a pinned host
+//    buffer (the vectors a gpu_to_cpu hands on are host vectors already).
 #pragma once
 
 #include <atomic>
@@ -237,10 +239,12 @@ private:
 class generator;
 class materializer;
 class exchange;
+class union_all_cpu;
+class mem_move;
 
 class h_operator_t {
 public:
-    typedef std::variant<generator *, materializer *, exchange *> op_t;
+    typedef std::variant<generator *, materializer *, exchange *, union_all_cpu *, mem_move *> op_t;
     op_t op;
 
     template <typename Op, typename... Args>
@@ -294,6 +298,37 @@ public:
     void open() {}
     void consume(buffer_t *data);
     void close() {}
+};
+
+// Host fan-in of `children` producers into one parent: the first child's
+// open opens the parent, the last child's close closes it; vectors go
+// straight through (union_all_cpu.cu:3-19).  The parent must accept
+// consumes from several threads (materializer and exchange do).
+class union_all_cpu {
+public:
+    h_operator_t *parent;
+    const int children;
+    int children_closed = 0;
+    int children_opened = 0;
+    std::mutex m;
+
+    union_all_cpu(h_operator_t *parent, int children);
+    void open();
+    void consume(buffer_t *data);
+    void close();
+};
+
+// Hands its parent host vectors: a device vector's rows (first column) are
+// copied into a pinned host buffer of the buffer manager, and the device
+// buffer goes back to the manager when it owns it (mem_move.cu:10-50).
+class mem_move {
+public:
+    h_operator_t *parent;
+
+    explicit mem_move(h_operator_t *parent);
+    void open();
+    void consume(buffer_t *data);
+    void close();
 };
 
 // Per-GPU consumer of an exchange: routes vectors to a device operator on

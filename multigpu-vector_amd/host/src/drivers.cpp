@@ -299,6 +299,57 @@ int mgv_ops_select_streams(const int32_t *pred, const int32_t *val, uint64_t n, 
     });
 }
 
+int mgv_ops_select_pipelines(const int32_t *pred, const int32_t *val, uint64_t n, int32_t thres,
+                             uint32_t vector_rows, int device, uint32_t grid, uint32_t block, uint32_t npipelines,
+                             int aggregate, int32_t *dst, uint64_t dst_cap, int64_t *value) {
+    return guarded("mgv_ops_select_pipelines", [&] {
+        require_manager();
+        check_vector_rows(vector_rows);
+        if (!pred || !val || !value) throw mgv::error(MGV_E_INVALID, "null argument");
+        if (npipelines == 0 || npipelines > 16) throw mgv::error(MGV_E_INVALID, "npipelines must be in [1, 16]");
+        if (dst && !aggregate && dst_cap < n) throw mgv::error(MGV_E_CAPACITY, "dst must hold n rows");
+        if (dst && aggregate && dst_cap < npipelines)
+            throw mgv::error(MGV_E_CAPACITY, "dst must hold one partial sum per pipeline");
+        const launch_conf conf = make_conf(grid, block);
+        Sink sink(dst);
+        // create_select_pipeline / create_sum_select_pipeline with
+        // npipelines launch confs (main9.cu:150-174, 202-226): one chain per
+        // conf, their host outputs joined by union_all_cpu
+        h_operator_t *xroot_pre =
+            npipelines > 1 ? h_operator_t::create<union_all_cpu>(sink.xroot, (int)npipelines) : sink.xroot;
+        std::vector<h_operator_t *> mmvs;
+        std::vector<d_operator_t *> ops, heads;
+        auto cleanup = [&] {
+            for (auto it = ops.rbegin(); it != ops.rend(); ++it) d_operator_t::destroy(*it);
+            for (h_operator_t *m : mmvs) h_operator_t::destroy(m);
+            if (xroot_pre != sink.xroot) h_operator_t::destroy(xroot_pre);
+        };
+        try {
+            for (uint32_t i = 0; i < npipelines; ++i) {
+                h_operator_t *mmv = h_operator_t::create<mem_move>(xroot_pre);
+                mmvs.push_back(mmv);
+                d_operator_t *g2c = d_operator_t::create<gpu_to_cpu_t>(conf, mmv, conf.device);
+                ops.push_back(g2c);
+                d_operator_t *up = g2c;
+                if (aggregate) {
+                    up = d_operator_t::create<aggregation_t>(conf, g2c, 8000, conf.get_blocks_per_grid(), 0);
+                    ops.push_back(up);
+                }
+                d_operator_t *sel = d_operator_t::create<select_t>(conf, up, less_eq_than<int32_t>(thres),
+                                                                   conf.get_blocks_per_grid(), 0);
+                ops.push_back(sel);
+                heads.push_back(sel);
+            }
+            run_generator(heads, conf, {(int32_t *)pred, (int32_t *)val}, n, vector_rows, device);
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        *value = aggregate ? (n ? sink.m()->result64 : 0) : (int64_t)sink.m()->size;
+        cleanup();
+    });
+}
+
 int mgv_ops_count_join(const int32_t *R, const int32_t *pred_r, uint64_t rn, const int32_t *L, const int32_t *pred_l,
                        uint64_t ln, int32_t bound, uint32_t vector_rows, int device, uint32_t grid, uint32_t block,
                        int32_t *dst, uint64_t *count) {

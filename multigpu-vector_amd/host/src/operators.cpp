@@ -496,6 +496,49 @@ void materializer::consume(buffer_t *data) {
     if (buffer_manager<int32_t>::owns(data)) buffer_manager<int32_t>::release_buffer(data);
 }
 
+// ------------------------------------------------------------------ union_all_cpu / mem_move
+
+union_all_cpu::union_all_cpu(h_operator_t *parent_, int children_) : parent(parent_), children(children_) {
+    if (!parent) throw mgv::error(MGV_E_INVALID, "union_all_cpu: null parent");
+    if (children < 1) throw mgv::error(MGV_E_INVALID, "union_all_cpu: at least one child");
+}
+
+void union_all_cpu::open() {
+    std::lock_guard<std::mutex> lk(m);
+    if (++children_opened == 1) parent->open();
+}
+
+void union_all_cpu::consume(buffer_t *data) { parent->consume(data); }
+
+void union_all_cpu::close() {
+    std::lock_guard<std::mutex> lk(m);
+    if (++children_closed == children) {
+        parent->close();
+        children_closed = 0;   // ready for the next open / close round
+        children_opened = 0;
+    }
+}
+
+mem_move::mem_move(h_operator_t *parent_) : parent(parent_) {
+    if (!parent) throw mgv::error(MGV_E_INVALID, "mem_move: null parent");
+}
+
+void mem_move::open() { parent->open(); }
+
+void mem_move::consume(buffer_t *data) {
+    if (data->device >= 0 && !data->scalar) {
+        buffer_t *host = buffer_manager<int32_t>::get_buffer();
+        const uint32_t n = data->count();
+        if (n) gpu(mgv_memcpy(host->data, data->data, (size_t)n * sizeof(int32_t)));
+        host->cnt = n;
+        if (buffer_manager<int32_t>::owns(data)) buffer_manager<int32_t>::release_buffer(data);
+        data = host;
+    }
+    parent->consume(data);
+}
+
+void mem_move::close() { parent->close(); }
+
 // ------------------------------------------------------------------ consumer
 
 consumer::consumer(p_operator_t parent_, const launch_conf &conf_) : parent(parent_), conf(conf_) {}

@@ -249,6 +249,7 @@ OPS_SIGNATURES = {
     "mgv_ops_split_sums": [VP, VP, U64, I32, U32, C.c_int, U32, U32, VP, VP],
     "mgv_ops_sum_select_streams": [VP, VP, U64, I32, U32, C.c_int, U32, U32, U32, C.POINTER(I64)],
     "mgv_ops_select_streams": [VP, VP, U64, I32, U32, C.c_int, U32, U32, U32, VP, U64, C.POINTER(U64)],
+    "mgv_ops_select_pipelines": [VP, VP, U64, I32, U32, C.c_int, U32, U32, U32, C.c_int, VP, U64, C.POINTER(I64)],
     "mgv_ops_count_join": [VP, VP, U64, VP, VP, U64, I32, U32, C.c_int, U32, U32, VP, C.POINTER(U64)],
     "mgv_ops_run_plain": [VP, U32, U32, Launch, VP],
     "mgv_serve_run": [VP, U32, VP, VP, VP],

@@ -178,6 +178,33 @@ def test_fan_in_rejects_bad_stream_count(ops):
             mgv.ops_call("mgv_ops_sum_select_streams", x.vp, x.vp, 1024, 5, 256, 0, 8, 256, k, C.byref(s))
 
 
+@pytest.mark.parametrize("npipelines,aggregate,where", [(1, 1, 0), (2, 1, 0), (3, 1, -1), (2, 0, 0), (3, 0, -1)])
+def test_select_pipelines_with_union_all_cpu(ops, orc, npipelines, aggregate, where):
+    """One chain per launch conf joined by mem_move -> union_all_cpu, as the
+    reference's plans do for several confs: the materializer sees every
+    pipeline's rows (or partial sums) exactly once."""
+    n = 1000003
+    a = orc.gen_bits(n, orc.seed(1, 12), 4, 31) - (1 << 30)
+    b = orc.gen_bits(n, orc.seed(1, 13), 4, 31)
+    bound = (1 << 29) - 1
+    (pb, pa), keep = cols(where, b, a)
+    cap = n if not aggregate else npipelines
+    dst = np.full(cap, -7, np.int32)
+    v = C.c_int64()
+    mgv.ops_call("mgv_ops_select_pipelines", pb, pa, n, bound, 16384, where, 64, 256, npipelines, aggregate,
+                 host_ptr(dst), cap, C.byref(v))
+    want = a[b <= bound]
+    if aggregate:
+        assert v.value == int(want.astype(np.int64).sum())
+        # one low-32-bit partial per pipeline; their int32 sum wraps like the exact sum's low bits
+        assert (int(dst.astype(np.int64).sum()) - v.value) % (1 << 32) == 0
+    else:
+        assert v.value == len(want)
+        assert np.array_equal(np.sort(dst[:v.value]), np.sort(want))
+        assert np.all(dst[v.value:] == -7)
+    del keep
+
+
 def test_select_plan_rejects_small_dst(ops):
     x = dev(np.arange(1024, dtype=np.int32))
     dst = np.zeros(10, np.int32)
