@@ -437,7 +437,8 @@ int mgv_groupby_partitioned(const int32_t *keys, const int32_t *vals, uint64_t n
  *             values; column 0 decides the destination)
  *   out       host array of nparents * ncols device columns; parent p's
  *             column c is out[p * ncols + c], `cap` rows each
- *   counts    device array of nparents append cursors (rows so far): a row
+ *   counts    device array of nparents append cursors (rows so far, 8-byte
+ *             aligned): a row
  *             for parent p goes to position counts[p]++ of its columns;
  *             rows at positions >= cap are not written but still counted,
  *             so counts[p] > cap after the call reports an overflow

@@ -141,6 +141,7 @@ int mgv_split(const int32_t *pred, int32_t bound, const int32_t *const *cols, ui
               uint32_t nparents, int32_t *const *out, uint64_t cap, uint64_t *counts, mgv_launch L, mgv_stream_t st) {
     if (!current_state()) return fail(MGV_E_NOT_INIT, "mgv_init was not called");
     if (!cols || !out || !counts) return fail(MGV_E_INVALID, "mgv_split: null argument");
+    if ((uintptr_t)counts & 7u) return fail(MGV_E_ALIGN, "mgv_split: counts must be 8-byte aligned");
     if (ncols == 0 || ncols > MGV_SPLIT_MAX_COLS)
         return fail(MGV_E_INVALID, "mgv_split: ncols must be in [1, %d]", MGV_SPLIT_MAX_COLS);
     if (nparents == 0 || nparents > MGV_SPLIT_MAX_PARENTS)

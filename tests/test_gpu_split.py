@@ -161,5 +161,8 @@ def test_split_rejects_bad_arguments(gpu):
                                    (1, 2, cp, (VP * 2)(outs[0].ptr, None))):
         with pytest.raises(mgv.MgvError):
             mgv.call("mgv_split", None, 0, cols, ncols, 16, npar, out, 16, cnt.vp, L, None)
+    with pytest.raises(mgv.MgvError) as e:   # cursors not 8-byte aligned
+        mgv.call("mgv_split", None, 0, cp, 1, 16, 2, op, 16, cnt.at(4), L, None)
+    assert e.value.code == mgv.E_ALIGN
     mgv.sync()
     assert cnt.download(np.uint64, 9).tolist() == [0] * 9
