@@ -43,6 +43,11 @@ struct PartIn {
     uint32_t rlimit;      // ... for keys in [0, rlimit); 0 = hash partitioning
     mgv_result *res;      // rows dropped for an out-of-range key -> res->errors
     const unsigned long long *nlimit;   // optional device row count: rows >= *nlimit are ignored
+    // Hash partitioning only: 0 = keys in, keys out; 1 = keys in, fmix32(key)
+    // out (partition by the hash's top bits as with 0); 2 = hashes in and out
+    // (partition by the top bits of the value itself).  fmix32 is a
+    // bijection, so hashes compare equal exactly when their keys do.
+    uint32_t key_hash;
 };
 
 // Plans the workspace of a partitioning of n rows into F partitions with

@@ -32,10 +32,16 @@ __device__ __forceinline__ uint32_t part_of(int32_t key, uint32_t F) {
     return (uint32_t)(((uint64_t)fmix32((uint32_t)key) * F) >> 32);
 }
 
+// Partition of an already hashed key: the top bits of the hash.
+__device__ __forceinline__ uint32_t part_of_hash(int32_t h, uint32_t F) {
+    return (uint32_t)(((uint64_t)(uint32_t)h * F) >> 32);
+}
+
 // Partition of a key: hash partitioning (joins) or key ranges (group-by:
 // partition = key >> rshift, keys outside [0, rlimit) are dropped).
 __device__ __forceinline__ uint32_t part_key(const PartIn &in, int32_t key, uint32_t F) {
-    return in.rlimit ? ((uint32_t)key >> in.rshift) : part_of(key, F);
+    if (in.rlimit) return (uint32_t)key >> in.rshift;
+    return in.key_hash == 2 ? part_of_hash(key, F) : part_of(key, F);
 }
 
 // Bit j set when row j of a quad has a key the partitioning accepts.
@@ -394,7 +400,8 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(PartLayout L, PartIn in)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if ((cm[q] >> j) & 1u) {
-                    sk[pos[q][j]] = q_get(ck[q], j);
+                    const int32_t key = q_get(ck[q], j);
+                    sk[pos[q][j]] = in.key_hash == 1 ? (int32_t)fmix32((uint32_t)key) : key;
                     sv[pos[q][j]] = q_get(cv[q], j);
                     if (ROWID)
                         sr[pos[q][j]] = (int32_t)(in.row_base +
@@ -424,7 +431,8 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(PartLayout L, PartIn in)
             unsigned long long dst[G];
 #pragma unroll
             for (int r = 0; r < G; ++r) {
-                const uint32_t p = total ? part_key(in, k[r], F) : 0u;
+                // the staged value is the hash when key_hash is set
+                const uint32_t p = total ? (in.key_hash ? part_of_hash(k[r], F) : part_key(in, k[r], F)) : 0u;
                 dst[r] = total ? base[p] + ii[r] : L.cap - 1;
             }
 #pragma unroll

@@ -46,17 +46,27 @@ struct LdsProbeArgs {
     mgv_result *res;
 };
 
+// The probe rows reach the probe hashed: the first partitioning writes
+// fmix32(key) in place of the key (the hash is computed there anyway), and
+// fmix32 is a bijection, so a row hits a slot exactly when its hash equals
+// the hash of the slot's key.  Slice keys are hashed once when a slice is
+// loaded, and a row's bucket is one multiply of its hash.
+__device__ __forceinline__ uint32_t bucket_of_hash(uint32_t h, uint32_t nb) {
+    return (uint32_t)(((uint64_t)h * nb) >> 32);
+}
+
 // Scans one global-memory bucket (64 bytes: 8 {key, payload} slots, empty =
-// (-1, -1)) for key: number of hitting slots and the sum of their payloads.
-// The empty pattern is excluded from the compare, so key -1 is handled.
-__device__ __forceinline__ void bucket_scan(const v4u (&q)[4], uint32_t key, uint32_t &hits, int64_t &psum) {
+// (-1, -1)) for the key whose hash is h: number of hitting slots and the sum
+// of their payloads.  The empty pattern is excluded from the compare, so key
+// -1 is handled.
+__device__ __forceinline__ void bucket_scan(const v4u (&q)[4], uint32_t h_key, uint32_t &hits, int64_t &psum) {
     uint32_t h = 0;
     int64_t t = 0;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
         const uint32_t sk = (s & 1) ? q[s >> 1].z : q[s >> 1].x;
         const uint32_t sp = (s & 1) ? q[s >> 1].w : q[s >> 1].y;
-        const bool hit = sk == key && !(sk == ~0u && sp == ~0u);
+        const bool hit = fmix32(sk) == h_key && !(sk == ~0u && sp == ~0u);
         h += hit ? 1u : 0u;
         t += hit ? (int64_t)(int32_t)sp : 0;
     }
@@ -66,8 +76,8 @@ __device__ __forceinline__ void bucket_scan(const v4u (&q)[4], uint32_t key, uin
 
 __device__ __forceinline__ bool bucket_full(const v4u (&q)[4]) { return !(q[3].z == ~0u && q[3].w == ~0u); }
 
-// A table slice in LDS is stored keys first: per bucket its 8 keys (two
-// 16-byte quads), its 8 payloads, and its fill count c.  The build fills a
+// A table slice in LDS is stored keys first: per bucket its 8 keys (hashed,
+// in two 16-byte quads), its 8 payloads, and its fill count c.  The build fills a
 // bucket's slots strictly in order (a slot is claimed only after every lower
 // slot was seen occupied, and slots are never freed), so the occupied slots
 // are exactly [0, c): slot s hits key iff s < c and its key equals key (an
@@ -180,7 +190,7 @@ __device__ __forceinline__ uint32_t queue_round(WaveQueue &Q, uint32_t head, uin
     // a chain ends at a bucket with an empty slot, or after visiting all nb
     // buckets once (when its next bucket is its home bucket again)
     const uint32_t nxt = (bkt + 1 == nb) ? 0 : bkt + 1;
-    const bool more = mine && full && nxt != bucket_of((int32_t)key, nb);
+    const bool more = mine && full && nxt != bucket_of_hash(key, nb);
     head += take;
     tail = queue_push(Q, tail, more, key, val, nxt);
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)head);
@@ -199,7 +209,7 @@ __device__ __forceinline__ void lds_probe_quad(const v4i &k, const v4i &v, uint3
     bool ins[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        b[j] = bucket_of(q_get(k, j), nb);
+        b[j] = bucket_of_hash((uint32_t)q_get(k, j), nb);
         const uint32_t idx = b[j] - lo;
         ins[j] = ((m >> j) & 1u) && idx < nload;
         ii[j] = ins[j] ? idx : 0u;
@@ -264,8 +274,9 @@ __device__ __forceinline__ void load_slice(Slice &S, const uint64_t *slots, uint
             const bool empty = (s & 1) ? (q.z == ~0u && q.w == ~0u) : (q.x == ~0u && q.y == ~0u);
             c = empty ? (uint32_t)s : c;
         }
-        S.key[key_at(i, 0)] = v4u{q0.x, q0.z, q1.x, q1.z};
-        S.key[key_at(i, 1)] = v4u{q2.x, q2.z, q3.x, q3.z};
+        // hashed keys (an empty slot's value is never compared: it is >= c)
+        S.key[key_at(i, 0)] = v4u{fmix32(q0.x), fmix32(q0.z), fmix32(q1.x), fmix32(q1.z)};
+        S.key[key_at(i, 1)] = v4u{fmix32(q2.x), fmix32(q2.z), fmix32(q3.x), fmix32(q3.z)};
         v4u *pay4 = reinterpret_cast<v4u *>(S.pay);
         pay4[i * 2] = v4u{q0.y, q0.w, q1.y, q1.w};
         pay4[i * 2 + 1] = v4u{q2.y, q2.w, q3.y, q3.w};
@@ -527,10 +538,10 @@ int mgv_hashjoin_probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32
     if (rc) return rc;
     hipStream_t s = stream_of(st);
     // level 1: select + hash partitions, packed (the total is the survivor count)
-    PartIn in1 = {pred, bound, keys, vals, 0, 0u, 0u, 0u, nullptr, nullptr};
+    PartIn in1 = {pred, bound, keys, vals, 0, 0u, 0u, 0u, nullptr, nullptr, 1u};   // writes fmix32(key)
     if ((rc = partition_rows(L1, in1, s, scatter_shape(LC)))) return rc;
     // level 2: the level-1 output, ordered by the same hash bits, split finer
-    PartIn in2 = {nullptr, 0, L1.col[0], L1.col[1], 0, 0u, 0u, 0u, nullptr, L1.pstart + L1.F};
+    PartIn in2 = {nullptr, 0, L1.col[0], L1.col[1], 0, 0u, 0u, 0u, nullptr, L1.pstart + L1.F, 2u};   // hashes
     if ((rc = partition_rows(L2, in2, s, scatter_shape(LC)))) return rc;
     uint32_t *next = (uint32_t *)ws;
     if ((rc = mgv_fill_u32(next, 1, 0, (mgv_stream_t)s))) return rc;
