@@ -740,8 +740,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on exactly one GPU: there is no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
     mgv.load()
