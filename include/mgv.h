@@ -444,8 +444,8 @@ int mgv_groupby_partitioned(const int32_t *keys, const int32_t *vals, uint64_t n
  *             so counts[p] > cap after the call reports an overflow
  * The order of the rows inside a parent's output is unspecified.  The
  * launch's grid (0 = one persistent grid) is honoured, its block is not
- * (blocks of 1024 threads take tiles of 32768 rows, 8192 with two to four
- * columns). */
+ * (one column: blocks of 512 threads take tiles of 16384 rows; two to four
+ * columns: 1024 threads, 8192 rows). */
 #define MGV_SPLIT_MAX_COLS 4
 #define MGV_SPLIT_MAX_PARENTS 8
 int mgv_split(const int32_t *pred, int32_t bound, const int32_t *const *cols, uint32_t ncols, uint64_t n,

@@ -10,14 +10,15 @@
 // parents beyond the second receive nothing.
 //
 // Here a split appends each destination's rows to that parent's output
-// columns: one block of 1024 threads takes a tile of 32768 rows (8192 with
-// two to four columns), every wave ballots its rows per destination, the
-// block adds up its waves' counts in LDS and reserves one contiguous run per
-// destination with a single global atomic, and each wave writes its rows
-// into its part of the run (each ballot's rows to consecutive positions).  A vector is read once and each
-// kept row written once: 4 * ncols bytes read per row (plus 4 for a filter
-// column) and 4 * ncols written per kept row, HBM-bound.  The order of the
-// rows inside a destination is unspecified, as it is in the reference.
+// columns: a block takes a tile of rows (512 threads x 32 rows with one
+// column, 1024 x 8 with two to four), every wave ballots its rows per
+// destination, the block adds up its waves' counts in LDS and reserves one
+// contiguous run per destination with a single global atomic, and each wave
+// writes its rows into its part of the run (each ballot's rows to
+// consecutive positions).  A vector is read once and each kept row written
+// once: 4 * ncols bytes read per row (plus 4 for a filter column) and
+// 4 * ncols written per kept row, HBM-bound.  The order of the rows inside a
+// destination is unspecified, as it is in the reference.
 #include <hip/hip_runtime.h>
 
 #include "mgv_internal.hpp"
@@ -26,13 +27,13 @@
 namespace mgv {
 
 constexpr int kSplitBlock = 1024;
-constexpr uint32_t kSplitWaves = kSplitBlock / kWave;
+constexpr int kSplitBlock1 = 512;                              // one column
 constexpr int kSplitDests = 2;                                 // x & 1 has two values
 
 // Rows per lane per tile; each tile costs one cursor atomic per destination.
-// A one-column tile has 32768 rows (127 VGPRs of the 128 a 1024-thread block
-// allows); with two to four columns 16 or 32 rows per lane spill, and tiles
-// keep 8192 rows.
+// One column: 32 rows per lane in 512-thread blocks, two per CU (16384-row
+// tiles, 128 VGPRs); two to four columns: 16 or 32 rows per lane spill, so 8
+// rows per lane in 1024-thread blocks (8192-row tiles).
 template <int NC>
 constexpr int split_rows() { return NC == 1 ? 32 : 8; }
 
@@ -47,11 +48,14 @@ struct SplitArgs {
     unsigned long long *counts;
 };
 
-template <int NC>
-__global__ __launch_bounds__(kSplitBlock) void k_split(SplitArgs a) {
+// The second launch bound is HIP's minimum waves per SIMD: 4 for 512-thread
+// blocks keeps two blocks (16 waves) per CU within 128 VGPRs.
+template <int NC, int BLOCK>
+__global__ __launch_bounds__(BLOCK, 2048 / BLOCK) void k_split(SplitArgs a) {
     constexpr int R = split_rows<NC>();
-    constexpr uint64_t kTile = (uint64_t)kSplitBlock * R;
-    __shared__ uint32_t wcnt[kSplitDests][kSplitWaves];
+    constexpr uint32_t kWaves = BLOCK / kWave;
+    constexpr uint64_t kTile = (uint64_t)BLOCK * R;
+    __shared__ uint32_t wcnt[kSplitDests][kWaves];
     __shared__ unsigned long long sbase[kSplitDests];
     const uint32_t lane = lane_id();
     const uint32_t w = wave_id();
@@ -103,7 +107,7 @@ __global__ __launch_bounds__(kSplitBlock) void k_split(SplitArgs a) {
         if (threadIdx.x < kSplitDests) {
             const int d = threadIdx.x;
             uint64_t sum = 0;
-            for (uint32_t i = 0; i < kSplitWaves; ++i) sum += wcnt[d][i];
+            for (uint32_t i = 0; i < kWaves; ++i) sum += wcnt[d][i];
             sbase[d] = sum ? atomicAdd(a.counts + d, (unsigned long long)sum) : 0ull;
         }
         __syncthreads();
@@ -163,18 +167,20 @@ int mgv_split(const int32_t *pred, int32_t bound, const int32_t *const *cols, ui
             a.out[d][c] = out[d * ncols + c];
         }
     if (n == 0) return MGV_OK;
-    const uint64_t tile = (uint64_t)kSplitBlock * (ncols == 1 ? split_rows<1>() : split_rows<4>());
+    const int block = ncols == 1 ? kSplitBlock1 : kSplitBlock;
+    const uint64_t tile = (uint64_t)block * (ncols == 1 ? split_rows<1>() : split_rows<4>());
     const uint64_t ntiles = (n + tile - 1) / tile;
-    static thread_local int cached = 0;
-    if (!cached) cached = persistent_blocks(k_split<4>, kSplitBlock);
-    uint64_t grid = L.grid ? L.grid : (uint64_t)cached;
+    static thread_local int cached1 = 0, cached = 0;
+    if (!cached1) cached1 = persistent_blocks(k_split<1, kSplitBlock1>, kSplitBlock1);
+    if (!cached) cached = persistent_blocks(k_split<4, kSplitBlock>, kSplitBlock);
+    uint64_t grid = L.grid ? L.grid : (uint64_t)(ncols == 1 ? cached1 : cached);
     if (grid > ntiles) grid = ntiles;
     hipStream_t s = stream_of(st);
     switch (ncols) {
-        case 1: hipLaunchKernelGGL(k_split<1>, dim3((uint32_t)grid), dim3(kSplitBlock), 0, s, a); break;
-        case 2: hipLaunchKernelGGL(k_split<2>, dim3((uint32_t)grid), dim3(kSplitBlock), 0, s, a); break;
-        case 3: hipLaunchKernelGGL(k_split<3>, dim3((uint32_t)grid), dim3(kSplitBlock), 0, s, a); break;
-        default: hipLaunchKernelGGL(k_split<4>, dim3((uint32_t)grid), dim3(kSplitBlock), 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_split<1, kSplitBlock1>), dim3((uint32_t)grid), dim3(block), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_split<2, kSplitBlock>), dim3((uint32_t)grid), dim3(block), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_split<3, kSplitBlock>), dim3((uint32_t)grid), dim3(block), 0, s, a); break;
+        default: hipLaunchKernelGGL((k_split<4, kSplitBlock>), dim3((uint32_t)grid), dim3(block), 0, s, a); break;
     }
     return check_launch("k_split");
 }
