@@ -167,7 +167,7 @@ def leg_c2(args, st):
         candidates = [("direct", 1), ("part", 64), ("lds", 32), ("lds", 64)]
         if "mode" in t2 and (t2["mode"], int(t2["fanout"])) not in candidates:
             candidates.append((t2["mode"], int(t2["fanout"])))
-    sweep = {}
+    sweep, sweep_res = {}, {}
     for mode, f in candidates:
         fn = make_probe(mode, f)
         t = Timer(st.vp)
@@ -176,7 +176,9 @@ def leg_c2(args, st):
             t.around(fn)
         st.sync()
         sweep[(mode, f)] = min(t.durations_ms()[1:] or t.durations_ms())
-        log(f"C2: probe {mode} fan-out {f}: {sweep[(mode, f)]:.3f} ms")
+        rr = res.read()
+        sweep_res[(mode, f)] = (rr.count, rr.sum)
+        log(f"C2: probe {mode} fan-out {f}: {sweep[(mode, f)]:.3f} ms, count {rr.count} sum {rr.sum}")
     best_mode, fanout = min(sweep, key=sweep.get)
     probe_fn = make_probe(best_mode, fanout)
 
@@ -202,13 +204,18 @@ def leg_c2(args, st):
     kms = timer.durations_ms()
     r = res.read()
 
-    # parity: COUNT has a closed form (#{R.k < 2^24}), counted by the select kernel
-    chk = mgv.ResultBuffer()
-    mgv.call("mgv_select_sum", rk.vp, rv.vp, nr, ns - 1, chk.vp, mgv.launch(0, 256, 0), st.vp)
-    st.sync()
-    c = chk.read()
-    ok_count = r.count == c.count
-    log(f"C2: probe count {r.count} (closed form {c.count}) sum {r.sum}")
+    # parity (checker, on the CPU): the closed form of the C2 join, COUNT =
+    # #{R.k < 2^24} and SUM = sum of R.v + w[perm^-1(R.k)] over those rows,
+    # from the inputs the GPU used; the timed steps' result and every swept
+    # candidate's must equal it
+    cf = closed_form_c2(args, st, sk, sw, rk, rv, ns, nr)
+    log(f"C2: probe count {r.count} sum {r.sum}; closed form count {cf[0]} sum {cf[1]}")
+    parity = {
+        "closed_form_count": cf[0], "closed_form_sum": cf[1],
+        "timed_path_equals_closed_form": (r.count, r.sum) == cf,
+        "every_candidate_equals_closed_form": all(v == cf for v in sweep_res.values()),
+        "candidates": {f"{m}{f}": list(v) for (m, f), v in sweep_res.items()},
+    }
 
     ms_per_step = wall * 1e3 / args.steps
     avg_kernel_ms = float(np.mean(kms))
@@ -246,63 +253,130 @@ def leg_c2(args, st):
         "build": {"ms": build_t, "path": build_path, "ms_by_path": build_by,
                   "grows_per_s": gb(ns) / (build_t / 1e3),
                   "gbps_algorithmic": (ns * 8 + ht.nbuckets * 64) / (build_t / 1e3) / 1e9},
-        "result": {"count": r.count, "sum": r.sum, "count_closed_form_ok": ok_count},
+        "result": {"count": r.count, "sum": r.sum, **parity},
         "probe_path_sweep_ms": {f"{m}{f}": v for (m, f), v in sweep.items()},
     }
 
     if args.cpu:
-        out["cpu_baseline"], out["sample_parity"] = cpu_baseline_c2(args, st, sk, sw, rk, rv, ht, ns, nr, L)
-    for b in (sk, sw, rk, rv, ht.buf, res, chk) + ((ws,) if ws is not None else ()):
+        out["cpu_c2"] = cpu_baseline_c2(args, st, sk, sw, rk, rv, ht, ns, nr)
+        out["cpu_c2"]["equals_gpu"] = out["cpu_c2"]["result"] == [r.count, r.sum]
+    for b in (sk, sw, rk, rv, ht.buf, res) + ((ws,) if ws is not None else ()):
         b.close()
     return out
 
 
-def cpu_baseline_c2(args, st, sk, sw, rk, rv, ht, ns, nr, L):
-    """Oracle chained-table join (the reference's algorithm, main8.cu:84-103)
-    on a bounded sample: full S build, probe of the first M rows of R.  The GPU
-    probes the same M rows and must give the same COUNT and SUM."""
+def cpu_threads(args):
+    """Threads of the all-threads CPU figures: the CPUs this process may run
+    on, limited to the host share given to one GPU (OMP_NUM_THREADS, which
+    the GPU host sets to its per-GPU share)."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    n = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
+
+
+def cpu_info():
+    info = {"hardware_concurrency": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        keys = {"model name": "cpu_model", "cpu family": "cpu_family", "model": "cpu_model_id",
+                "stepping": "stepping", "cpu MHz": "cpu_mhz"}
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in keys and keys[k] not in info:
+                info[keys[k]] = v.strip()
+            if not line.strip() and "cpu_model" in info:
+                break
+    except OSError:
+        pass
+    return info
+
+
+def oracle():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py
-    orc = oracle_py.Oracle()
+    return oracle_py.Oracle()
+
+
+def closed_form_c2(args, st, sk, sw, rk, rv, ns, nr):
+    """Checker: (COUNT, SUM) of the C2 join by direct addressing on the CPU
+    (the build keys are a permutation of [0, ns))."""
     st.sync()
     hk, hw = sk.download(np.int32, ns), sw.download(np.int32, ns)
-    M = args.cpu_sample
-    hrk, hrv = rk.download(np.int32, M), rv.download(np.int32, M)
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    hrk, hrv = rk.download(np.int32, nr), rv.download(np.int32, nr)
+    got = oracle().join_dense(hk, hw, ns, hrk, hrv, threads=cpu_threads(args))
+    assert got is not None, "C2 build keys are not a permutation"
+    return got
+
+
+def cpu_baseline_c2(args, st, sk, sw, rk, rv, ht, ns, nr):
+    """CPU baseline of the C2 join: the oracle's chained hash table (the
+    reference's own CPU algorithm, main8.cu:84-103, and its device build,
+    hashjoin.cu:41-60).  All threads: a threaded build of all S rows and a
+    probe of all R rows.  One thread: the build and a probe of the first
+    cpu_sample R rows.  Both must give the GPU's results on the same rows."""
+    orc = oracle()
+    st.sync()
+    hk, hw = sk.download(np.int32, ns), sw.download(np.int32, ns)
+    hrk, hrv = rk.download(np.int32, nr), rv.download(np.int32, nr)
+    T = cpu_threads(args)
     t0 = time.perf_counter()
-    t = orc.table(hk, hw, log_buckets=25)
+    t = orc.table(hk, hw, log_buckets=25, threads=T)
     t_build = time.perf_counter() - t0
     t0 = time.perf_counter()
-    want = orc.probe(t, hrk, hrv, threads=threads)
+    want = orc.probe(t, hrk, hrv, threads=T)
     t_probe = time.perf_counter() - t0
-    t1 = time.perf_counter()
-    want1 = orc.probe(t, hrk[: M // 8], hrv[: M // 8], threads=1)
-    t_probe1 = time.perf_counter() - t1
     orc.free(t)
+    M = min(args.cpu_sample, nr)
+    t0 = time.perf_counter()
+    t1 = orc.table(hk, hw, log_buckets=25)
+    t_build1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    want1 = orc.probe(t1, hrk[:M], hrv[:M])
+    t_probe1 = time.perf_counter() - t0
+    orc.free(t1)
     res = mgv.ResultBuffer()
-    mgv.call("mgv_hashjoin_probe", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp, M, 0, 0, res.vp, L, st.vp)
+    mgv.call("mgv_hashjoin_probe", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp, M, 0, 0, res.vp,
+             mgv.launch(0, 256), st.vp)
     st.sync()
     g = res.read()
     res.close()
-    parity = {"rows": M, "gpu": [g.count, g.sum], "cpu": [want["count"], want["sum"]],
-              "equal": (g.count, g.sum) == (want["count"], want["sum"])}
-    log(f"C2 cpu: build {t_build:.2f}s probe {M} rows {t_probe:.2f}s on {threads} threads; parity {parity['equal']}")
-    cpu = {"value": M / t_probe / 1e9, "unit": "Grows/s", "cores": threads, "kind": "port",
-           "sample": f"oracle chained hash join: build all {ns} S rows ({t_build:.2f} s, 1 thread), "
-                     f"probe first {M} R rows",
-           "single_thread_value": (M // 8) / t_probe1 / 1e9, "build_s": t_build,
-           "cpu_model": cpu_model()}
-    return cpu, parity
+    log(f"C2 cpu: {T} threads build {t_build:.2f} s probe {nr} rows {t_probe:.2f} s; 1 thread build "
+        f"{t_build1:.2f} s probe {M} rows {t_probe1:.2f} s")
+    return {
+        "threads": T, "build_s": t_build, "probe_s": t_probe, "probe_rows": nr,
+        "grows_per_s": nr / t_probe / 1e9,
+        "one_thread": {"build_s": t_build1, "probe_s": t_probe1, "probe_rows": M, "grows_per_s": M / t_probe1 / 1e9},
+        "result": [want["count"], want["sum"]],
+        "one_thread_sample_equals_gpu": (g.count, g.sum) == (want1["count"], want1["sum"]),
+    }
 
 
-def cpu_model():
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return None
+def cpu_baseline_c1(args, a, b, n, bound, gpu):
+    """CPU baseline of C1, exactly the configuration: SELECT SUM(a) WHERE b <=
+    2^30 - 1 over 2^24 rows, int64 accumulator (the oracle's sum_select_cpu,
+    main9.cu:56-61), on one thread and on all threads; best of 5 runs each."""
+    orc = oracle()
+    ha, hb = a.download(np.int32, n), b.download(np.int32, n)
+    T = cpu_threads(args)
+    out = {"rows": n, "threads": T}
+    for name, fn in (("one_thread", lambda: orc.select_sum(hb, ha, bound)[:2]),
+                     ("all_threads", lambda: orc.select_sum_mt(hb, ha, bound, T))):
+        ts, got = [], None
+        for _ in range(5):
+            t0 = time.perf_counter()
+            got = fn()
+            ts.append(time.perf_counter() - t0)
+        ms = min(ts) * 1e3
+        out[name] = {"ms": ms, "grows_per_s": n / (ms / 1e3) / 1e9, "gbps": n * 8 / (ms / 1e3) / 1e9,
+                     "count": int(got[0]), "sum": int(got[1]), "equals_gpu": (int(got[0]), int(got[1])) == gpu}
+    log(f"C1 cpu: 1 thread {out['one_thread']['ms']:.2f} ms, {T} threads {out['all_threads']['ms']:.2f} ms "
+        f"for {n} rows; equal to GPU {out['one_thread']['equals_gpu'] and out['all_threads']['equals_gpu']}")
+    return out
 
 
 # ------------------------------------------------------------------ C1 scan
@@ -343,6 +417,11 @@ def leg_c1(args, st):
                         "count": r.count, "sum": r.sum, "grows_per_s": n / (kms / 1e3) / 1e9}
         key = "c1_2^24_graph100" if n == 1 << 24 else "c1_2^28"
         out[key] = best
+        if n == 1 << 24 and args.cpu:
+            cpu = cpu_baseline_c1(args, a, b, n, bound, (best["count"], best["sum"]))
+            out["cpu"] = cpu
+            best["vs_cpu_all_threads"] = best["grows_per_s"] / cpu["all_threads"]["grows_per_s"]
+            best["vs_cpu_one_thread"] = best["grows_per_s"] / cpu["one_thread"]["grows_per_s"]
         for x in (a, b, res):
             x.close()
     out["c1_fused_batch"] = c1_fused_batch(args, st, bound)
@@ -546,11 +625,42 @@ def leg_c3(args, st):
             "paths_equal": same, "totals_equal_probe_sum": bool(totals_ok), "oracle_equal_first_2^22": oracle_ok}
         log(f"C3 s={sel:.0%}: {best} {ms:.3f} ms ({nr / ms / 1e6:.1f} Grows/s), paths equal {same}, "
             f"totals {totals_ok}, oracle {oracle_ok}")
+        if sel == 0.50 and args.cpu:
+            cpu = cpu_baseline_c3(args, st, rc, rk, rv, h["sk"], h["sg"], nr, bound, G, (cnt, gs))
+            out["cpu_s50"] = cpu
+            out["selectivity"]["50%"]["vs_cpu_all_threads"] = (nr / (ms / 1e3) / 1e9) / cpu["grows_per_s"]
     orc.free(otab)
     for b in (rc, rk, rv, sk, sg, ht.buf, res, gbuf, ws):
         b.close()
     mgv.load().mgv_free(T)
     return out
+
+
+def cpu_baseline_c3(args, st, rc, rk, rv, sk, sg, nr, bound, G, gpu):
+    """CPU baseline of C3 at one selectivity: the oracle's chained table over
+    S (threaded build), then select -> probe -> GROUP BY S.g over all R rows
+    on all threads, and over the first 2^24 rows on one thread; the group
+    arrays of all rows must equal the GPU's."""
+    orc = oracle()
+    st.sync()
+    h = {k: b.download(np.int32, nr) for k, b in (("rc", rc), ("rk", rk), ("rv", rv))}
+    T = cpu_threads(args)
+    t0 = time.perf_counter()
+    tab = orc.table(sk, sg, threads=T)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    wc, wsum = orc.probe_groupby(tab, h["rk"], h["rv"], h["rc"], bound, G, threads=T)
+    t_all = time.perf_counter() - t0
+    M = min(nr, 1 << 24)
+    t0 = time.perf_counter()
+    orc.probe_groupby(tab, h["rk"][:M], h["rv"][:M], h["rc"][:M], bound, G)
+    t_one = time.perf_counter() - t0
+    orc.free(tab)
+    equal = bool(np.array_equal(wc, gpu[0]) and np.array_equal(wsum, gpu[1]))
+    log(f"C3 cpu (s=50%): {T} threads {t_all:.2f} s for {nr} rows, 1 thread {t_one:.2f} s for {M} rows; "
+        f"equal to GPU {equal}")
+    return {"threads": T, "build_s": t_build, "probe_groupby_s": t_all, "rows": nr, "grows_per_s": nr / t_all / 1e9,
+            "one_thread": {"rows": M, "s": t_one, "grows_per_s": M / t_one / 1e9}, "equals_gpu": equal}
 
 
 # ------------------------------------------------------------------ C4 batch
@@ -735,7 +845,7 @@ def main():
     ap.add_argument("--c5-requests", type=int, default=300)
     ap.add_argument("--c4-reps", type=int, default=5)
     ap.add_argument("--cpu", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 25)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 25, help="R rows of the one-thread C2 figure")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
 
@@ -764,6 +874,19 @@ def main():
         mgv.load_ops()
         extra["c5"] = leg_c5(args, st)
 
+    cpu = None
+    if args.cpu and "cpu_c2" in c2:
+        c = c2["cpu_c2"]
+        cpu = {"value": c["grows_per_s"], "unit": "Grows/s", "cores": c["threads"], "kind": "port",
+               "sample": f"oracle chained hash join (the reference's CPU algorithm, main8.cu:84-103): threaded "
+                         f"build of all {1 << 24} S rows, probe of all {1 << 28} R rows on {c['threads']} threads; "
+                         f"one-thread figure on the first {c['one_thread']['probe_rows']} R rows",
+               **cpu_info(), "c2": c}
+        if "c1" in extra and "cpu" in extra["c1"]:
+            cpu["c1"] = extra["c1"]["cpu"]
+        if "c3" in extra and "cpu_s50" in extra["c3"]:
+            cpu["c3_s50"] = extra["c3"]["cpu_s50"]
+
     line = {
         "metric": "probe Grows/s & %HBM peak; batched vs plain queries/s at bit-exact, p99 lat",
         "value": c2["value"],
@@ -783,8 +906,7 @@ def main():
                    "probe_path": c2["roofline"]["probe_path"], "fanout": c2["roofline"]["fanout"],
                    "parallelism": "1 GPU"},
         "roofline": c2["roofline"],
-        "cpu_baseline": c2.get("cpu_baseline"),
-        "sample_parity": c2.get("sample_parity"),
+        "cpu_baseline": cpu,
         "probe_kernel_ms": {k: c2[k] for k in ("kernel_ms_avg", "kernel_ms_min", "kernel_ms_median")},
         "probe_path_sweep_ms": c2["probe_path_sweep_ms"],
         "build": c2["build"],
@@ -792,7 +914,47 @@ def main():
         "device": {"name": p.name.decode(), "arch": p.arch.decode(), "cus": p.cus},
     }
     line.update(extra)
+    failed = parity_failures(line)
+    line["parity_ok"] = not failed
     print(json.dumps(line), flush=True)
+    if failed:
+        log("PARITY FAILURE: " + ", ".join(failed))
+        sys.exit(3)
+
+
+def parity_failures(line):
+    """Names of the parity flags of the JSON line that are false."""
+    flags = {
+        "c2 timed path = closed form": line["result"]["timed_path_equals_closed_form"],
+        "c2 every candidate = closed form": line["result"]["every_candidate_equals_closed_form"],
+    }
+    cpu = line.get("cpu_baseline") or {}
+    if "c2" in cpu:
+        flags["c2 cpu = gpu"] = cpu["c2"]["equals_gpu"]
+        flags["c2 cpu one-thread sample = gpu"] = cpu["c2"]["one_thread_sample_equals_gpu"]
+    for k in ("one_thread", "all_threads"):
+        if k in cpu.get("c1", {}):
+            flags[f"c1 cpu {k} = gpu"] = cpu["c1"][k]["equals_gpu"]
+    if "c3_s50" in cpu:
+        flags["c3 cpu = gpu"] = cpu["c3_s50"]["equals_gpu"]
+    c1 = line.get("c1")
+    if c1 and "c1_fused_batch" in c1:
+        flags["c1 fused batch = per-query calls"] = c1["c1_fused_batch"]["results_equal_per_query_calls"]
+    for sel, v in (line.get("c3") or {}).get("selectivity", {}).items():
+        flags[f"c3 {sel} paths equal"] = v["paths_equal"]
+        flags[f"c3 {sel} totals"] = v["totals_equal_probe_sum"]
+        flags[f"c3 {sel} oracle"] = v["oracle_equal_first_2^22"]
+    c4 = line.get("c4")
+    if c4:
+        flags["c4 fused = plain"] = c4["parity"]["fused_equals_plain"]
+        flags["c4 oracle"] = c4["parity"]["oracle_equal"]
+    for k, v in (line.get("f3") or {}).items():
+        flags[f"f3 {k} kept rows"] = v["kept_rows_ok"]
+    c5 = line.get("c5")
+    if c5:
+        flags["c5 mix = alone = saturated"] = c5["parity"]["mix_equals_alone_equals_saturated"]
+        flags["c5 no build/group errors"] = c5["parity"]["build_or_group_errors"] == 0
+    return [k for k, v in flags.items() if not v]
 
 
 if __name__ == "__main__":

@@ -468,7 +468,16 @@ __global__ __launch_bounds__(256) void k_lds_build_overflow(LdsBuildArgs a) {
 
 // Level-2 fan-out for a table of nb buckets: the smallest power of two
 // F2 >= 2 * F1 that divides nb into slices of at most kSliceMax buckets.
+// When F1 alone already cuts the table into such slices (F1 > kMaxFanout
+// asks for it), there is one level only: F2 = F1.
 static int lds_fanout(uint32_t nb, uint32_t F1, uint32_t *F2) {
+    if (F1 > kMaxFanout) {
+        if (nb % F1 != 0 || nb / F1 > kSliceMax || nb / F1 == 0)
+            return fail(MGV_E_INVALID, "LDS probe: one level of %u partitions does not cut %u buckets into slices "
+                        "of <= %u buckets", F1, nb, kSliceMax);
+        *F2 = F1;
+        return MGV_OK;
+    }
     uint32_t f = 2 * F1;
     while (f <= kMaxFanoutWide && (nb % f != 0 || nb / f > kSliceMax)) f <<= 1;
     if (f > kMaxFanoutWide || nb % f != 0 || nb / f > kSliceMax || nb / f == 0)
@@ -478,20 +487,24 @@ static int lds_fanout(uint32_t nb, uint32_t F1, uint32_t *F2) {
     return MGV_OK;
 }
 
+// Workspace of the LDS paths: 256 bytes of counters, then the level-1 and
+// level-2 partitionings (one level: only the second, at fan-out F1).
 static int lds_plan(uint64_t n, uint32_t nb, uint32_t F1, void *ws, uint64_t ws_bytes, PartLayout *L1,
                     PartLayout *L2, uint32_t *F2, uint64_t *need) {
-    if (F1 < 2 || F1 > kMaxFanout || (F1 & (F1 - 1)))
-        return fail(MGV_E_INVALID, "LDS probe: level-1 fan-out %u not a power of two in [2,%u]", F1, kMaxFanout);
+    if (F1 < 2 || F1 > kMaxFanoutWide || (F1 & (F1 - 1)))
+        return fail(MGV_E_INVALID, "LDS probe: level-1 fan-out %u not a power of two in [2,%u]", F1, kMaxFanoutWide);
     int rc = lds_fanout(nb, F1, F2);
     if (rc) return rc;
+    const bool one = *F2 == F1;
     uint64_t b1 = 0, b2 = 0;
-    if ((rc = plan_layout(n, F1, 2, nullptr, 0, nullptr, &b1, 1))) return rc;
+    if (!one && (rc = plan_layout(n, F1, 2, nullptr, 0, nullptr, &b1, 1))) return rc;
     if ((rc = plan_layout(n, *F2, 2, nullptr, 0, nullptr, &b2, kPartAlign))) return rc;
     const uint64_t o1 = 256, o2 = 256 + (b1 + 255) / 256 * 256;
     if (need) *need = o2 + b2;
     if (!L1) return MGV_OK;
     if (!ws || ws_bytes < o2 + b2) return fail(MGV_E_CAPACITY, "LDS probe: workspace too small");
-    if ((rc = plan_layout(n, F1, 2, (char *)ws + o1, b1, L1, nullptr, 1))) return rc;
+    if (!one && (rc = plan_layout(n, F1, 2, (char *)ws + o1, b1, L1, nullptr, 1))) return rc;
+    if (one) L1->F = 0;   // marks the single level
     return plan_layout(n, *F2, 2, (char *)ws + o2, b2, L2, nullptr, kPartAlign);
 }
 
@@ -539,10 +552,14 @@ int mgv_hashjoin_probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32
     hipStream_t s = stream_of(st);
     // level 1: select + hash partitions, packed (the total is the survivor count)
     PartIn in1 = {pred, bound, keys, vals, 0, 0u, 0u, 0u, nullptr, nullptr, 1u};   // writes fmix32(key)
-    if ((rc = partition_rows(L1, in1, s, scatter_shape(LC)))) return rc;
-    // level 2: the level-1 output, ordered by the same hash bits, split finer
-    PartIn in2 = {nullptr, 0, L1.col[0], L1.col[1], 0, 0u, 0u, 0u, nullptr, L1.pstart + L1.F, 2u};   // hashes
-    if ((rc = partition_rows(L2, in2, s, scatter_shape(LC)))) return rc;
+    if (L1.F == 0) {   // one level: straight into the slices' partitions
+        if ((rc = partition_rows(L2, in1, s, scatter_shape(LC)))) return rc;
+    } else {
+        if ((rc = partition_rows(L1, in1, s, scatter_shape(LC)))) return rc;
+        // level 2: the level-1 output, ordered by the same hash bits, split finer
+        PartIn in2 = {nullptr, 0, L1.col[0], L1.col[1], 0, 0u, 0u, 0u, nullptr, L1.pstart + L1.F, 2u};   // hashes
+        if ((rc = partition_rows(L2, in2, s, scatter_shape(LC)))) return rc;
+    }
     uint32_t *next = (uint32_t *)ws;
     if ((rc = mgv_fill_u32(next, 1, 0, (mgv_stream_t)s))) return rc;
     static thread_local int cached = 0;
@@ -582,9 +599,13 @@ int mgv_hashjoin_build_lds(uint64_t *slots, uint32_t nbuckets, const int32_t *ke
     hipStream_t s = stream_of(st);
     const int32_t *pcol = payload_mode == MGV_PAYLOAD_COLUMN ? payload : nullptr;   // NONE: zeros
     PartIn in1 = {pred, bound, keys, pcol, 0, 0u, 0u, 0u, nullptr, nullptr};
-    if ((rc = partition_rows(L1, in1, s, scatter_shape(LC)))) return rc;
-    PartIn in2 = {nullptr, 0, L1.col[0], L1.col[1], 0, 0u, 0u, 0u, nullptr, L1.pstart + L1.F};
-    if ((rc = partition_rows(L2, in2, s, scatter_shape(LC)))) return rc;
+    if (L1.F == 0) {
+        if ((rc = partition_rows(L2, in1, s, scatter_shape(LC)))) return rc;
+    } else {
+        if ((rc = partition_rows(L1, in1, s, scatter_shape(LC)))) return rc;
+        PartIn in2 = {nullptr, 0, L1.col[0], L1.col[1], 0, 0u, 0u, 0u, nullptr, L1.pstart + L1.F};
+        if ((rc = partition_rows(L2, in2, s, scatter_shape(LC)))) return rc;
+    }
     uint32_t *ctr = (uint32_t *)ws;   // [0]: partition counter, [2..3]: overflow count
     if ((rc = mgv_fill_u32(ctr, 4, 0, (mgv_stream_t)s))) return rc;
     LdsBuildArgs a = {slots, nbuckets, nbuckets / F2, L2, ctr, (unsigned long long *)(ctr + 2), okey, opay, res};

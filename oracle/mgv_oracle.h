@@ -118,7 +118,22 @@ typedef struct orc_chained_table {
 
 int      orc_ct_build(orc_chained_table *t, const int32_t *keys, const int32_t *payload,
                       uint64_t n, uint32_t log_buckets);
+/* The same table built by nthreads threads with an atomic exchange on the
+ * bucket head (the reference's device build, hashjoin.cu:54-55); chain
+ * order differs from orc_ct_build, chain contents do not. */
+int      orc_ct_build_mt(orc_chained_table *t, const int32_t *keys, const int32_t *payload,
+                         uint64_t n, uint32_t log_buckets, int nthreads);
 void     orc_ct_free(orc_chained_table *t);
+
+/* Join against a build side with distinct keys in [0, kmax) by direct
+ * addressing: COUNT and int64 SUM(rv[r] + sw[s]) over the matching rows
+ * (optionally pred[r] <= bound).  For C2 (build keys a permutation of
+ * [0, 2^24)) this is the closed form COUNT = #{r : R.k < 2^24}, SUM = sum
+ * of R.v + w[perm^-1(R.k)].  Returns -1 for an out-of-range or repeated
+ * build key.  sw, pred and rv may be NULL. */
+int      orc_join_dense(const int32_t *sk, const int32_t *sw, uint64_t ns, uint32_t kmax,
+                        const int32_t *pred, int32_t bound, const int32_t *rk, const int32_t *rv,
+                        uint64_t nr, uint64_t *count, int64_t *sum, int nthreads);
 
 /* Probe: for every probe row r (optionally filtered by pred[r] <= bound when
  * pred != NULL), for every build row s with key[s] == rk[r]:
@@ -145,6 +160,10 @@ void     orc_ct_probe_mt(const orc_chained_table *t, const int32_t *pred, int32_
 void     orc_ct_probe_groupby(const orc_chained_table *t, const int32_t *pred, int32_t bound,
                               const int32_t *rk, const int32_t *rv, uint64_t nr,
                               uint64_t *cnt, int64_t *sum, uint32_t ngroups);
+/* orc_ct_probe_groupby on nthreads threads (per-thread group arrays). */
+int      orc_ct_probe_groupby_mt(const orc_chained_table *t, const int32_t *pred, int32_t bound,
+                                 const int32_t *rk, const int32_t *rv, uint64_t nr,
+                                 uint64_t *cnt, int64_t *sum, uint32_t ngroups, int nthreads);
 
 /* Plain group-by: cnt[key[i]] += 1, sum[key[i]] += val[i]; keys in [0, ngroups). */
 void     orc_groupby(const int32_t *key, const int32_t *val, uint64_t n,

@@ -63,6 +63,10 @@ class Oracle:
         L.orc_select_sum_mt.argtypes = [VP, VP, U64, I32, C.POINTER(U64), C.POINTER(C.c_int64), C.c_int]
         L.orc_ct_build.argtypes = [C.POINTER(ChainedTable), VP, VP, U64, U32]
         L.orc_ct_free.argtypes = [C.POINTER(ChainedTable)]
+        L.orc_ct_build_mt.argtypes = [C.POINTER(ChainedTable), VP, VP, U64, U32, C.c_int]
+        L.orc_join_dense.argtypes = [VP, VP, U64, U32, VP, I32, VP, VP, U64, C.POINTER(U64), C.POINTER(C.c_int64),
+                                     C.c_int]
+        L.orc_ct_probe_groupby_mt.argtypes = [C.POINTER(ChainedTable), VP, I32, VP, VP, U64, VP, VP, U32, C.c_int]
         L.orc_ct_probe.argtypes = [C.POINTER(ChainedTable), VP, I32, VP, VP, U64, U64, C.POINTER(JoinResult)]
         L.orc_ct_probe_mt.argtypes = [C.POINTER(ChainedTable), VP, I32, VP, VP, U64, C.POINTER(JoinResult), C.c_int]
         L.orc_ct_probe_groupby.argtypes = [C.POINTER(ChainedTable), VP, I32, VP, VP, U64, VP, VP, U32]
@@ -130,11 +134,14 @@ class Oracle:
         return c.value, s.value
 
     # -------------------------------------------------------------- join
-    def table(self, keys, payload=None, log_buckets=None):
+    def table(self, keys, payload=None, log_buckets=None, threads=1):
         if log_buckets is None:
             log_buckets = max(4, int(np.ceil(np.log2(max(1, len(keys))))) + 1)
         t = ChainedTable()
-        rc = self.L.orc_ct_build(C.byref(t), _ptr(keys), _ptr(payload), len(keys), log_buckets)
+        if threads > 1:
+            rc = self.L.orc_ct_build_mt(C.byref(t), _ptr(keys), _ptr(payload), len(keys), log_buckets, threads)
+        else:
+            rc = self.L.orc_ct_build(C.byref(t), _ptr(keys), _ptr(payload), len(keys), log_buckets)
         assert rc == 0
         t._keep = (keys, payload)
         return t
@@ -151,12 +158,26 @@ class Oracle:
         return {"count": r.count, "sum": r.sum, "digest_sum": r.digest_sum, "digest_xor": r.digest_xor,
                 "count32": r.count32}
 
-    def probe_groupby(self, t, rk, rv, pred, bound, ngroups):
+    def probe_groupby(self, t, rk, rv, pred, bound, ngroups, threads=1):
         cnt = np.zeros(ngroups, np.uint64)
         s = np.zeros(ngroups, np.int64)
-        self.L.orc_ct_probe_groupby(C.byref(t), _ptr(pred), bound, _ptr(rk), _ptr(rv), len(rk), _ptr(cnt), _ptr(s),
-                                    ngroups)
+        if threads > 1:
+            rc = self.L.orc_ct_probe_groupby_mt(C.byref(t), _ptr(pred), bound, _ptr(rk), _ptr(rv), len(rk),
+                                                _ptr(cnt), _ptr(s), ngroups, threads)
+            assert rc == 0
+        else:
+            self.L.orc_ct_probe_groupby(C.byref(t), _ptr(pred), bound, _ptr(rk), _ptr(rv), len(rk), _ptr(cnt),
+                                        _ptr(s), ngroups)
         return cnt, s
+
+    def join_dense(self, sk, sw, kmax, rk, rv=None, pred=None, bound=0, threads=1):
+        """(COUNT, SUM) of the join against distinct build keys in [0, kmax)
+        by direct addressing; None when a build key is repeated or out of
+        range."""
+        c, s = U64(), C.c_int64()
+        rc = self.L.orc_join_dense(_ptr(sk), _ptr(sw), len(sk), kmax, _ptr(pred), bound, _ptr(rk), _ptr(rv), len(rk),
+                                   C.byref(c), C.byref(s), threads)
+        return None if rc else (c.value, s.value)
 
     def groupby(self, keys, vals, ngroups):
         cnt = np.zeros(ngroups, np.uint64)

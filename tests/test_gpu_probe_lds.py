@@ -250,3 +250,62 @@ def test_tiny_probes_through_both_partitioned_paths(gpu, orc, n):
         mgv.sync()
         r = res.read()
         assert (r.count, r.sum) == want
+
+
+def _gen(name, n, *args):
+    buf = mgv.DeviceBuffer(4 * n)
+    mgv.call(name, buf.vp, n, *args, None)
+    return buf
+
+
+@pytest.mark.parametrize("fanout", [32, 64])
+def test_c2_table_geometry_lds_build_and_probe_equal_closed_form(gpu, orc, fanout):
+    """C2's own table: 2^24 distinct keys (a seeded permutation) at load
+    factor 0.5 is 2^22 buckets, so both LDS paths run with a level-2 fan-out
+    of 2048 (the 2048-way count and scatter instances and 2048 LDS slices of
+    2048 buckets).  2^25 probe rows of the C2 distribution; COUNT and SUM
+    must equal the closed form COUNT = #{R.k < 2^24}, SUM = sum of R.v +
+    w[perm^-1(R.k)], computed on the CPU by direct addressing."""
+    ns, nr = 1 << 24, 1 << 25
+    sk = _gen("mgv_gen_perm", ns, ns, mgv.seed(2, 0), 0)
+    sw = _gen("mgv_gen_bits", ns, mgv.seed(2, 1), 0, 16)
+    rk = _gen("mgv_gen_bits", nr, mgv.seed(2, 2), 0, 25)
+    rv = _gen("mgv_gen_bits", nr, mgv.seed(2, 3), 0, 16)
+    mgv.sync()
+    hk, hw, hrk, hrv = sk.download(np.int32, ns), sw.download(np.int32, ns), rk.download(np.int32, nr), \
+        rv.download(np.int32, nr)
+    # the device generator is the oracle's (spot check of every column)
+    assert np.array_equal(hk[:4096], orc.gen_perm(4096, ns, orc.seed(2, 0), 0))
+    assert np.array_equal(hrk[-4096:], orc.gen_bits(nr, orc.seed(2, 2), 0, 25)[-4096:])
+    assert np.array_equal(hw[:4096], orc.gen_bits(4096, orc.seed(2, 1), 0, 16))
+    assert np.array_equal(hrv[:4096], orc.gen_bits(4096, orc.seed(2, 3), 0, 16))
+    want = orc.join_dense(hk, hw, ns, hrk, hrv, threads=8)
+    assert want[0] == int((hrk < ns).sum())
+
+    ht = mgv.HashTable(ns, 0.5)
+    assert ht.nbuckets == 1 << 22
+    need = C.c_uint64()
+    mgv.call("mgv_build_lds_workspace_size", ns, ht.nbuckets, fanout, C.byref(need))
+    bws = mgv.DeviceBuffer(need.value)
+    res = mgv.ResultBuffer(3)
+    mgv.call("mgv_hashjoin_build_lds", ht.buf.vp, ht.nbuckets, sk.vp, sw.vp, mgv.MGV_PAYLOAD_COLUMN, None, 0, ns, 0,
+             fanout, bws.vp, bws.nbytes, res.slot(0), mgv.Launch(0, 1024, -1, 2 << 8), None)
+    mgv.sync()
+    bws.close()
+    assert res.read(0).errors == 0
+    _assert_prefix_filled(ht.slots(), ht.nbuckets, ns)
+
+    mgv.call("mgv_probe_lds_workspace_size", nr, ht.nbuckets, fanout, C.byref(need))
+    ws = mgv.DeviceBuffer(need.value)
+    for rep in range(2):   # the workspace is reused as in the bench's steps
+        res.clear(None)
+        mgv.call("mgv_hashjoin_probe_lds", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp, nr, fanout, ws.vp,
+                 ws.nbytes, res.slot(1), mgv.Launch(0, 1024, -1, 2 << 8), None)
+        mgv.sync()
+        r = res.read(1)
+        assert (r.count, r.sum) == want, (rep, fanout)
+    mgv.call("mgv_hashjoin_probe", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp, nr, 0, 0, res.slot(2),
+             mgv.launch(0, 256), None)
+    mgv.sync()
+    r = res.read(2)
+    assert (r.count, r.sum) == want

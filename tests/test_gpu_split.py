@@ -93,9 +93,24 @@ def test_split_appends_to_the_cursors(gpu, orc):
         assert np.array_equal(canon(s.parent(p)), canon(want[p]))
 
 
+@pytest.mark.parametrize("n", [16383, 16384, 16385, 16384 * 5 + 1])
+@pytest.mark.parametrize("filtered", [False, True])
+def test_split_one_column_tile_edges(gpu, orc, n, filtered):
+    """One column is split in tiles of 16384 rows (512 threads x 32 rows):
+    inputs just below, at and above a tile, and one row past five tiles."""
+    cols = seeded_cols(orc, n, 1, 45)
+    pred = orc.gen_range(n, orc.seed(20, 9), 9, 1000) if filtered else None
+    want = orc.split(cols, 2, pred=pred, bound=499)
+    s = Split(2, 1, n)
+    s.run(cols, pred=pred, bound=499)
+    assert s.cursors().tolist() == [len(want[0][0]), len(want[1][0])]
+    for p in range(2):
+        assert np.array_equal(canon(s.parent(p)), canon(want[p]))
+
+
 @pytest.mark.parametrize("grid", [1, 3, 7])
 def test_split_small_grids_cover_every_tile(gpu, orc, grid):
-    n = 32768 * 9 + 5   # nine whole tiles of one column and a ragged one
+    n = 16384 * 18 + 5   # eighteen whole 16384-row tiles of one column and a ragged one
     cols = seeded_cols(orc, n, 1, 40)
     want = orc.split(cols, 2)
     s = Split(2, 1, n)

@@ -339,3 +339,99 @@ def test_split_partitions_the_kept_rows_by_parity(orc, nparents, ncols, filtered
 def test_split_of_nothing(orc):
     parts = orc.split([np.zeros(0, np.int32)], 2)
     assert [len(p[0]) for p in parts] == [0, 0]
+
+
+# ----------------------------------------------------------------- mix64
+
+# SplitMix64 (Steele, Lea, Flood, OOPSLA'14; reference C code by S. Vigna):
+# state += 0x9e3779b97f4a7c15, output = mix64(state).  Published output
+# sequences of the reference generator for two seeds.
+SPLITMIX64_KAT = [
+    (1234567, [6457827717110365317, 3203168211198807973, 9817491932198370423, 4593380528125082431,
+               16408922859458223821]),
+    (1477776061723855037, [1985237415132408290, 2979275885539914483, 13511426838097143398]),
+]
+
+
+@pytest.mark.parametrize("seed,outs", SPLITMIX64_KAT)
+def test_mix64_is_the_splitmix64_finaliser(orc, seed, outs):
+    s = seed
+    for want in outs:
+        s = (s + 0x9E3779B97F4A7C15) & (2**64 - 1)
+        assert orc.L.orc_mix64(s) == want
+
+
+# ----------------------------------------------------------------- Zipf, s != 1
+
+@pytest.mark.parametrize("n,s", [(1 << 16, 0.8), (4096, 1.3), (1000, 0.5)])
+def test_zipf_cdf_weights_and_normaliser(orc, n, s):
+    """Threshold gaps are the Zipf weights: (T[k] - T[k-1]) / T[0] = (k+1)^-s,
+    and T[0] / 2^32 = 1 / H(n, s) with the generalized harmonic number from
+    the Euler-Maclaurin expansion around Riemann's zeta (scipy), not from the
+    sum the oracle adds up."""
+    from scipy.special import zeta
+    T = orc.zipf_cdf(n, s).astype(np.float64)
+    w = np.diff(np.concatenate([[0.0], T]))
+    k = np.array([1, 2, 3, 7, 100, n // 2, n - 2])
+    # each threshold is floored, so a gap is within 2 of its exact value
+    assert np.all(np.abs(w[k] - w[0] * (k + 1.0) ** -s) <= 2.5)
+    assert T[-1] == 2.0**32
+    assert np.all(np.diff(T) >= 0)
+    H = zeta(s) + n ** (1 - s) / (1 - s) + n**-s / 2 - s * n ** (-s - 1) / 12
+    assert abs(T[0] / 2**32 - 1.0 / H) < 1e-7
+
+
+# ----------------------------------------------------------------- direct-address join, threaded oracle
+
+def test_join_dense_equals_chained_join_and_closed_form(orc, golden):
+    g = golden["c2_small"]
+    sk, sw, rk, rv = _c2_small(orc, g)
+    t = orc.table(sk, sw)
+    c = orc.gen_bits(len(rk), orc.seed(3, 0), 0, 31)
+    for pred, bound in ((None, 0), (c, (1 << 30) - 1), (c, -1)):
+        want = orc.probe(t, rk, rv, pred=pred, bound=bound)
+        for threads in (1, 3):
+            got = orc.join_dense(sk, sw, g["ns"], rk, rv, pred=pred, bound=bound, threads=threads)
+            assert got == (want["count"], want["sum"])
+    orc.free(t)
+    assert orc.join_dense(sk, sw, g["ns"], rk, rv)[0] == int((rk < g["ns"]).sum()) == g["count"]
+    assert orc.join_dense(sk, None, g["ns"], rk, None) == (g["count"], 0)
+
+
+def test_join_dense_rejects_repeated_or_out_of_range_keys(orc):
+    rk = np.arange(10, dtype=np.int32)
+    assert orc.join_dense(np.array([1, 2, 1], np.int32), None, 10, rk) is None
+    assert orc.join_dense(np.array([1, 12], np.int32), None, 10, rk) is None
+    assert orc.join_dense(np.array([-1], np.int32), None, 10, rk) is None
+    assert orc.join_dense(np.array([3, 9], np.int32), np.array([5, -7], np.int32), 10, rk, rk) == (2, 3 + 5 + 9 - 7)
+
+
+def test_threaded_build_and_groupby_equal_serial(orc):
+    ns, nr, G = 50000, 200000, 97
+    sk = orc.gen_range(ns, orc.seed(2, 10), 4, 20000)       # repeated keys
+    sg = orc.gen_range(ns, orc.seed(2, 11), 4, G)
+    rk = orc.gen_range(nr, orc.seed(2, 12), 4, 25000)
+    rv = orc.gen_bits(nr, orc.seed(2, 13), 4, 16)
+    c = orc.gen_bits(nr, orc.seed(3, 0), 4, 31)
+    t1 = orc.table(sk, sg, log_buckets=14)
+    t8 = orc.table(sk, sg, log_buckets=14, threads=8)
+    # the same rows hang off every bucket
+    for b in range(0, 1 << 14, 997):
+        rows = []
+        for t in (t1, t8):
+            heads = np.ctypeslib.as_array((np.ctypeslib.ctypes.c_int32 * (1 << 14)).from_address(t.heads))
+            nxt = np.ctypeslib.as_array((np.ctypeslib.ctypes.c_int32 * ns).from_address(t.next))
+            cur, chain = int(heads[b]), []
+            while cur >= 0:
+                chain.append(cur)
+                cur = int(nxt[cur])
+            rows.append(sorted(chain))
+        assert rows[0] == rows[1]
+    w1 = orc.probe(t1, rk, rv, pred=c, bound=(1 << 30) - 1)
+    assert orc.probe(t8, rk, rv, pred=c, bound=(1 << 30) - 1, threads=5) == w1
+    g1 = orc.probe_groupby(t1, rk, rv, c, (1 << 30) - 1, G)
+    g8 = orc.probe_groupby(t8, rk, rv, c, (1 << 30) - 1, G, threads=6)
+    assert np.array_equal(g1[0], g8[0]) and np.array_equal(g1[1], g8[1])
+    assert int(g1[0].sum()) == w1["count"]
+    orc.free(t1)
+    orc.free(t8)
