@@ -389,6 +389,21 @@ int mgv_hashjoin_probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32
                            const int32_t *keys, const int32_t *vals, uint64_t n, uint32_t fanout, void *ws,
                            uint64_t ws_bytes, mgv_result *res, mgv_launch launch, mgv_stream_t s);
 
+/* mgv_hashjoin_probe_lds without a workspace argument and without any
+ * allocation: the partitionings use a scratch array of
+ * MGV_PROBE_NOALLOC_SCRATCH_BYTES that the library holds in each GPU's
+ * memory from mgv_init until the process ends.  Calls whose workspace
+ * (mgv_probe_lds_workspace_size) exceeds it fail with MGV_E_CAPACITY before
+ * any launch.  Every call on a GPU uses that GPU's one array: calls that
+ * overlap on one GPU (different streams, or host threads) race and must be
+ * serialised by the caller.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under stream capture, serving
+ * loops with a fixed footprint). */
+#define MGV_PROBE_NOALLOC_SCRATCH_BYTES (32ull << 20)
+int mgv_hashjoin_probe_lds_noalloc(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
+                                   const int32_t *keys, const int32_t *vals, uint64_t n, uint32_t fanout,
+                                   mgv_result *res, mgv_launch launch, mgv_stream_t s);
+
 /* ------------------------------------------------------------ select / aggregate */
 
 /* SELECT COUNT(*), SUM(val) WHERE pred <= bound.  Replaces the

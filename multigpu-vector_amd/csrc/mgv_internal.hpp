@@ -21,7 +21,12 @@ struct DeviceState {
     int cus = 0;
     size_t lds_per_block = 0;
     hipStream_t default_stream = nullptr;
+    void *probe_scratch = nullptr;   // this device's copy of the no-alloc probe's scratch array
 };
+
+// Resolves st.probe_scratch (the device must be current); called once per
+// device by mgv_init.
+int probe_scratch_setup(DeviceState &st);
 
 // State of the calling thread's current device (mgv_init must have run).
 DeviceState *current_state();

@@ -526,22 +526,10 @@ static int build_plan(uint64_t n, uint32_t nb, uint32_t F1, void *ws, uint64_t w
     return MGV_OK;
 }
 
-}  // namespace mgv
-
-using namespace mgv;
-
-extern "C" {
-
-int mgv_probe_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes) {
-    if (!bytes || nbuckets == 0) return fail(MGV_E_INVALID, "mgv_probe_lds_workspace_size: bad argument");
-    uint32_t F2 = 0;
-    return lds_plan(n, nbuckets, fanout, nullptr, 0, nullptr, nullptr, &F2, bytes);
-}
-
-int mgv_hashjoin_probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
-                           const int32_t *keys, const int32_t *vals, uint64_t n, uint32_t fanout, void *ws,
-                           uint64_t ws_bytes, mgv_result *res, mgv_launch LC, mgv_stream_t st) {
-    if (!current_state()) return fail(MGV_E_NOT_INIT, "mgv_init was not called");
+// The probe of mgv_hashjoin_probe_lds on workspace ws (both entry points).
+static int probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
+                     const int32_t *keys, const int32_t *vals, uint64_t n, uint32_t fanout, void *ws,
+                     uint64_t ws_bytes, mgv_result *res, mgv_launch LC, mgv_stream_t st) {
     if (!slots || !keys || !res || nbuckets == 0) return fail(MGV_E_INVALID, "probe_lds: null argument");
     if (n == 0) return MGV_OK;
     if (n >= (1ull << 31)) return fail(MGV_E_INVALID, "probe_lds: n must be < 2^31");
@@ -568,6 +556,61 @@ int mgv_hashjoin_probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32
     LdsProbeArgs a = {slots, nbuckets, nbuckets / F2, L2, next, res};
     hipLaunchKernelGGL(k_lds_probe, dim3(grid), dim3(kLdsBlock), 0, s, a);
     return check_launch("k_lds_probe");
+}
+
+// Scratch of mgv_hashjoin_probe_lds_noalloc: the partition workspace of
+// callers that cannot allocate.  The array lives in each GPU's memory for as
+// long as the library is loaded; host code reaches it only through the
+// device address hipGetSymbolAddress returns for the current device (kept in
+// DeviceState::probe_scratch), never through the array's name.
+constexpr uint64_t kProbeScratchBytes = MGV_PROBE_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kProbeScratchBytes];
+
+int probe_scratch_setup(DeviceState &st) {
+    void *p = nullptr;
+    const int rc = hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (rc) return rc;
+    if (!p) return fail(MGV_E_NOT_INIT, "hipGetSymbolAddress(scratch) returned a null address");
+    st.probe_scratch = p;
+    return MGV_OK;
+}
+
+}  // namespace mgv
+
+using namespace mgv;
+
+extern "C" {
+
+int mgv_probe_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes) {
+    if (!bytes || nbuckets == 0) return fail(MGV_E_INVALID, "mgv_probe_lds_workspace_size: bad argument");
+    uint32_t F2 = 0;
+    return lds_plan(n, nbuckets, fanout, nullptr, 0, nullptr, nullptr, &F2, bytes);
+}
+
+int mgv_hashjoin_probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
+                           const int32_t *keys, const int32_t *vals, uint64_t n, uint32_t fanout, void *ws,
+                           uint64_t ws_bytes, mgv_result *res, mgv_launch LC, mgv_stream_t st) {
+    if (!current_state()) return fail(MGV_E_NOT_INIT, "mgv_init was not called");
+    return probe_lds(slots, nbuckets, pred, bound, keys, vals, n, fanout, ws, ws_bytes, res, LC, st);
+}
+
+int mgv_hashjoin_probe_lds_noalloc(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
+                                   const int32_t *keys, const int32_t *vals, uint64_t n, uint32_t fanout,
+                                   mgv_result *res, mgv_launch LC, mgv_stream_t st) {
+    DeviceState *ds = current_state();
+    if (!ds) return fail(MGV_E_NOT_INIT, "mgv_init was not called");
+    if (!ds->probe_scratch) return fail(MGV_E_NOT_INIT, "probe_lds_noalloc: scratch array not resolved");
+    if (!slots || !keys || !res || nbuckets == 0) return fail(MGV_E_INVALID, "probe_lds_noalloc: null argument");
+    if (n == 0) return MGV_OK;
+    uint32_t F2 = 0;
+    uint64_t need = 0;
+    int rc = lds_plan(n, nbuckets, fanout, nullptr, 0, nullptr, nullptr, &F2, &need);
+    if (rc) return rc;
+    if (need > kProbeScratchBytes)
+        return fail(MGV_E_CAPACITY, "probe_lds_noalloc: %llu rows need %llu bytes of scratch, the array holds %llu",
+                    (unsigned long long)n, (unsigned long long)need, (unsigned long long)kProbeScratchBytes);
+    return probe_lds(slots, nbuckets, pred, bound, keys, vals, n, fanout, ds->probe_scratch, kProbeScratchBytes,
+                     res, LC, st);
 }
 
 int mgv_build_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes) {

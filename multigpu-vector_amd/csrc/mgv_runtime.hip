@@ -156,6 +156,11 @@ int mgv_init(int device) {
     st.lds_per_block = p.sharedMemPerBlock;
     rc = hip_status(hipStreamCreateWithFlags(&st.default_stream, hipStreamNonBlocking), "hipStreamCreate");
     if (rc) return rc;
+    if ((rc = probe_scratch_setup(st))) {
+        (void)hipStreamDestroy(st.default_stream);
+        st = DeviceState();
+        return rc;
+    }
     st.init = true;
     return MGV_OK;
 }

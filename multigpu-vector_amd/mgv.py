@@ -170,6 +170,7 @@ SIGNATURES = {
     "mgv_build_lds_workspace_size": [U64, U32, U32, C.POINTER(U64)],
     "mgv_hashjoin_build_lds": [VP, U32, VP, VP, I32, VP, I32, U64, I64, U32, VP, U64, VP, Launch, VP],
     "mgv_hashjoin_probe_lds": [VP, U32, VP, I32, VP, VP, U64, U32, VP, U64, VP, Launch, VP],
+    "mgv_hashjoin_probe_lds_noalloc": [VP, U32, VP, I32, VP, VP, U64, U32, VP, Launch, VP],
     "mgv_groupby_partitioned": [VP, VP, U64, VP, VP, U32, VP, U64, VP, Launch, VP],
     "mgv_split": [VP, I32, VP, U32, U64, U32, VP, U64, VP, Launch, VP],
 }
