@@ -28,6 +28,7 @@ import numpy as np  # noqa: E402
 import mgv  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
+MGV_LAUNCH_COUNTED_PART = 1 << 12
 
 
 def log(*a):
@@ -140,12 +141,13 @@ def leg_c2(args, st):
 
     def make_probe(mode, fanout):
         """mode "direct" (one pass over R), "part" (one-level partitioned,
-        fan-out F) or "lds" (two-level partitioned, slices probed from LDS)."""
+        fan-out F), "lds" (two-level count-free partitioning, slices probed
+        from LDS) or "ldsc" (the same with counted partitionings)."""
         nonlocal ws
         need = C.c_uint64()
         if mode == "part":
             mgv.call("mgv_partition_workspace_size", nr, fanout, 2, C.byref(need))
-        elif mode == "lds":
+        elif mode in ("lds", "ldsc"):
             mgv.call("mgv_probe_lds_workspace_size", nr, ht.nbuckets, fanout, C.byref(need))
         if need.value and (ws is None or ws.nbytes < need.value):
             if ws is not None:
@@ -154,9 +156,10 @@ def leg_c2(args, st):
         if mode == "part":
             return lambda: mgv.call("mgv_hashjoin_probe_partitioned", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp,
                                     nr, 0, 0, fanout, ws.vp, ws.nbytes, res.vp, L, st.vp)
-        if mode == "lds":
+        if mode in ("lds", "ldsc"):
+            Lm = Llds if mode == "lds" else mgv.Launch(0, 1024, -1, Llds.reserved | MGV_LAUNCH_COUNTED_PART)
             return lambda: mgv.call("mgv_hashjoin_probe_lds", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp, nr,
-                                    fanout, ws.vp, ws.nbytes, res.vp, Llds, st.vp)
+                                    fanout, ws.vp, ws.nbytes, res.vp, Lm, st.vp)
         return lambda: mgv.call("mgv_hashjoin_probe", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp, nr, 0, 0,
                                 res.vp, L, st.vp)
 
@@ -164,7 +167,7 @@ def leg_c2(args, st):
     if args.probe != "auto":
         candidates = [(args.probe, args.fanout if args.fanout > 0 else (1 if args.probe == "direct" else 64))]
     else:
-        candidates = [("direct", 1), ("part", 64), ("lds", 32), ("lds", 64)]
+        candidates = [("direct", 1), ("part", 64), ("ldsc", 32), ("lds", 32), ("lds", 64)]
         if "mode" in t2 and (t2["mode"], int(t2["fanout"])) not in candidates:
             candidates.append((t2["mode"], int(t2["fanout"])))
     sweep, sweep_res = {}, {}
@@ -241,8 +244,10 @@ def leg_c2(args, st):
             "kernel": {"direct": "k_pipeline<PProbeSum<false,true,false>> (mgv_hashjoin_probe)",
                        "part": "mgv_hashjoin_probe_partitioned: k_part_prep + k_part_count + k_part_scan + "
                                "k_part_scatter + k_part_probe (events around the whole call)",
-                       "lds": "mgv_hashjoin_probe_lds: two partitionings (k_part_prep/count/scan/scatter x 2) + "
-                              "k_lds_probe (events around the whole call)"}[best_mode],
+                       "ldsc": "mgv_hashjoin_probe_lds (counted): two partitionings (k_part_prep/count/scan/"
+                               "scatter x 2) + k_lds_probe (events around the whole call)",
+                       "lds": "mgv_hashjoin_probe_lds: counter fill + two count-free scatters (k_fill_scatter x 2) + "
+                              "k_lds_probe + k_probe_hashed (events around the whole call)"}[best_mode],
             "probe_path": best_mode,
             "fanout": fanout,
             "scatter_shape": shape,
@@ -486,6 +491,62 @@ def c1_fused_batch(args, st, bound):
     return best
 
 
+# ------------------------------------------------------------------ host-resident columns
+
+def leg_f1(args, st):
+    """The reference's path for host-resident inputs (exchange.cu:196-291):
+    generator -> exchange -> consumer (double-buffered pinned H2D on a copy
+    stream, events) -> select -> aggregation -> gpu_to_cpu, over 2^26 rows of
+    {b, a} in pinned host memory, vectors of 2 Mi rows (two columns fill one
+    4 Mi-row device buffer of the consumer's staging; mgv_ops_sum_select).
+    Reported: its rate; the same plan over device-resident columns; a plain
+    H2D copy of the same bytes; and the overlap factor (copy time + device
+    time) / host-column time, above 1 when copies and kernels overlap."""
+    n, V = 1 << 26, 2 << 20
+    mgv.ops_call("mgv_ops_init", 8, 2, 4)
+    a = gen_bits(n, mgv.seed(1, 0), 7, 20, st.vp)
+    b = gen_bits(n, mgv.seed(1, 1), 7, 31, st.vp)
+    st.sync()
+    ha, hb = C.c_void_p(), C.c_void_p()
+    mgv.call("mgv_host_malloc", C.byref(ha), 4 * n)
+    mgv.call("mgv_host_malloc", C.byref(hb), 4 * n)
+    mgv.call("mgv_memcpy", ha, a.vp, 4 * n)
+    mgv.call("mgv_memcpy", hb, b.vp, 4 * n)
+    bound = (1 << 30) - 1
+    dst = np.zeros(4, np.int32)
+
+    def plan(pb, pa, where):
+        s = C.c_int64()
+        t0 = time.perf_counter()
+        mgv.ops_call("mgv_ops_sum_select", pb, pa, n, bound, V, where, 256, 1024, dst.ctypes.data_as(C.c_void_p),
+                     C.byref(s))
+        return time.perf_counter() - t0, s.value
+
+    host = [plan(hb, ha, -1) for _ in range(3)]
+    dev = [plan(b.vp, a.vp, 0) for _ in range(3)]
+    copy = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        mgv.call("mgv_memcpy", b.vp, hb, 4 * n)
+        mgv.call("mgv_memcpy", a.vp, ha, 4 * n)
+        copy.append(time.perf_counter() - t0)
+    th, tdv, tc = min(x[0] for x in host[1:]), min(x[0] for x in dev[1:]), min(copy[1:])
+    sums = {x[1] for x in host + dev}
+    out = {"rows": n, "vector_rows": V, "bytes": 8 * n,
+           "host_columns": {"ms": th * 1e3, "grows_per_s": n / th / 1e9, "h2d_gbps": 8 * n / th / 1e9},
+           "device_columns": {"ms": tdv * 1e3, "grows_per_s": n / tdv / 1e9},
+           "plain_h2d_copy": {"ms": tc * 1e3, "gbps": 8 * n / tc / 1e9},
+           "overlap": (tc + tdv) / th, "sums_equal": len(sums) == 1}
+    log(f"F1 host columns {th * 1e3:.1f} ms ({8 * n / th / 1e9:.1f} GB/s), device columns {tdv * 1e3:.1f} ms, "
+        f"plain copy {tc * 1e3:.1f} ms ({8 * n / tc / 1e9:.1f} GB/s), overlap {out['overlap']:.2f}, "
+        f"sums equal {out['sums_equal']}")
+    mgv.call("mgv_host_free", ha)
+    mgv.call("mgv_host_free", hb)
+    a.close()
+    b.close()
+    return out
+
+
 # ------------------------------------------------------------------ split
 
 def leg_f3(args, st):
@@ -547,8 +608,9 @@ def leg_c3(args, st):
     in [0, 2^31), k = Zipf(1.0) rank over 2^22 build keys mapped through a
     seeded permutation, v in [0, 2^16)}; S has 2^22 unique keys (a seeded
     permutation) with g in [0, 2^10).  GROUP BY S.g: COUNT, SUM(R.v) WHERE
-    R.c <= t - 1 for selectivities 1/10/50/90 %.  Direct and partitioned
-    paths are timed; checks: the group arrays of both paths agree, their
+    R.c <= t - 1 for selectivities 1/10/50/90 %.  The direct, the
+    partitioned (L2) and the LDS-slice paths are timed; checks: the group
+    arrays of all paths agree, their
     totals agree with a probe-sum kernel (COUNT, and SUM(R.v) = SUM(R.v +
     S.g) - sum_g g * COUNT_g), and the first 2^22 rows agree with the oracle."""
     nr, ns, G = 1 << 28, 1 << 22, 1 << 10
@@ -568,7 +630,11 @@ def leg_c3(args, st):
     assert res.read(0).errors == 0
     gbuf = mgv.DeviceBuffer(16 * G)
     ws = mgv.partition_workspace(nr, 64, 2)
+    need = C.c_uint64()
+    mgv.call("mgv_probe_groupby_lds_workspace_size", nr, ht.nbuckets, 32, C.byref(need))
+    wsl = mgv.DeviceBuffer(need.value)
     L = mgv.launch(0, 256, 1)
+    Llds = mgv.Launch(0, 1024, -1, 2 << 8)
     out = {"probe_rows": nr, "build_rows": ns, "groups": G, "zipf_s": 1.0, "selectivity": {}}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py
@@ -580,12 +646,15 @@ def leg_c3(args, st):
     for sel, t in ((0.01, 21474836), (0.10, 214748365), (0.50, 1073741824), (0.90, 1932735283)):
         bound = t - 1
         paths = {}
-        for name in ("direct", "part64"):
+        for name in ("direct", "part64", "lds32"):
             def run():
                 mgv.call("mgv_fill_u64", gbuf.vp, 2 * G, 0, st.vp)
                 if name == "direct":
                     mgv.call("mgv_hashjoin_probe_groupby", ht.buf.vp, ht.nbuckets, rc.vp, bound, rk.vp, rv.vp, nr,
                              gbuf.vp, gbuf.at(8 * G), G, L, st.vp)
+                elif name == "lds32":
+                    mgv.call("mgv_hashjoin_probe_groupby_lds", ht.buf.vp, ht.nbuckets, rc.vp, bound, rk.vp, rv.vp,
+                             nr, gbuf.vp, gbuf.at(8 * G), G, 32, wsl.vp, wsl.nbytes, Llds, st.vp)
                 else:
                     mgv.call("mgv_hashjoin_probe_groupby_partitioned", ht.buf.vp, ht.nbuckets, rc.vp, bound, rk.vp,
                              rv.vp, nr, gbuf.vp, gbuf.at(8 * G), G, 64, ws.vp, ws.nbytes, L, st.vp)
@@ -597,7 +666,8 @@ def leg_c3(args, st):
                 times.append(tm.durations_ms()[0])
             paths[name] = (min(times[1:]), gbuf.download(np.uint64, G), gbuf.download(np.int64, G, 8 * G))
         cnt, gs = paths["direct"][1], paths["direct"][2]
-        same = bool(np.array_equal(cnt, paths["part64"][1]) and np.array_equal(gs, paths["part64"][2]))
+        same = all(bool(np.array_equal(cnt, paths[x][1]) and np.array_equal(gs, paths[x][2]))
+                   for x in ("part64", "lds32"))
         # probe-sum totals: COUNT and SUM(R.v + S.g)
         res.clear(st.vp)
         mgv.call("mgv_hashjoin_probe", ht.buf.vp, ht.nbuckets, rc.vp, bound, rk.vp, rv.vp, nr, 0, 0, res.slot(1), L,
@@ -616,7 +686,7 @@ def leg_c3(args, st):
                          np.array_equal(gbuf.download(np.int64, G, 8 * G), wsum))
         g8 = 1.0 - (1.0 - sel) ** 8
         algo = nr * (4 + 8 * g8 + 8 * sel)
-        best = min(("direct", "part64"), key=lambda n: paths[n][0])
+        best = min(paths, key=lambda n: paths[n][0])
         ms = paths[best][0]
         out["selectivity"][f"{int(sel * 100)}%"] = {
             "bound": bound, "best_path": best, "ms": ms, "grows_per_s": nr / (ms / 1e3) / 1e9,
@@ -630,7 +700,7 @@ def leg_c3(args, st):
             out["cpu_s50"] = cpu
             out["selectivity"]["50%"]["vs_cpu_all_threads"] = (nr / (ms / 1e3) / 1e9) / cpu["grows_per_s"]
     orc.free(otab)
-    for b in (rc, rk, rv, sk, sg, ht.buf, res, gbuf, ws):
+    for b in (rc, rk, rv, sk, sg, ht.buf, res, gbuf, ws, wsl):
         b.close()
     mgv.load().mgv_free(T)
     return out
@@ -677,10 +747,11 @@ def tuned(workload: str):
 
 def leg_c4(args, st):
     """256 select -> join -> SUM queries: one hipGraph (clear + one persistent
-    launch for the builds + one for the probes) against the plain path (one
-    launch per vector per query).  Both are timed by wall clock from issue to
-    completion; each query's (COUNT, SUM) must agree between the two paths,
-    and with the oracle for the small queries."""
+    launch for the builds + one for the probes), the same graph with the
+    large probes routed through LDS table slices on side streams, and the
+    plain path (one launch per vector per query).  All are timed by wall
+    clock from issue to completion; each query's (COUNT, SUM) must agree
+    between the paths, and with the oracle for the small queries."""
     import workloads
     t = tuned("c4") or {}
     V = int(t.get("vector_rows", 16384))
@@ -718,6 +789,19 @@ def leg_c4(args, st):
     bw, be = timed(run_graph, args.c4_reps)
     fused = batch.results()
     assert int(fused["build_errors"].sum()) == 0
+    mgv.call("mgv_graph_destroy", g)
+    # routed: the large probes through LDS table slices, in the same graph
+    thr = int(t.get("route_threshold", 1 << 20))
+    nst = int(t.get("route_streams", 4))
+    g = batch.capture_routed(V, L, st.vp, thr, nst)
+    timed(run_graph, 2)
+    rw, re_ = timed(run_graph, args.c4_reps)
+    routed = batch.results()
+    routed_same = bool(np.array_equal(routed["count"], fused["count"]) and
+                       np.array_equal(routed["sum"], fused["sum"]))
+    nbig = len(batch._routed(V, thr, nst)["big"])
+    log(f"C4 routed ({nbig} queries >= {thr} rows through LDS slices on {nst} streams): {min(rw) * 1e3:.2f} ms "
+        f"wall, equal to fused: {routed_same}")
     plain_out = {}
     plain_res = None
     for pv in (4096, 65536):
@@ -747,16 +831,23 @@ def leg_c4(args, st):
         checked += 1
     mgv.call("mgv_graph_destroy", g)
     fused_qps = len(shapes) / min(bw)
+    routed_qps = len(shapes) / min(rw)
+    best_w, best_e = (rw, re_) if routed_qps > fused_qps else (bw, be)
     algo = batch.algorithmic_bytes()
     out = {
         "queries": len(shapes), "probe_rows": batch.r_rows, "build_rows": batch.s_rows,
         "tuned": {"vector_rows": V, "block": blk, "compact": compact, "load_factor": lf, "from_file": bool(t)},
-        "batched": {"wall_ms": min(bw) * 1e3, "gpu_ms": min(be), "queries_per_s": fused_qps,
-                    "algorithmic_gbps": algo / (min(be) / 1e3) / 1e9,
-                    "frac": algo / (min(be) / 1e3) / 1e9 / HBM_PEAK_GBPS, "work_items": batch.items(V)},
+        "batched": {"wall_ms": min(best_w) * 1e3, "gpu_ms": min(best_e), "queries_per_s": max(fused_qps, routed_qps),
+                    "path": "routed" if routed_qps > fused_qps else "fused",
+                    "algorithmic_gbps": algo / (min(best_e) / 1e3) / 1e9,
+                    "frac": algo / (min(best_e) / 1e3) / 1e9 / HBM_PEAK_GBPS, "work_items": batch.items(V)},
+        "fused_only": {"wall_ms": min(bw) * 1e3, "gpu_ms": min(be), "queries_per_s": fused_qps},
+        "routed": {"wall_ms": min(rw) * 1e3, "gpu_ms": min(re_), "queries_per_s": routed_qps,
+                   "lds_queries": nbig, "threshold_rows": thr, "side_streams": nst},
         "plain": plain_out,
-        "speedup_vs_best_plain": fused_qps / best_plain["queries_per_s"],
+        "speedup_vs_best_plain": max(fused_qps, routed_qps) / best_plain["queries_per_s"],
         "parity": {"fused_equals_plain": all(v["results_equal_fused"] for v in plain_out.values()),
+                   "routed_equals_fused": routed_same,
                    "oracle_checked_queries": checked, "oracle_equal": bool(ok)},
         "total_count": int(fused["count"].sum()), "total_sum": int(fused["sum"].sum()),
     }
@@ -775,13 +866,15 @@ def leg_c5(args, st):
     over 2^16 Zipf(0.8) groups), admitted against a 16 GiB HBM budget.
     Three runs of the same requests: saturated (all arrive at t=0) to find
     the capacity, Poisson arrivals at 70 % of that capacity (the reported
-    latencies), and one request at a time (the alone reference): every
-    request's (COUNT, SUM, group digest) must be identical in all three."""
+    latencies), Poisson arrivals at 95 % of it (batches merge), and one
+    request at a time (the alone reference): every request's (COUNT, SUM,
+    group digest) must be identical in all four.  Join probe keys follow
+    Zipf(1.0) over their build keys (the skewed select -> join shape)."""
     import workloads
     t0 = time.perf_counter()
     tables = workloads.ServeTables(build_logs=range(16, 27), probe_cap=1 << 29,
                                    group_rows=(1 << 30,) * 5 + (1 << 29,), ngroups=1 << 16, zipf_s=0.8,
-                                   stream=st.vp)
+                                   join_zipf_s=1.0, stream=st.vp)
     st.sync()
     log(f"C5: {tables.bytes / 2**30:.1f} GiB of resident columns generated in {time.perf_counter() - t0:.1f} s")
     n = args.c5_requests
@@ -797,11 +890,17 @@ def leg_c5(args, st):
     mix, sm = workloads.serve(reqs, n, budget, max_inflight=4, max_batch=64, vector_rows=V)
     log(f"C5 Poisson {rate:.1f} q/s: p50 {sm.p50_s * 1e3:.2f} ms p99 {sm.p99_s * 1e3:.2f} ms, "
         f"{sm.queries_per_s:.1f} q/s, mean batch {sm.mean_batch:.1f}")
+    # near capacity: requests keep arriving while others run, so batches merge
+    rate_hi = 0.95 * ss.queries_per_s
+    reqs_hi, _ = tables.requests(n, rate_hi, seed=0xC5)
+    mix_hi, sh = workloads.serve(reqs_hi, n, budget, max_inflight=4, max_batch=64, vector_rows=V)
+    log(f"C5 Poisson {rate_hi:.1f} q/s: p50 {sh.p50_s * 1e3:.2f} ms p99 {sh.p99_s * 1e3:.2f} ms, "
+        f"{sh.queries_per_s:.1f} q/s, mean batch {sh.mean_batch:.1f}")
 
     def trip(r):
         return [(r[i].count, r[i].sum, r[i].digest) for i in range(n)]
 
-    same = trip(mix) == trip(alone) == trip(sat)
+    same = trip(mix) == trip(alone) == trip(sat) == trip(mix_hi)
     errs = sum(mix[i].errors for i in range(n))
     kinds = [d["kind"] for d in desc]
     lat_join = sorted(mix[i].latency_s for i in range(n) if kinds[i] == "join")
@@ -820,8 +919,13 @@ def leg_c5(args, st):
                     "peak_budget_gib": sm.peak_bytes / 2**30,
                     "join_p50_ms": pct(lat_join, 0.5), "join_p99_ms": pct(lat_join, 0.99),
                     "groupby_p50_ms": pct(lat_grp, 0.5), "groupby_p99_ms": pct(lat_grp, 0.99)},
+        "poisson_near_capacity": {"rate_qps": rate_hi, "queries_per_s": sh.queries_per_s, "p50_ms": sh.p50_s * 1e3,
+                                  "p99_ms": sh.p99_s * 1e3, "mean_batch": sh.mean_batch, "batches": sh.batches,
+                                  "max_inflight_seen": sh.max_inflight_seen},
         "saturated": {"queries_per_s": ss.queries_per_s, "p50_ms": ss.p50_s * 1e3, "p99_ms": ss.p99_s * 1e3,
                       "mean_batch": ss.mean_batch},
+        "results_through_completion_ring": int(sm.ring_results + sh.ring_results + ss.ring_results
+                                               + sa.ring_results),
         "one_at_a_time": {"queries_per_s": sa.queries_per_s, "p50_ms": sa.p50_s * 1e3, "p99_ms": sa.p99_s * 1e3},
         "parity": {"mix_equals_alone_equals_saturated": bool(same), "build_or_group_errors": int(errs)},
     }
@@ -837,11 +941,11 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="probe workgroup size (0 = tuned, else 256)")
     ap.add_argument("--coop", type=int, default=-1, help="4-lane cooperative probe (-1 = tuned)")
     ap.add_argument("--load-factor", type=float, default=0.5)
-    ap.add_argument("--probe", default="auto", choices=["auto", "direct", "part", "lds"],
+    ap.add_argument("--probe", default="auto", choices=["auto", "direct", "part", "lds", "ldsc"],
                     help="probe path (auto = time the candidates, keep the fastest)")
     ap.add_argument("--fanout", type=int, default=-1, help="fan-out of the partitioned paths (with --probe)")
     ap.add_argument("--scatter", type=int, default=-1, help="scatter workgroup shape MGV_SCATTER_* (-1 = tuned)")
-    ap.add_argument("--legs", default="c1,c3,c4,c5,f3")
+    ap.add_argument("--legs", default="c1,c3,c4,c5,f3,f1")
     ap.add_argument("--c5-requests", type=int, default=300)
     ap.add_argument("--c4-reps", type=int, default=5)
     ap.add_argument("--cpu", type=int, default=1)
@@ -870,6 +974,9 @@ def main():
         extra["c4"] = leg_c4(args, st)
     if "f3" in legs:
         extra["f3"] = leg_f3(args, st)
+    if "f1" in legs:
+        mgv.load_ops()
+        extra["f1"] = leg_f1(args, st)
     if "c5" in legs:
         mgv.load_ops()
         extra["c5"] = leg_c5(args, st)
@@ -947,9 +1054,12 @@ def parity_failures(line):
     c4 = line.get("c4")
     if c4:
         flags["c4 fused = plain"] = c4["parity"]["fused_equals_plain"]
+        flags["c4 routed = fused"] = c4["parity"]["routed_equals_fused"]
         flags["c4 oracle"] = c4["parity"]["oracle_equal"]
     for k, v in (line.get("f3") or {}).items():
         flags[f"f3 {k} kept rows"] = v["kept_rows_ok"]
+    if line.get("f1"):
+        flags["f1 host columns = device columns"] = line["f1"]["sums_equal"]
     c5 = line.get("c5")
     if c5:
         flags["c5 mix = alone = saturated"] = c5["parity"]["mix_equals_alone_equals_saturated"]

@@ -133,6 +133,10 @@ typedef struct mgv_launch {
 #define MGV_SCATTER_1024x8192 2u
 #define MGV_SCATTER_1024x4096 3u
 #define MGV_LAUNCH_SCATTER(shape) ((uint32_t)(shape) << 8)
+/* Launch flag of mgv_hashjoin_probe_lds: partition with count passes and
+ * exact partition sizes instead of the default count-free partitioning into
+ * fixed-capacity regions (same results; kept for comparison). */
+#define MGV_LAUNCH_COUNTED_PART (1u << 12)
 
 /* ------------------------------------------------------------ datagen */
 
@@ -228,6 +232,11 @@ typedef struct mgv_pool mgv_pool;
 #define MGV_POOL_END (-1)     /* no producer registered and no buffer available (end of stream) */
 #define MGV_POOL_RETRY (-2)   /* 100 attempts failed; the reference's (buffer_t *)1 sentinel */
 int mgv_pool_create(uint32_t nbuffers, uint32_t rows, mgv_pool **pool);
+/* The same pool with its buffers in pinned coherent host memory: kernels
+ * write rows over PCIe and the host reads them in place (mgv_pool_buffer
+ * returns host pointers), so handing a full buffer to the host needs no
+ * copy engine or kernel.  For output bound to the host (gpu_to_cpu). */
+int mgv_pool_create_host(uint32_t nbuffers, uint32_t rows, mgv_pool **pool);
 int mgv_pool_destroy(mgv_pool *pool);
 /* acquire_buffer_blocked_unsafe (src/buffer_pool.cuh:83-96) run by one lane
  * on the device; *buffer = index, MGV_POOL_END or MGV_POOL_RETRY. */
@@ -267,6 +276,71 @@ int mgv_channel_create(uint32_t nslots, mgv_channel **channel);
 int mgv_channel_destroy(mgv_channel *channel);
 int mgv_channel_emit(mgv_channel *channel, uint32_t slot, const mgv_result *res, uint32_t seq, mgv_stream_t s);
 int mgv_channel_poll(mgv_channel *channel, uint32_t slot, mgv_result *out, uint32_t *seq);
+
+/* ------------------------------------------------------------ completion ring */
+
+/* Device -> host ring of completions: slots in pinned coherent host memory,
+ * each claimed by a ticket (one atomic on a device counter) and published
+ * with a system-scope release; the host takes them in ticket order.  Used
+ * for full output buffers (id = pool buffer, value = rows) and completed
+ * queries (id = query id, value = result slot, res = its result).  A slot
+ * the host has not taken yet makes the push fail instead of wait: the entry
+ * is counted as lost and mgv_ring_poll reports MGV_E_CAPACITY, so size the
+ * ring for the most entries that can be outstanding.  Replaces gpu_to_cpu's
+ * ring (src/operators/gpu_to_cpu.cu:42-58 throw2cpu, 149-165 catcher). */
+typedef struct mgv_ring_entry {
+    uint64_t id;
+    uint64_t value;
+    mgv_result res;
+} mgv_ring_entry;
+#define MGV_RING_EOF (~0ull)   /* id of the entry mgv_stream_out_close publishes last */
+typedef struct mgv_ring mgv_ring;
+int mgv_ring_create(uint32_t nslots, mgv_ring **ring);
+int mgv_ring_destroy(mgv_ring *ring);
+/* One entry {id, value, *res (zeros when res is NULL)} pushed by a kernel on
+ * stream s, after the work before it on s. */
+int mgv_ring_emit(mgv_ring *ring, uint64_t id, uint64_t value, const mgv_result *res, mgv_stream_t s);
+/* n entries {ids[i], i, res[i]} (ids and res device arrays). */
+int mgv_ring_emit_results(mgv_ring *ring, const uint64_t *ids, const mgv_result *res, uint32_t n, mgv_stream_t s);
+/* Host: 0 and the oldest entry, 1 when none is published yet; MGV_E_CAPACITY
+ * once an entry was lost.  One host thread polls a ring. */
+int mgv_ring_poll(mgv_ring *ring, mgv_ring_entry *out);
+
+/* ------------------------------------------------------------ rotating output buffers */
+
+/* Hands a pool buffer back from the host without a kernel: the index goes
+ * into the pool's return ring in pinned memory, from which the device takes
+ * it when its stack is empty.  Safe while kernels run. */
+int mgv_pool_release_host(mgv_pool *pool, int32_t buffer);
+
+/* Output of a streaming select: rows are appended to the current pool
+ * buffer with one reservation per wave (buffer::try_write /
+ * try_partial_final_write, src/buffer.cuh:92-134); a wave whose reservation
+ * does not fit replaces the buffer with a fresh one from the pool
+ * (buffer_manager::acquire_buffer_blocked_to, src/buffer_manager.cuh:
+ * 147-176: one lane, a CAS of a tagged {tag, index} word).  A replaced
+ * buffer is published on the ring as {buffer, rows} once every row reserved
+ * in it is written; the host copies it out and hands it back with
+ * mgv_pool_release_host.  When no buffer comes back within 10 s the wave's
+ * rows are dropped and counted (stats), never waited for longer.  The ring
+ * needs nbuffers + 2 slots.  The thread that hands buffers back must not
+ * wait for the device while the kernel runs (hipFree, hipStreamCreate and
+ * hipDeviceSynchronize wait for every queue): the kernel waits for it in
+ * turn, and its rows are dropped when the 10 s pass. */
+typedef struct mgv_stream_out mgv_stream_out;
+int mgv_stream_out_create(mgv_pool *pool, mgv_ring *ring, mgv_stream_out **out);
+int mgv_stream_out_destroy(mgv_stream_out *out);
+/* Starts a new stream of rows (no current buffer; counters zero). */
+int mgv_stream_out_open(mgv_stream_out *out, mgv_stream_t s);
+/* After the last append on s: retires the current buffer and publishes
+ * {MGV_RING_EOF, rows handed over, res.overflow = rows dropped}
+ * (gpu_to_cpu::at_close, src/operators/gpu_to_cpu.cu:108-124). */
+int mgv_stream_out_close(mgv_stream_out *out, mgv_stream_t s);
+/* Appends {val[i] : pred[i] <= bound} (unstable_select -> gpu_to_cpu,
+ * src/operators/select3.cu:19-85 with gpu_to_cpu::consume_warp,
+ * src/operators/gpu_to_cpu.cu:61-105).  val may equal pred. */
+int mgv_select_stream(const int32_t *pred, const int32_t *val, uint64_t n, int32_t bound, mgv_stream_out *out,
+                      mgv_launch launch, mgv_stream_t s);
 
 /* ------------------------------------------------------------ fused query batches */
 
@@ -375,10 +449,18 @@ int mgv_hashjoin_build_lds(uint64_t *slots, uint32_t nbuckets, const int32_t *ke
  * with pred <= bound) are hash-partitioned twice -- `fanout` partitions,
  * then nbuckets / S partitions whose table slices of S <= 2048 buckets
  * (128 KiB) fit in LDS -- and each slice is probed from LDS by one block.
- * nbuckets must split into power-of-two slices of at most 2048 buckets (a
- * power-of-two multiple of 2 * fanout slices, at most 2048 of them);
- * n < 2^31.  The workspace (size from mgv_probe_lds_workspace_size) holds
- * both partitionings (16 bytes per row).  The table must come from
+ * The partitionings need no count passes: each partition is a region of
+ * fixed capacity (a few standard deviations above its expected size)
+ * filled by one atomic reservation per tile, and rows beyond a full region
+ * (skewed keys) go to an overflow area that is probed from the table
+ * directly.  MGV_LAUNCH_COUNTED_PART selects exact counted partitionings
+ * instead (same results).
+ * nbuckets must split into power-of-two slices of at most 2048 buckets:
+ * when nbuckets / fanout is such a slice there is one partitioning level
+ * only, otherwise the second level cuts each of the fanout partitions into
+ * a power of two of slices (at most 2048 in all).  n < 2^31.  The workspace
+ * (size from mgv_probe_lds_workspace_size) holds both partitionings (about
+ * 32 bytes per row with their overflow areas).  The table must come from
  * mgv_hashjoin_build or a batch build (mgv_batch_run), which fill each
  * bucket's slots in order: the LDS copy of a slice keeps per bucket only
  * the keys, the payloads and the number of filled slots.  Replaces
@@ -388,6 +470,24 @@ int mgv_probe_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout,
 int mgv_hashjoin_probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
                            const int32_t *keys, const int32_t *vals, uint64_t n, uint32_t fanout, void *ws,
                            uint64_t ws_bytes, mgv_result *res, mgv_launch launch, mgv_stream_t s);
+
+/* Probe + group-by through LDS-resident table slices (same results as
+ * mgv_hashjoin_probe_groupby): the probe rows (optionally only those with
+ * pred <= bound) are partitioned as in mgv_hashjoin_probe_lds, each slice
+ * of at most 1024 buckets is probed from LDS by one block that aggregates
+ * COUNT and SUM(vals) per group (group = payload of the matching slot,
+ * groups >= ngroups dropped) in an LDS table flushed into gcnt / gsum with
+ * atomics; rows whose partition overflowed (skewed keys) are probed from the
+ * table directly with the same per-block aggregation.  ngroups <= 4096;
+ * nbuckets must split into power-of-two slices of at most 1024 buckets;
+ * workspace from mgv_probe_groupby_lds_workspace_size.  No reference
+ * counterpart for the group-by; the probe replaces
+ * src/operators/hashjoin.cu:135-182. */
+int mgv_probe_groupby_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes);
+int mgv_hashjoin_probe_groupby_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
+                                   const int32_t *keys, const int32_t *vals, uint64_t n, uint64_t *gcnt, int64_t *gsum,
+                                   uint32_t ngroups, uint32_t fanout, void *ws, uint64_t ws_bytes, mgv_launch launch,
+                                   mgv_stream_t s);
 
 /* mgv_hashjoin_probe_lds without a workspace argument and without any
  * allocation: the partitionings use a scratch array of
@@ -399,7 +499,7 @@ int mgv_hashjoin_probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32
  * serialised by the caller.  No counterpart in the reference: for callers
  * that cannot allocate device memory (code under stream capture, serving
  * loops with a fixed footprint). */
-#define MGV_PROBE_NOALLOC_SCRATCH_BYTES (32ull << 20)
+#define MGV_PROBE_NOALLOC_SCRATCH_BYTES (64ull << 20)
 int mgv_hashjoin_probe_lds_noalloc(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
                                    const int32_t *keys, const int32_t *vals, uint64_t n, uint32_t fanout,
                                    mgv_result *res, mgv_launch launch, mgv_stream_t s);

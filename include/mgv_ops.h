@@ -27,6 +27,11 @@ int mgv_ops_init(uint32_t nbuffers, uint32_t buff_buffer_size, uint32_t keep_thr
 /* buffer_manager<int32_t>::destroy (src/buffer_manager.cu:213). */
 int mgv_ops_shutdown(void);
 const char *mgv_ops_last_error(void);
+/* Buffers of the manager's device pool: free in the pool (its stack and the
+ * buffers handed back from the host), held in the host-side cache of
+ * h_get_buffer, and the pool's size; and free pinned host buffers.  With no
+ * plan running, device_free + device_cached == device_total. */
+int mgv_ops_pool_stats(uint32_t *device_free, uint32_t *device_cached, uint32_t *device_total, uint32_t *host_free);
 
 /* generator -> exchange -> unstable_select(less_eq_than(thres)) ->
  * aggregation -> gpu_to_cpu -> exchange -> materializer
@@ -166,6 +171,7 @@ typedef struct mgv_serve_stats {
     uint32_t max_inflight_seen;
     uint64_t peak_bytes;       /* peak reserved HBM of the budget */
     double mean_batch;         /* queries per batch */
+    uint64_t ring_results;     /* results that reached the host through the completion ring */
 } mgv_serve_stats;
 
 /* Continuous-batching request loop.  Requests (sorted by arrival_s) are
@@ -175,8 +181,12 @@ typedef struct mgv_serve_stats {
  * batch: one persistent launch for its builds and one for its probes and
  * group-bys (the fused kernel of mgv_batch_run; a group-by over more than
  * 4096 groups runs as mgv_groupby_partitioned, its workspace counted in its
- * footprint), after which the results are copied to pinned host memory.  Up to max_inflight batches overlap on the
- * GPU; a batch's memory is returned when its results are on the host.
+ * footprint), after which the results are copied to pinned host memory
+ * and each request's {id, result} is published on a device-written
+ * completion ring (mgv_ring, the gpu_to_cpu ring of src/operators/
+ * gpu_to_cpu.cu:42-58,149-165), which the loop polls.  Up to max_inflight
+ * batches overlap on the GPU; a batch's memory is returned when all its
+ * requests have come off the ring.
  * Admission is FIFO (a request that does not fit blocks the ones behind it).
  * Fails with MGV_E_CAPACITY for a request larger than the whole budget. */
 int mgv_serve_run(const mgv_serve_request *requests, uint32_t n, const mgv_serve_config *config,

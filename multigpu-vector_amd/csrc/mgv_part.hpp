@@ -8,6 +8,7 @@
 #include <cstdint>
 
 #include "../../include/mgv.h"
+#include "mgv_wave.hpp"
 
 namespace mgv {
 
@@ -62,5 +63,40 @@ int partition_rows(const PartLayout &L, const PartIn &in, hipStream_t s, uint32_
 
 // The scatter shape requested by a launch descriptor (bits 8-11 of its flags).
 inline uint32_t scatter_shape(const mgv_launch &L) { return (L.reserved >> 8) & 0xfu; }
+
+// Partition of an already hashed key: the top bits of the hash.
+__device__ __forceinline__ uint32_t part_of_hash(int32_t h, uint32_t F) {
+    return (uint32_t)(((uint64_t)(uint32_t)h * F) >> 32);
+}
+
+// Loads the 4 rows [i0, i0+4) of a column (vector load when all are in range).
+__device__ __forceinline__ v4i load_quad(const int32_t *p, uint64_t i0, uint64_t hi) {
+    if (i0 + 3 < hi) return load_stream4(p + i0);
+    v4i q = {0, 0, 0, 0};
+    if (i0 < hi) q.x = load_stream1(p + i0);
+    if (i0 + 1 < hi) q.y = load_stream1(p + i0 + 1);
+    if (i0 + 2 < hi) q.z = load_stream1(p + i0 + 2);
+    return q;
+}
+
+// Exclusive scan of one value per thread over a BLOCK-thread block:
+// wave-level inclusive scan with ds_bpermute shuffles, then the wave totals.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t scan_block_excl(uint32_t v, uint32_t *wtot) {
+    const uint32_t l = lane_id(), w = wave_id();
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, kWave);
+        if (l >= (uint32_t)d) x += y;
+    }
+    if (l == kWave - 1) wtot[w] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t i = 0; i < w; ++i) before += wtot[i];
+    __syncthreads();
+    return before + x - v;
+}
+
 
 }  // namespace mgv

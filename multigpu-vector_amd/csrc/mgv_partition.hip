@@ -32,11 +32,6 @@ __device__ __forceinline__ uint32_t part_of(int32_t key, uint32_t F) {
     return (uint32_t)(((uint64_t)fmix32((uint32_t)key) * F) >> 32);
 }
 
-// Partition of an already hashed key: the top bits of the hash.
-__device__ __forceinline__ uint32_t part_of_hash(int32_t h, uint32_t F) {
-    return (uint32_t)(((uint64_t)(uint32_t)h * F) >> 32);
-}
-
 // Partition of a key: hash partitioning (joins) or key ranges (group-by:
 // partition = key >> rshift, keys outside [0, rlimit) are dropped).
 __device__ __forceinline__ uint32_t part_key(const PartIn &in, int32_t key, uint32_t F) {
@@ -110,16 +105,6 @@ int plan_layout(uint64_t n, uint32_t F, uint32_t ncols, void *ws, uint64_t ws_by
 __global__ void k_part_prep(PartLayout L) {
     for (uint32_t p = threadIdx.x; p < L.F; p += blockDim.x) L.ptotal[p] = 0;
     if (threadIdx.x < 8) L.xhead[threadIdx.x] = 0;
-}
-
-// Loads the 4 rows [i0, i0+4) of a column (vector load when all are in range).
-__device__ __forceinline__ v4i load_quad(const int32_t *p, uint64_t i0, uint64_t hi) {
-    if (i0 + 3 < hi) return load_stream4(p + i0);
-    v4i q = {0, 0, 0, 0};
-    if (i0 < hi) q.x = load_stream1(p + i0);
-    if (i0 + 1 < hi) q.y = load_stream1(p + i0 + 1);
-    if (i0 + 2 < hi) q.z = load_stream1(p + i0 + 2);
-    return q;
 }
 
 __device__ __forceinline__ uint32_t quad_mask(const PartIn &in, uint64_t i0, uint64_t hi, v4i *predq) {
@@ -232,25 +217,6 @@ __global__ __launch_bounds__(kScanThreads) void k_part_scan(PartLayout L) {
         }
     }
     if (t == kScanThreads - 1) L.pstart[L.F] = s[t];
-}
-
-// Exclusive scan of one value per thread over a BLOCK-thread block:
-// wave-level inclusive scan with ds_bpermute shuffles, then the wave totals.
-template <int BLOCK>
-__device__ __forceinline__ uint32_t scan_block_excl(uint32_t v, uint32_t *wtot) {
-    const uint32_t l = lane_id(), w = wave_id();
-    uint32_t x = v;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, kWave);
-        if (l >= (uint32_t)d) x += y;
-    }
-    if (l == kWave - 1) wtot[w] = x;
-    __syncthreads();
-    uint32_t before = 0;
-    for (uint32_t i = 0; i < w; ++i) before += wtot[i];
-    __syncthreads();
-    return before + x - v;
 }
 
 // LDS-staged scatter: each tile of TILE rows is counting-sorted by partition

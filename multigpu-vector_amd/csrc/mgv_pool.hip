@@ -14,6 +14,10 @@
 // wave ever waits on a lock held by another lane: a failed CAS just retries
 // with the value it returned, so lanes of one wave cannot deadlock.
 //
+// The pool's structures and device operations are in mgv_pool.hpp; buffers
+// come back from the host either with a kernel (mgv_pool_release) or,
+// without one, through the pool's return ring (mgv_pool_release_host).
+//
 // Result channel: a pinned host ring of slots {flag, mgv_result}.  A kernel
 // copies a device result into a slot and publishes it with a system-scope
 // release; a host thread polls the flag.  This is the reduced form of
@@ -23,99 +27,32 @@
 #include <cstring>
 #include <new>
 
-#include "mgv_internal.hpp"
-#include "mgv_wave.hpp"
+#include "mgv_pool.hpp"
 
 namespace mgv {
-
-constexpr uint32_t kNull = 0xffffffffu;
-
-struct DevBuffer {           // the device-visible part of buffer<int32_t, cap>
-    uint32_t cnt;
-    uint32_t pad;
-    int32_t *data;
-};
-
-struct DevPool {
-    unsigned long long head;  // (tag << 32) | index, index kNull = empty
-    uint32_t producers;
-    uint32_t available;
-    uint32_t nbuffers;
-    uint32_t rows;
-    uint32_t *next;           // [nbuffers]
-    DevBuffer *bufs;          // [nbuffers]
-    uint32_t *owner;          // [nbuffers] stress-test ownership flags
-};
-
-__device__ __forceinline__ unsigned long long ld_head(DevPool *p) {
-    return __hip_atomic_load(&p->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// push index i (one lane)
-__device__ void stack_push(DevPool *p, uint32_t i) {
-    unsigned long long old = ld_head(p);
-    for (;;) {
-        __hip_atomic_store(&p->next[i], (uint32_t)old, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long nw = (((old >> 32) + 1) << 32) | i;
-        // release: next[i] is visible before the new head
-        if (__hip_atomic_compare_exchange_strong(&p->head, &old, nw, __ATOMIC_RELEASE, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT))
-            return;
-    }
-}
-
-// pop (one lane): index or kNull when empty
-__device__ uint32_t stack_try_pop(DevPool *p) {
-    unsigned long long old = __hip_atomic_load(&p->head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    for (;;) {
-        const uint32_t i = (uint32_t)old;
-        if (i == kNull) return kNull;
-        const uint32_t n = __hip_atomic_load(&p->next[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long nw = (((old >> 32) + 1) << 32) | n;
-        if (__hip_atomic_compare_exchange_strong(&p->head, &old, nw, __ATOMIC_ACQUIRE, __ATOMIC_ACQUIRE,
-                                                 __HIP_MEMORY_SCOPE_AGENT))
-            return i;
-    }
-}
-
-// acquire_buffer_blocked_unsafe (src/buffer_pool.cuh:83-96): at most 100
-// attempts; a buffer index, MGV_POOL_END when no producer is registered and
-// no buffer is available, or MGV_POOL_RETRY when the attempts ran out.
-__device__ int32_t pool_acquire(DevPool *p) {
-    for (int tries = 0; tries < 100; ++tries) {
-        const uint32_t i = stack_try_pop(p);
-        if (i != kNull) {
-            atomicSub(&p->available, 1u);
-            __hip_atomic_store(&p->bufs[i].cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return (int32_t)i;
-        }
-        const uint32_t prod = __hip_atomic_load(&p->producers, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t avail = __hip_atomic_load(&p->available, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (prod == 0 && avail == 0) return MGV_POOL_END;
-    }
-    return MGV_POOL_RETRY;
-}
-
-// release_buffer (src/buffer_pool.cuh:125-130)
-__device__ void pool_release(DevPool *p, uint32_t i) {
-    atomicAdd(&p->available, 1u);
-    stack_push(p, i);
-}
 
 __global__ void k_pool_init(DevPool *p) {
     // one thread: link all buffers into the stack
     for (uint32_t i = 0; i < p->nbuffers; ++i) {
         p->next[i] = (i + 1 < p->nbuffers) ? i + 1 : kNull;
         p->bufs[i].cnt = 0;
+        p->bufs[i].gen = 0;
+        p->bufs[i].done = 0;
+        p->bufs[i].final_rows = 0;
         p->owner[i] = 0;
     }
     p->head = p->nbuffers ? 0ull : (unsigned long long)kNull;
     p->producers = 0;
     p->available = p->nbuffers;
+    p->ret_tail = 0;
 }
 
 __global__ void k_pool_acquire(DevPool *p, int32_t *out) {
-    if (threadIdx.x == 0) *out = pool_acquire(p);
+    if (threadIdx.x == 0) {
+        const int32_t b = pool_acquire(p);
+        if (b >= 0) p->bufs[b].cnt = 0;   // an acquired buffer starts empty
+        *out = b;
+    }
 }
 
 __global__ void k_pool_release(DevPool *p, uint32_t i) {
@@ -136,6 +73,7 @@ __global__ void k_pool_stats(DevPool *p, uint32_t *out) {
         uint32_t n = 0;
         for (uint32_t i = (uint32_t)p->head; i != kNull && n <= p->nbuffers; i = p->next[i]) ++n;
         out[2] = n;   // buffers on the stack (quiescent pool)
+        out[3] = (uint32_t)p->ret_tail;   // return-ring entries taken (low 32 bits)
     }
 }
 
@@ -216,14 +154,6 @@ __global__ void k_emit_result(const mgv_result *src, HostSlot *slot, uint32_t se
 
 using namespace mgv;
 
-struct mgv_pool {
-    DevPool *dev;          // device struct
-    DevPool host;          // host copy of the pointers
-    int32_t *arena;
-    int32_t *acq_slot;     // pinned host int
-    uint32_t *stats;       // pinned host uint[3]
-};
-
 struct mgv_channel {
     HostSlot *slots;       // pinned host ring
     uint32_t nslots;
@@ -231,30 +161,68 @@ struct mgv_channel {
 
 extern "C" {
 
-int mgv_pool_create(uint32_t nbuffers, uint32_t rows, mgv_pool **out) {
+static void pool_free(mgv_pool *P) {
+    (void)hipFree(P->dev);
+    if (P->host_arena) {
+        if (P->arena) (void)hipHostFree(P->arena);
+    } else {
+        (void)hipFree(P->arena);
+    }
+    (void)hipFree(P->host.next);
+    (void)hipFree(P->host.bufs);
+    (void)hipFree(P->host.owner);
+    if (P->acq_slot) (void)hipHostFree(P->acq_slot);
+    if (P->stats) (void)hipHostFree(P->stats);
+    if (P->ret) (void)hipHostFree(P->ret);
+    delete P->ret_mu;
+    delete P;
+}
+
+// A pool whose buffers are device memory, or pinned coherent host memory
+// (host_arena: kernels write the rows over PCIe, the host reads them in
+// place).
+static int pool_create(uint32_t nbuffers, uint32_t rows, bool host_arena, mgv_pool **out) {
     if (!current_state()) return fail(MGV_E_NOT_INIT, "mgv_init was not called");
-    if (!out || nbuffers == 0 || nbuffers >= kNull || rows == 0 || rows % 256)
-        return fail(MGV_E_INVALID, "mgv_pool_create: need nbuffers > 0 and rows a positive multiple of 256");
+    if (!out || nbuffers == 0 || nbuffers >= kNull - 1 || rows == 0 || rows % 256 || rows >= kSeal)
+        return fail(MGV_E_INVALID, "mgv_pool_create: need nbuffers > 0 and rows a positive multiple of 256 below 2^31");
     mgv_pool *P = new (std::nothrow) mgv_pool();
     if (!P) return fail(MGV_E_CAPACITY, "out of host memory");
-    int rc = 0;
+    P->ret_mu = new (std::nothrow) std::mutex();
+    int rc = P->ret_mu ? 0 : fail(MGV_E_CAPACITY, "out of host memory");
+    const uint32_t nret = nbuffers + 1;   // every buffer fits at once: the host never finds it full
     do {
+        if (rc) break;
         if ((rc = hip_status(hipMalloc(&P->dev, sizeof(DevPool)), "hipMalloc"))) break;
-        if ((rc = hip_status(hipMalloc(&P->arena, (size_t)nbuffers * rows * 4), "hipMalloc"))) break;
+        P->host_arena = host_arena;
+        if (host_arena) {
+            if ((rc = hip_status(hipHostMalloc((void **)&P->arena, (size_t)nbuffers * rows * 4, hipHostMallocCoherent),
+                                 "hipHostMalloc")))
+                break;
+        } else if ((rc = hip_status(hipMalloc(&P->arena, (size_t)nbuffers * rows * 4), "hipMalloc"))) {
+            break;
+        }
         if ((rc = hip_status(hipMalloc(&P->host.next, (size_t)nbuffers * 4), "hipMalloc"))) break;
         if ((rc = hip_status(hipMalloc(&P->host.bufs, (size_t)nbuffers * sizeof(DevBuffer)), "hipMalloc"))) break;
         if ((rc = hip_status(hipMalloc(&P->host.owner, (size_t)nbuffers * 4), "hipMalloc"))) break;
         if ((rc = hip_status(hipHostMalloc((void **)&P->acq_slot, 64, hipHostMallocDefault), "hipHostMalloc"))) break;
         if ((rc = hip_status(hipHostMalloc((void **)&P->stats, 64, hipHostMallocDefault), "hipHostMalloc"))) break;
+        if ((rc = hip_status(hipHostMalloc((void **)&P->ret, (size_t)nret * sizeof(ReturnSlot), hipHostMallocCoherent),
+                             "hipHostMalloc")))
+            break;
+        memset(P->ret, 0, (size_t)nret * sizeof(ReturnSlot));
+        P->ret_head = 0;
         P->host.nbuffers = nbuffers;
         P->host.rows = rows;
         P->host.head = 0;
         P->host.producers = 0;
         P->host.available = nbuffers;
+        P->host.ret = P->ret;
+        P->host.nret = nret;
+        P->host.ret_tail = 0;
         // descriptors: data pointers into the arena
         DevBuffer *d = new (std::nothrow) DevBuffer[nbuffers];
         if (!d) { rc = fail(MGV_E_CAPACITY, "out of host memory"); break; }
-        for (uint32_t i = 0; i < nbuffers; ++i) d[i] = DevBuffer{0, 0, P->arena + (size_t)i * rows};
+        for (uint32_t i = 0; i < nbuffers; ++i) d[i] = DevBuffer{0, 0, P->arena + (size_t)i * rows, 0, 0};
         rc = hip_status(hipMemcpy(P->host.bufs, d, (size_t)nbuffers * sizeof(DevBuffer), hipMemcpyHostToDevice),
                         "hipMemcpy");
         delete[] d;
@@ -266,28 +234,37 @@ int mgv_pool_create(uint32_t nbuffers, uint32_t rows, mgv_pool **out) {
         *out = P;
         return MGV_OK;
     } while (0);
-    (void)hipFree(P->dev);
-    (void)hipFree(P->arena);
-    (void)hipFree(P->host.next);
-    (void)hipFree(P->host.bufs);
-    (void)hipFree(P->host.owner);
-    if (P->acq_slot) (void)hipHostFree(P->acq_slot);
-    if (P->stats) (void)hipHostFree(P->stats);
-    delete P;
+    pool_free(P);
     return rc;
+}
+
+int mgv_pool_create(uint32_t nbuffers, uint32_t rows, mgv_pool **out) {
+    return pool_create(nbuffers, rows, false, out);
+}
+
+int mgv_pool_create_host(uint32_t nbuffers, uint32_t rows, mgv_pool **out) {
+    return pool_create(nbuffers, rows, true, out);
 }
 
 int mgv_pool_destroy(mgv_pool *P) {
     if (!P) return MGV_OK;
     (void)hipDeviceSynchronize();
-    (void)hipFree(P->dev);
-    (void)hipFree(P->arena);
-    (void)hipFree(P->host.next);
-    (void)hipFree(P->host.bufs);
-    (void)hipFree(P->host.owner);
-    (void)hipHostFree(P->acq_slot);
-    (void)hipHostFree(P->stats);
-    delete P;
+    pool_free(P);
+    return MGV_OK;
+}
+
+int mgv_pool_release_host(mgv_pool *P, int32_t buffer) {
+    if (!P || buffer < 0 || (uint32_t)buffer >= P->host.nbuffers)
+        return fail(MGV_E_INVALID, "mgv_pool_release_host: bad buffer %d", buffer);
+    std::lock_guard<std::mutex> lk(*P->ret_mu);
+    ReturnSlot *s = &P->ret[P->ret_head % P->host.nret];
+    // the ring has a slot per buffer (plus one), so a slot the device has not
+    // emptied means a buffer was released twice
+    if (__atomic_load_n(&s->flag, __ATOMIC_ACQUIRE) != 0u)
+        return fail(MGV_E_CAPACITY, "mgv_pool_release_host: return ring full (buffer %d released twice?)", buffer);
+    __atomic_store_n(&s->index, buffer, __ATOMIC_RELAXED);
+    __atomic_store_n(&s->flag, 1u, __ATOMIC_RELEASE);
+    ++P->ret_head;
     return MGV_OK;
 }
 
@@ -345,7 +322,12 @@ int mgv_pool_stats(mgv_pool *P, uint32_t *available, uint32_t *producers, uint32
     if ((rc = hip_status(hipStreamSynchronize(s), "hipStreamSynchronize"))) return rc;
     if (available) *available = P->stats[0];
     if (producers) *producers = P->stats[1];
-    if (stacked) *stacked = P->stats[2];
+    if (stacked) {
+        // buffers on the stack plus those handed back through the return ring
+        // and not yet taken by the device
+        std::lock_guard<std::mutex> lk(*P->ret_mu);
+        *stacked = P->stats[2] + (uint32_t)((uint32_t)P->ret_head - P->stats[3]);
+    }
     return MGV_OK;
 }
 

@@ -41,9 +41,30 @@ struct LdsProbeArgs {
     const uint64_t *slots;
     uint32_t nbuckets;
     uint32_t slice;          // buckets per level-2 partition
-    PartLayout L;            // level-2 layout
+    PartLayout L;            // level-2 layout (counted partitioning)
+    uint32_t nparts;         // level-2 partitions
+    // fill partitioning: partition p holds rows [p * cap, p * cap + min(fill[p], cap))
+    const unsigned long long *fill;   // null: the counted layout L
+    uint64_t cap;
+    const int32_t *fcol[2];  // its hash and value columns
     uint32_t *next;          // partition counter (zeroed before the launch)
-    mgv_result *res;
+};
+
+// Count-free partitioning: F regions of `cap` rows each, filled by
+// reservations (one atomic per tile and partition) in any order, and an
+// overflow area for the rows of a region that is full.  With cap set a few
+// standard deviations above n / F, hash partitions of any input fit; a
+// skewed input's excess goes to the overflow area, so every input is
+// partitioned completely.
+struct FillLayout {
+    uint32_t F;
+    uint32_t pad;
+    uint64_t cap;                 // rows per region (multiple of 64)
+    int32_t *col[2];              // hashes, values: F regions, 64 rows of padding, the overflow area, 64 more
+    unsigned long long *fill;     // [F] rows reserved in each region (may pass cap)
+    unsigned long long *ofill;    // rows reserved in the overflow area
+    uint64_t ostart;              // first row of the overflow area: F * cap + 64
+    uint64_t ocap;                // its rows
 };
 
 // The probe rows reach the probe hashed: the first partitioning writes
@@ -83,11 +104,15 @@ __device__ __forceinline__ bool bucket_full(const v4u (&q)[4]) { return !(q[3].z
 // are exactly [0, c): slot s hits key iff s < c and its key equals key (an
 // empty slot never hits, whatever the key), and the bucket is full iff
 // c == 8.  A probe row then reads 32 bytes of keys and 1 byte of count, and a
-// payload only when it hits, instead of the 64-byte bucket.
-struct Slice {
-    v4u key[(kSliceMax + kSliceExtra) * 2];
-    uint32_t pay[(kSliceMax + kSliceExtra) * 8];
-    uint8_t cnt[kSliceMax + kSliceExtra];
+// payload only when it hits, instead of the 64-byte bucket.  NB is the slice
+// size in buckets (the probe's 2048, the group-by probe's 1024, which leaves
+// LDS for its group table).
+template <uint32_t NB>
+struct SliceT {
+    static constexpr uint32_t kBuckets = NB;
+    v4u key[(NB + kSliceExtra) * 2];
+    uint32_t pay[(NB + kSliceExtra) * 8];
+    uint8_t cnt[NB + kSliceExtra];
 };
 
 // LDS quad of key quad w (slots 4w..4w+3) of slice bucket idx.  Buckets' key
@@ -104,21 +129,109 @@ __device__ __forceinline__ uint32_t slice_hits(const v4u &k0, const v4u &k1, uin
     return m & ((1u << c) - 1u);
 }
 
-// Payload sum of the hit slots hm of slice bucket idx, given the payload of
-// the lowest hit slot (p1, read by the caller).  Duplicate build keys (two or
-// more hits in one bucket) read the other payloads here.
-__device__ __forceinline__ void slice_payloads(const Slice &S, uint32_t idx, uint32_t hm, int32_t p1, uint32_t &hits,
-                                               int64_t &psum) {
-    const uint32_t h = (uint32_t)__builtin_popcount(hm);
-    int64_t t = h ? (int64_t)p1 : 0;
-    if (ballot(h > 1)) {
-        for (uint32_t m = hm & (hm - 1); m; m &= m - 1) t += (int64_t)(int32_t)S.pay[idx * 8 + __builtin_ctz(m)];
-    }
-    psum += t;
-    hits += h;
-}
-
 __device__ __forceinline__ uint32_t lowest_slot(uint32_t hm) { return hm ? (uint32_t)__builtin_ctz(hm) : 0u; }
+
+// What a probe does with its matches.  AccSum: COUNT and SUM(val + payload)
+// in registers (the join aggregate, hashjoin.cu:135-182 with
+// aggregation.cu:27-66); AccGroup: GROUP BY payload, COUNT and SUM(val),
+// in a per-block LDS table of NG groups flushed with global atomics (groups
+// >= ngroups are dropped, as mgv_hashjoin_probe_groupby drops them).
+struct LdsAccArgs {
+    mgv_result *res;       // AccSum
+    uint64_t *gcnt;        // AccGroup
+    int64_t *gsum;
+    uint32_t ngroups;
+};
+
+struct AccSum {
+    struct Smem {};
+    uint64_t cnt = 0, sum = 0;   // sum: two's-complement int64
+    __device__ __forceinline__ void open(Smem *, const LdsAccArgs &) {}
+    // hit mask hm of LDS bucket idx (p1 = payload of its lowest hit)
+    template <class S>
+    __device__ __forceinline__ void lds(const S &sl, uint32_t idx, uint32_t hm, int32_t p1, int32_t val) {
+        const uint32_t h = (uint32_t)__builtin_popcount(hm);
+        int64_t t = h ? (int64_t)p1 : 0;
+        if (ballot(h > 1)) {
+            for (uint32_t m = hm & (hm - 1); m; m &= m - 1) t += (int64_t)(int32_t)sl.pay[idx * 8 + __builtin_ctz(m)];
+        }
+        if (h) {
+            cnt += h;
+            sum += (uint64_t)(t + (int64_t)h * (int64_t)val);
+        }
+    }
+    // a global-memory bucket of a row whose hash is h_key
+    __device__ __forceinline__ void bucket(const v4u (&q)[4], uint32_t h_key, int32_t val) {
+        uint32_t hits = 0;
+        int64_t psum = 0;
+        bucket_scan(q, h_key, hits, psum);
+        if (hits) {
+            cnt += hits;
+            sum += (uint64_t)(psum + (int64_t)hits * (int64_t)val);
+        }
+    }
+    // every wave of the block calls close
+    __device__ __forceinline__ void close(const LdsAccArgs &a) {
+        cnt = wave_sum_u64(cnt);
+        sum = wave_sum_u64(sum);
+        if (lane_id() == 0) {
+            if (cnt) atomic_add_global((unsigned long long *)&a.res->count, (unsigned long long)cnt);
+            if (sum) atomic_add_global((unsigned long long *)&a.res->sum, (unsigned long long)sum);
+        }
+    }
+};
+
+template <uint32_t NG>
+struct AccGroup {
+    struct Smem {
+        uint32_t cnt[NG];
+        unsigned long long sum[NG];
+    };
+    Smem *sm = nullptr;
+    uint32_t ngroups = 0;
+    __device__ __forceinline__ void add(uint32_t g, int32_t v) {
+        if (g < ngroups) {
+            atomic_add_lds(&sm->cnt[g], 1u);
+            atomic_add_lds(&sm->sum[g], (unsigned long long)(int64_t)v);
+        }
+    }
+    // the whole block calls open
+    __device__ __forceinline__ void open(Smem *s, const LdsAccArgs &a) {
+        sm = s;
+        ngroups = a.ngroups < NG ? a.ngroups : NG;
+        for (uint32_t g = threadIdx.x; g < ngroups; g += blockDim.x) {
+            sm->cnt[g] = 0;
+            sm->sum[g] = 0;
+        }
+        __syncthreads();
+    }
+    template <class S>
+    __device__ __forceinline__ void lds(const S &sl, uint32_t idx, uint32_t hm, int32_t p1, int32_t val) {
+        if (hm) add((uint32_t)p1, val);
+        if (ballot(__builtin_popcount(hm) > 1)) {
+            for (uint32_t m = hm & (hm - 1); m; m &= m - 1) add(sl.pay[idx * 8 + __builtin_ctz(m)], val);
+        }
+    }
+    __device__ __forceinline__ void bucket(const v4u (&q)[4], uint32_t h_key, int32_t val) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const uint32_t sk = (s & 1) ? q[s >> 1].z : q[s >> 1].x;
+            const uint32_t sp = (s & 1) ? q[s >> 1].w : q[s >> 1].y;
+            if (fmix32(sk) == h_key && !(sk == ~0u && sp == ~0u)) add(sp, val);
+        }
+    }
+    // the whole block calls close
+    __device__ __forceinline__ void close(const LdsAccArgs &a) {
+        __syncthreads();
+        for (uint32_t g = threadIdx.x; g < ngroups; g += blockDim.x) {
+            const uint32_t c = sm->cnt[g];
+            if (c) {
+                atomic_add_global((unsigned long long *)&a.gcnt[g], (unsigned long long)c);
+                atomic_add_global((unsigned long long *)&a.gsum[g], sm->sum[g]);
+            }
+        }
+    }
+};
 
 // Rows whose chain continues past their home bucket (about 5 % at load
 // factor 0.5, but some row of nearly every wave) are not followed on the
@@ -149,9 +262,10 @@ __device__ __forceinline__ uint32_t queue_push(WaveQueue &Q, uint32_t tail, bool
 // Probes up to 64 queued entries (one per lane) and re-queues the chains
 // that go on; returns the new head and, through tail, the new tail.  A
 // bucket inside the loaded range is read from LDS, any other from the table.
-__device__ __forceinline__ uint32_t queue_round(WaveQueue &Q, uint32_t head, uint32_t &tail, const Slice &S,
+template <class S, class Acc>
+__device__ __forceinline__ uint32_t queue_round(WaveQueue &Q, uint32_t head, uint32_t &tail, const S &sl,
                                                 const uint64_t *slots, uint32_t nb, uint32_t lo, uint32_t nload,
-                                                uint64_t &cnt, uint64_t &sum) {
+                                                Acc &acc) {
     const uint32_t avail = tail - head;
     const uint32_t take = avail < kWave ? avail : kWave;
     const uint32_t l = lane_id();
@@ -166,26 +280,24 @@ __device__ __forceinline__ uint32_t queue_round(WaveQueue &Q, uint32_t head, uin
     }
     wave_sync();   // every lane has read its entry before slots are reused
     const uint32_t idx = bkt - lo;
-    uint32_t hits = 0;
-    int64_t psum = 0;
+    const bool in_lds = mine && idx < nload;
     bool full = false;
-    if (mine && idx < nload) {
-        const v4u k0 = S.key[key_at(idx, 0)], k1 = S.key[key_at(idx, 1)];
-        const uint32_t c = S.cnt[idx];
-        const uint32_t hm = slice_hits(k0, k1, c, key);
-        slice_payloads(S, idx, hm, (int32_t)S.pay[idx * 8 + lowest_slot(hm)], hits, psum);
+    uint32_t hm = 0, ii = 0;
+    if (in_lds) {
+        const v4u k0 = sl.key[key_at(idx, 0)], k1 = sl.key[key_at(idx, 1)];
+        const uint32_t c = sl.cnt[idx];
+        hm = slice_hits(k0, k1, c, key);
+        ii = idx;
         full = c == 8;
-    } else if (mine) {
+    }
+    acc.lds(sl, ii, hm, (int32_t)sl.pay[ii * 8 + lowest_slot(hm)], val);
+    if (mine && !in_lds) {
         const v4u *g = reinterpret_cast<const v4u *>(slots + (uint64_t)bkt * 8);
         v4u q[4];
 #pragma unroll
         for (int w = 0; w < 4; ++w) q[w] = load_global(g + w);
-        bucket_scan(q, key, hits, psum);
+        acc.bucket(q, key, val);
         full = bucket_full(q);
-    }
-    if (mine && hits) {
-        cnt += hits;
-        sum += (uint64_t)(psum + (int64_t)hits * (int64_t)val);
     }
     // a chain ends at a bucket with an empty slot, or after visiting all nb
     // buckets once (when its next bucket is its home bucket again)
@@ -201,10 +313,10 @@ __device__ __forceinline__ uint32_t queue_round(WaveQueue &Q, uint32_t head, uin
 // payloads are read together; rows whose bucket is full continue at the next
 // bucket, and rows whose home bucket lies outside the slice start at their
 // home bucket, through the wave's queue.
-__device__ __forceinline__ void lds_probe_quad(const v4i &k, const v4i &v, uint32_t m, const Slice &S,
+template <class S, class Acc>
+__device__ __forceinline__ void lds_probe_quad(const v4i &k, const v4i &v, uint32_t m, const S &sl,
                                                const uint64_t *slots, uint32_t nb, uint32_t lo, uint32_t nload,
-                                               uint64_t &cnt, uint64_t &sum, WaveQueue &Q, uint32_t &head,
-                                               uint32_t &tail) {
+                                               Acc &acc, WaveQueue &Q, uint32_t &head, uint32_t &tail) {
     uint32_t b[4], ii[4];
     bool ins[4];
 #pragma unroll
@@ -218,32 +330,26 @@ __device__ __forceinline__ void lds_probe_quad(const v4i &k, const v4i &v, uint3
     uint32_t c[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        k0[j] = S.key[key_at(ii[j], 0)];
-        k1[j] = S.key[key_at(ii[j], 1)];
-        c[j] = S.cnt[ii[j]];
+        k0[j] = sl.key[key_at(ii[j], 0)];
+        k1[j] = sl.key[key_at(ii[j], 1)];
+        c[j] = sl.cnt[ii[j]];
     }
     uint32_t hm[4];
     int32_t p1[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) hm[j] = ins[j] ? slice_hits(k0[j], k1[j], c[j], (uint32_t)q_get(k, j)) : 0u;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) p1[j] = (int32_t)S.pay[ii[j] * 8 + lowest_slot(hm[j])];
+    for (int j = 0; j < 4; ++j) p1[j] = (int32_t)sl.pay[ii[j] * 8 + lowest_slot(hm[j])];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        uint32_t hits = 0;
-        int64_t psum = 0;
-        slice_payloads(S, ii[j], hm[j], p1[j], hits, psum);
-        if (hits) {
-            cnt += hits;
-            sum += (uint64_t)(psum + (int64_t)hits * (int64_t)q_get(v, j));
-        }
+        acc.lds(sl, ii[j], hm[j], p1[j], q_get(v, j));
         const bool live = (m >> j) & 1u;
         const uint32_t from = ins[j] ? ((b[j] + 1 == nb) ? 0 : b[j] + 1) : b[j];
         const bool go = live && (!ins[j] || (c[j] == 8 && from != b[j]));
         tail = queue_push(Q, tail, go, (uint32_t)q_get(k, j), q_get(v, j), from);
         // fewer than kWave entries were queued before this push and at most
         // kWave were added, so the queue never holds more than kQueue - 1
-        while (tail - head >= kWave) head = queue_round(Q, head, tail, S, slots, nb, lo, nload, cnt, sum);
+        while (tail - head >= kWave) head = queue_round(Q, head, tail, sl, slots, nb, lo, nload, acc);
     }
 }
 
@@ -260,9 +366,11 @@ __device__ __forceinline__ uint32_t load_rows4(const int32_t *K, const int32_t *
     return left >= 4 ? 0xfu : (1u << left) - 1u;
 }
 
-// Copies slice buckets [lo, lo + nload) of the table into S, one bucket per
-// thread: keys, payloads and the fill count (the first empty slot, or 8).
-__device__ __forceinline__ void load_slice(Slice &S, const uint64_t *slots, uint32_t lo, uint32_t nload) {
+// Copies slice buckets [lo, lo + nload) of the table into the slice, one
+// bucket per thread: keys, payloads and the fill count (the first empty
+// slot, or 8).
+template <class S>
+__device__ __forceinline__ void load_slice(S &sl, const uint64_t *slots, uint32_t lo, uint32_t nload) {
     for (uint32_t i = threadIdx.x; i < nload; i += kLdsBlock) {
         const v4u *g = reinterpret_cast<const v4u *>(slots + (uint64_t)(lo + i) * 8);
         const v4u q0 = load_global(g), q1 = load_global(g + 1), q2 = load_global(g + 2), q3 = load_global(g + 3);
@@ -275,37 +383,46 @@ __device__ __forceinline__ void load_slice(Slice &S, const uint64_t *slots, uint
             c = empty ? (uint32_t)s : c;
         }
         // hashed keys (an empty slot's value is never compared: it is >= c)
-        S.key[key_at(i, 0)] = v4u{fmix32(q0.x), fmix32(q0.z), fmix32(q1.x), fmix32(q1.z)};
-        S.key[key_at(i, 1)] = v4u{fmix32(q2.x), fmix32(q2.z), fmix32(q3.x), fmix32(q3.z)};
-        v4u *pay4 = reinterpret_cast<v4u *>(S.pay);
+        sl.key[key_at(i, 0)] = v4u{fmix32(q0.x), fmix32(q0.z), fmix32(q1.x), fmix32(q1.z)};
+        sl.key[key_at(i, 1)] = v4u{fmix32(q2.x), fmix32(q2.z), fmix32(q3.x), fmix32(q3.z)};
+        v4u *pay4 = reinterpret_cast<v4u *>(sl.pay);
         pay4[i * 2] = v4u{q0.y, q0.w, q1.y, q1.w};
         pay4[i * 2 + 1] = v4u{q2.y, q2.w, q3.y, q3.w};
-        S.cnt[i] = (uint8_t)c;
+        sl.cnt[i] = (uint8_t)c;
     }
 }
 
-__global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a) {
-    __shared__ __align__(16) Slice S;
+template <uint32_t NB, class Acc>
+__global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a, LdsAccArgs aa) {
+    __shared__ __align__(16) SliceT<NB> S;
     __shared__ WaveQueue s_queue[kLdsBlock / kWave];
+    __shared__ __align__(16) typename Acc::Smem s_acc;
     __shared__ uint32_t s_part;
     const uint32_t t = threadIdx.x;
     WaveQueue &Q = s_queue[t / kWave];
     const uint32_t nb = a.nbuckets;
-    const int32_t *K = a.L.col[0];
-    const int32_t *V = a.L.col[1];
-    uint64_t cnt = 0;
-    uint64_t sum = 0;   // two's-complement int64
+    const int32_t *K = a.fill ? a.fcol[0] : a.L.col[0];
+    const int32_t *V = a.fill ? a.fcol[1] : a.L.col[1];
+    Acc acc;
+    acc.open(&s_acc, aa);
     for (;;) {
         if (t == 0) s_part = atomicAdd(a.next, 1u);
         __syncthreads();
         const uint32_t p = s_part;
         __syncthreads();   // s_part is rewritten next round; the slice is free again
-        if (p >= a.L.F) break;
-        const uint64_t r0 = a.L.pstart[p];
-        const uint64_t r1 = r0 + a.L.ptotal[p];
+        if (p >= a.nparts) break;
+        uint64_t r0, r1;
+        if (a.fill) {
+            r0 = (uint64_t)p * a.cap;
+            const unsigned long long f = a.fill[p];
+            r1 = r0 + (f < a.cap ? f : a.cap);
+        } else {
+            r0 = a.L.pstart[p];
+            r1 = r0 + a.L.ptotal[p];
+        }
         if (r0 == r1) continue;
         const uint32_t lo = p * a.slice;
-        const uint32_t nload = min(a.slice + kSliceExtra, nb - lo);
+        const uint32_t nload = min(a.slice + kSliceExtra, nb - lo);   // a.slice <= NB
         // the first rows' loads overlap the slice copy
         v4i k[kAhead], v[kAhead];
         uint32_t m[kAhead];
@@ -334,7 +451,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a) {
             }
 #pragma unroll
             for (int x = 0; x < kAhead; ++x)
-                lds_probe_quad(k[x], v[x], m[x], S, a.slots, nb, lo, nload, cnt, sum, Q, head, tail);
+                lds_probe_quad(k[x], v[x], m[x], S, a.slots, nb, lo, nload, acc, Q, head, tail);
 #pragma unroll
             for (int x = 0; x < kAhead; ++x) {
                 k[x] = kn[x];
@@ -344,15 +461,316 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a) {
             i0 = i1;
         }
         // the rest of the queue, before the slice is replaced
-        while (tail != head) head = queue_round(Q, head, tail, S, a.slots, nb, lo, nload, cnt, sum);
+        while (tail != head) head = queue_round(Q, head, tail, S, a.slots, nb, lo, nload, acc);
         __syncthreads();
     }
-    cnt = wave_sum_u64(cnt);
-    sum = wave_sum_u64(sum);
-    if (lane_id() == 0) {
-        if (cnt) atomic_add_global((unsigned long long *)&a.res->count, (unsigned long long)cnt);
-        if (sum) atomic_add_global((unsigned long long *)&a.res->sum, (unsigned long long)sum);
+    acc.close(aa);
+}
+
+// ------------------------------------------------------------------ fill partitioning
+
+constexpr int kFillBlock = 1024;
+constexpr int kFillTile = 8192;
+
+// One tile of rows of a fill scatter's input: the rows of segment `seg`
+// starting at `r0` (below `r1`).
+struct FillTile {
+    const int32_t *k, *v, *pred;
+    uint64_t r0, r1;
+};
+
+// Scatters rows into a FillLayout by the top bits of their hash, tile by
+// tile: each tile is counting-sorted in LDS by partition, every partition
+// with rows in the tile reserves its run with one atomic on its fill count
+// (the part of a run past the region's capacity goes to the overflow area),
+// and the staged rows are written run by run.  Input: FROM_UP = false, n
+// rows of key (hashed here with fmix32), val and an optional filter pred <=
+// bound; FROM_UP = true, the regions and the overflow area of an upstream
+// FillLayout U (rows already hashed), whose partitions are coarser: a tile
+// then spans F / U.F partitions, so the runs are long.  The next tile's
+// columns load while the current one is sorted; the write step issues a
+// fixed number of stores per thread (lanes past the tile's last staged row
+// repeat its store; an empty tile stores into the padding after the
+// regions), so the compiler can wait for the loads without the stores.
+template <bool FROM_UP, uint32_t MAXF>
+__global__ __launch_bounds__(kFillBlock) void k_fill_scatter(FillLayout O, FillLayout U, const int32_t *key,
+                                                             const int32_t *val, const int32_t *pred, int32_t bound,
+                                                             uint64_t n) {
+    constexpr int BLOCK = kFillBlock, TILE = kFillTile;
+    constexpr int Q = TILE / (4 * BLOCK);
+    constexpr int R = TILE / BLOCK;
+    constexpr int PER = MAXF > (uint32_t)BLOCK ? (int)(MAXF / BLOCK) : 1;
+    static_assert(Q >= 1 && TILE % (4 * BLOCK) == 0, "fill scatter shape");
+    __shared__ unsigned long long dbase[MAXF];   // slot i of partition p -> region row dbase[p] + i
+    __shared__ unsigned long long obase[MAXF];   // slot i past rlim[p] -> overflow row obase[p] + i
+    __shared__ uint32_t rlim[MAXF];              // slots below rlim[p] fit the region
+    __shared__ uint32_t tcnt[MAXF];
+    __shared__ uint16_t toff[MAXF];
+    __shared__ uint32_t wtot[BLOCK / kWave];
+    __shared__ uint32_t s_total;
+    __shared__ uint32_t s_tiles[BLOCK + 1];      // FROM_UP: cumulative tiles per upstream segment
+    __shared__ int32_t sk[TILE];
+    __shared__ int32_t sv[TILE];
+    const uint32_t F = O.F;
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (F + BLOCK - 1) / BLOCK;
+    for (uint32_t p = t; p < F; p += BLOCK) tcnt[p] = 0;
+    // the tiles of the input
+    uint32_t ntiles;
+    if (FROM_UP) {
+        const uint32_t nseg = U.F + 1;   // regions, then the overflow area (nseg <= BLOCK)
+        uint64_t len = 0;
+        if (t < U.F) {
+            const unsigned long long f = U.fill[t];
+            len = f < U.cap ? f : U.cap;
+        } else if (t == U.F) {
+            const unsigned long long f = *U.ofill;
+            len = f < U.ocap ? f : U.ocap;
+        }
+        const uint32_t nt = t < nseg ? (uint32_t)((len + TILE - 1) / TILE) : 0u;
+        const uint32_t before = scan_block_excl<BLOCK>(nt, wtot);
+        if (t < nseg) s_tiles[t + 1] = before + nt;
+        if (t == 0) s_tiles[0] = 0;
+        __syncthreads();
+        ntiles = s_tiles[nseg];
+    } else {
+        ntiles = (uint32_t)((n + TILE - 1) / TILE);
     }
+    auto tile_of = [&](uint32_t ti) {
+        FillTile ft;
+        if (FROM_UP) {
+            // the segment s with s_tiles[s] <= ti < s_tiles[s + 1]
+            uint32_t a = 0, b = U.F;
+            while (a < b) {
+                const uint32_t mid = (a + b + 1) / 2;
+                if (s_tiles[mid] <= ti) a = mid; else b = mid - 1;
+            }
+            const uint64_t off = (uint64_t)(ti - s_tiles[a]) * TILE;
+            if (a < U.F) {
+                const unsigned long long f = U.fill[a];
+                ft.k = U.col[0] + (uint64_t)a * U.cap;
+                ft.v = U.col[1] + (uint64_t)a * U.cap;
+                ft.r1 = f < U.cap ? f : U.cap;
+            } else {
+                const unsigned long long f = *U.ofill;
+                ft.k = U.col[0] + U.ostart;
+                ft.v = U.col[1] + U.ostart;
+                ft.r1 = f < U.ocap ? f : U.ocap;
+            }
+            ft.pred = nullptr;
+            ft.r0 = off;
+        } else {
+            ft.k = key;
+            ft.v = val;
+            ft.pred = pred;
+            ft.r0 = (uint64_t)ti * TILE;
+            ft.r1 = n;
+        }
+        if (ft.r1 > ft.r0 + TILE) ft.r1 = ft.r0 + TILE;
+        return ft;
+    };
+    v4i nk[Q], nv[Q], nc[Q];
+    uint32_t nb[Q];
+    auto load_tile = [&](const FillTile &ft) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const uint64_t i0 = ft.r0 + 4ull * t + (uint64_t)q * 4 * BLOCK;
+            uint32_t m = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) m |= (uint32_t)(i0 + j < ft.r1) << j;
+            nb[q] = m;
+            nk[q] = load_quad(ft.k, i0, ft.r1);
+            nv[q] = ft.v ? load_quad(ft.v, i0, ft.r1) : v4i{0, 0, 0, 0};
+            nc[q] = ft.pred ? load_quad(ft.pred, i0, ft.r1) : v4i{0, 0, 0, 0};
+        }
+    };
+    uint32_t ti = blockIdx.x;
+    bool has_pred = false;
+    if (ti < ntiles) {
+        const FillTile ft = tile_of(ti);
+        has_pred = ft.pred != nullptr;
+        load_tile(ft);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        asm volatile("" ::"v"(nk[q].x), "v"(nk[q].y), "v"(nk[q].z), "v"(nk[q].w), "v"(nv[q].x), "v"(nv[q].y),
+                     "v"(nv[q].z), "v"(nv[q].w), "v"(nc[q].x), "v"(nc[q].y), "v"(nc[q].z), "v"(nc[q].w));
+    __syncthreads();
+    const uint64_t dummy = (uint64_t)F * O.cap + 32;   // padding row: stores of an empty tile
+    for (; ti < ntiles; ti += gridDim.x) {
+        // this tile's rows as hashes, with their masks
+        v4i ch[Q], cv[Q];
+        uint32_t cm[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            uint32_t m = nb[q];
+            if (has_pred) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) m &= ~((uint32_t)(q_get(nc[q], j) > bound) << j);
+            }
+            cm[q] = m;
+            cv[q] = nv[q];
+            if (FROM_UP) {
+                ch[q] = nk[q];
+            } else {
+                ch[q] = v4i{(int32_t)fmix32((uint32_t)nk[q].x), (int32_t)fmix32((uint32_t)nk[q].y),
+                            (int32_t)fmix32((uint32_t)nk[q].z), (int32_t)fmix32((uint32_t)nk[q].w)};
+            }
+        }
+        if (ti + gridDim.x < ntiles) load_tile(tile_of(ti + gridDim.x));
+        // 1. rank every row within its partition
+        uint32_t rank[Q][4];
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bool ok = (cm[q] >> j) & 1u;
+                rank[q][j] = ok ? atomicAdd(&tcnt[part_of_hash(q_get(ch[q], j), F)], 1u) : 0u;
+            }
+        __syncthreads();
+        // 2. owners: staging offsets, run reservations, overflow
+        uint32_t c[PER];
+        uint32_t mine = 0;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const uint32_t p = t * per + e;
+            c[e] = (e < (int)per && p < F) ? tcnt[p] : 0u;
+            mine += c[e];
+        }
+        uint32_t start = scan_block_excl<BLOCK>(mine, wtot);
+        if (t == BLOCK - 1) s_total = start + mine;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const uint32_t p = t * per + e;
+            if (e < (int)per && p < F) {
+                toff[p] = (uint16_t)start;
+                if (c[e]) {
+                    const unsigned long long off = atomic_add_global(&O.fill[p], (unsigned long long)c[e]);
+                    const uint64_t room = off < O.cap ? O.cap - off : 0;
+                    const uint32_t fit = room < c[e] ? (uint32_t)room : c[e];
+                    dbase[p] = (uint64_t)p * O.cap + off - start;
+                    rlim[p] = start + fit;
+                    if (fit < c[e]) {
+                        const unsigned long long o =
+                            atomic_add_global(O.ofill, (unsigned long long)(c[e] - fit));
+                        obase[p] = O.ostart + o - (start + fit);
+                    }
+                }
+                tcnt[p] = 0;
+                start += c[e];
+            }
+        }
+        __syncthreads();
+        // 3. stage rows in partition order
+        uint32_t pos[Q][4];
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bool ok = (cm[q] >> j) & 1u;
+                pos[q][j] = toff[ok ? part_of_hash(q_get(ch[q], j), F) : 0u] + rank[q][j];
+            }
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if ((cm[q] >> j) & 1u) {
+                    sk[pos[q][j]] = q_get(ch[q], j);
+                    sv[pos[q][j]] = q_get(cv[q], j);
+                }
+        __syncthreads();
+        // 4. write the runs: exactly R stores per column and thread
+        const uint32_t total = s_total;
+        constexpr int G = R < 8 ? R : 8;
+#pragma unroll
+        for (int r0 = 0; r0 < R; r0 += G) {
+            int32_t k[G], v[G];
+            uint32_t ii[G];
+#pragma unroll
+            for (int r = 0; r < G; ++r) {
+                ii[r] = total ? min(t + (uint32_t)(r0 + r) * BLOCK, total - 1) : 0u;
+                k[r] = sk[ii[r]];
+                v[r] = sv[ii[r]];
+            }
+            uint64_t dst[G];
+#pragma unroll
+            for (int r = 0; r < G; ++r) {
+                const uint32_t p = total ? part_of_hash(k[r], F) : 0u;
+                dst[r] = !total ? dummy : (ii[r] < rlim[p] ? dbase[p] : obase[p]) + ii[r];
+            }
+#pragma unroll
+            for (int r = 0; r < G; ++r) {
+                O.col[0][dst[r]] = k[r];
+                O.col[1][dst[r]] = v[r];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Probes hashed rows against the global table (the overflow area of a fill
+// partitioning, rows whose partition was full): one row per lane, its chain
+// followed bucket by bucket as in mgv_hashjoin_probe.
+template <class Acc>
+__global__ __launch_bounds__(256) void k_probe_hashed(const uint64_t *slots, uint32_t nb, const int32_t *H,
+                                                      const int32_t *V, const unsigned long long *nrows, uint64_t ncap,
+                                                      LdsAccArgs aa) {
+    __shared__ __align__(16) typename Acc::Smem s_acc;
+    Acc acc;
+    acc.open(&s_acc, aa);
+    const unsigned long long f = *nrows;
+    const uint64_t n = f < ncap ? f : ncap;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * 256; i0 < n; i0 += (uint64_t)gridDim.x * 256) {
+        const uint64_t i = i0 + threadIdx.x;
+        if (i < n) {
+            const uint32_t h = (uint32_t)H[i];
+            const int32_t v = V[i];
+            const uint32_t home = bucket_of_hash(h, nb);
+            uint32_t b = home;
+            for (;;) {
+                const v4u *g = reinterpret_cast<const v4u *>(slots + (uint64_t)b * 8);
+                v4u q[4];
+#pragma unroll
+                for (int w = 0; w < 4; ++w) q[w] = load_global(g + w);
+                acc.bucket(q, h, v);
+                if (!bucket_full(q)) break;
+                b = (b + 1 == nb) ? 0 : b + 1;
+                if (b == home) break;
+            }
+        }
+    }
+    acc.close(aa);
+}
+
+// Rows per region of a fill partitioning of up to n rows into F regions:
+// n / F plus eight standard deviations of a uniform hash partition's size,
+// plus 256, in multiples of 64 rows.
+static uint64_t fill_cap(uint64_t n, uint32_t F) {
+    const double lam = (double)n / F;
+    const uint64_t c = (uint64_t)(lam + 8.0 * __builtin_sqrt(lam)) + 256;
+    return (c + 63) / 64 * 64;
+}
+
+// Lays out a fill partitioning of up to n rows (regions, padding, overflow
+// area, padding; two columns) at ws + off; the counters fill (F words) and
+// ofill are placed by the caller.  Returns the end offset.
+static uint64_t fill_plan(uint64_t n, uint32_t F, char *ws, uint64_t off, unsigned long long *fill,
+                          unsigned long long *ofill, FillLayout *L) {
+    const uint64_t cap = fill_cap(n, F);
+    const uint64_t rows = (uint64_t)F * cap + 64 + n + 64;
+    const uint64_t cbytes = (rows * 4 + 255) / 256 * 256;
+    if (L) {
+        L->F = F;
+        L->pad = 0;
+        L->cap = cap;
+        L->col[0] = (int32_t *)(ws + off);
+        L->col[1] = (int32_t *)(ws + off + cbytes);
+        L->fill = fill;
+        L->ofill = ofill;
+        L->ostart = (uint64_t)F * cap + 64;
+        L->ocap = n;
+    }
+    return off + 2 * cbytes;
 }
 
 // ------------------------------------------------------------------ build
@@ -467,22 +885,22 @@ __global__ __launch_bounds__(256) void k_lds_build_overflow(LdsBuildArgs a) {
 }
 
 // Level-2 fan-out for a table of nb buckets: the smallest power of two
-// F2 >= 2 * F1 that divides nb into slices of at most kSliceMax buckets.
-// When F1 alone already cuts the table into such slices (F1 > kMaxFanout
-// asks for it), there is one level only: F2 = F1.
-static int lds_fanout(uint32_t nb, uint32_t F1, uint32_t *F2) {
-    if (F1 > kMaxFanout) {
-        if (nb % F1 != 0 || nb / F1 > kSliceMax || nb / F1 == 0)
+// F2 >= 2 * F1 that divides nb into slices of at most smax buckets.  When F1
+// alone already cuts the table into such slices (or F1 > kMaxFanout asks for
+// one level), there is one level only: F2 = F1.
+static int lds_fanout(uint32_t nb, uint32_t F1, uint32_t *F2, uint32_t smax = kSliceMax) {
+    if (F1 > kMaxFanout || (nb % F1 == 0 && nb / F1 <= smax && nb / F1 > 0)) {
+        if (nb % F1 != 0 || nb / F1 > smax || nb / F1 == 0)
             return fail(MGV_E_INVALID, "LDS probe: one level of %u partitions does not cut %u buckets into slices "
-                        "of <= %u buckets", F1, nb, kSliceMax);
+                        "of <= %u buckets", F1, nb, smax);
         *F2 = F1;
         return MGV_OK;
     }
     uint32_t f = 2 * F1;
-    while (f <= kMaxFanoutWide && (nb % f != 0 || nb / f > kSliceMax)) f <<= 1;
-    if (f > kMaxFanoutWide || nb % f != 0 || nb / f > kSliceMax || nb / f == 0)
+    while (f <= kMaxFanoutWide && (nb % f != 0 || nb / f > smax)) f <<= 1;
+    if (f > kMaxFanoutWide || nb % f != 0 || nb / f > smax || nb / f == 0)
         return fail(MGV_E_INVALID, "LDS probe: %u buckets do not split into <= %u slices of <= %u buckets",
-                    nb, kMaxFanoutWide, kSliceMax);
+                    nb, kMaxFanoutWide, smax);
     *F2 = f;
     return MGV_OK;
 }
@@ -508,6 +926,30 @@ static int lds_plan(uint64_t n, uint32_t nb, uint32_t F1, void *ws, uint64_t ws_
     return plan_layout(n, *F2, 2, (char *)ws + o2, b2, L2, nullptr, kPartAlign);
 }
 
+// Workspace of the fill-partitioned probe: 256 bytes of counters (the
+// partition counter of the probe, the two overflow counts), the fill counts
+// of both levels, then the level-1 layout (two levels only) and the level-2
+// layout.
+static int fill_probe_plan(uint64_t n, uint32_t nb, uint32_t F1, void *ws, uint64_t ws_bytes, FillLayout *L1,
+                           FillLayout *L2, uint32_t *F2, uint64_t *need, uint32_t smax = kSliceMax) {
+    if (F1 < 2 || F1 > kMaxFanoutWide || (F1 & (F1 - 1)))
+        return fail(MGV_E_INVALID, "LDS probe: level-1 fan-out %u not a power of two in [2,%u]", F1, kMaxFanoutWide);
+    int rc = lds_fanout(nb, F1, F2, smax);
+    if (rc) return rc;
+    const bool one = *F2 == F1;
+    char *b = (char *)ws;
+    unsigned long long *ctr = (unsigned long long *)ws;
+    const uint64_t counters = (256 + (uint64_t)(F1 + *F2) * 8 + 255) / 256 * 256;
+    uint64_t off = counters;
+    if (!one) off = fill_plan(n, F1, b, off, ws ? ctr + 32 : nullptr, ws ? ctr + 1 : nullptr, ws ? L1 : nullptr);
+    off = fill_plan(n, *F2, b, off, ws ? ctr + 32 + F1 : nullptr, ws ? ctr + 2 : nullptr, ws ? L2 : nullptr);
+    if (need) *need = off;
+    if (!ws) return MGV_OK;
+    if (ws_bytes < off) return fail(MGV_E_CAPACITY, "LDS probe: workspace too small");
+    if (one) L1->F = 0;
+    return MGV_OK;
+}
+
 // Build workspace: the probe's (counters + both partitionings of n rows)
 // followed by the overflow list (n keys, n payloads).
 static int build_plan(uint64_t n, uint32_t nb, uint32_t F1, void *ws, uint64_t ws_bytes, PartLayout *L1,
@@ -526,6 +968,77 @@ static int build_plan(uint64_t n, uint32_t nb, uint32_t F1, void *ws, uint64_t w
     return MGV_OK;
 }
 
+// Workspace of either partitioning (a caller's workspace serves both).
+static int probe_ws_need(uint64_t n, uint32_t nb, uint32_t F1, uint64_t *need) {
+    uint32_t F2 = 0;
+    uint64_t a = 0, b = 0;
+    int rc = lds_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, &F2, &a);
+    if (rc) return rc;
+    if ((rc = fill_probe_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, &F2, &b))) return rc;
+    *need = a > b ? a : b;
+    return MGV_OK;
+}
+
+// The fill-partitioned probe: no count passes; level 1 by fmix32(key) into
+// fanout regions, level 2 by the same hash bits into the slices' regions,
+// then the LDS probe per slice (slices of at most NB buckets) and the hashed
+// probe of the overflow rows, both with accumulator Acc.
+template <uint32_t NB, class Acc>
+static int probe_lds_fill(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
+                          const int32_t *keys, const int32_t *vals, uint64_t n, uint32_t fanout, void *ws,
+                          uint64_t ws_bytes, const LdsAccArgs &aa, mgv_launch LC, hipStream_t s) {
+    for (const int32_t *c : {pred, keys, vals})
+        if (c && !aligned16(c)) return fail(MGV_E_ALIGN, "probe_lds: column is not 16-byte aligned");
+    FillLayout L1 = {}, L2 = {};
+    uint32_t F2 = 0;
+    uint64_t need = 0;
+    int rc = fill_probe_plan(n, nbuckets, fanout, nullptr, 0, nullptr, nullptr, &F2, &need, NB);
+    if (rc) return rc;
+    if (!ws || ws_bytes < need) return fail(MGV_E_CAPACITY, "LDS probe: workspace too small");
+    if ((rc = fill_probe_plan(n, nbuckets, fanout, ws, ws_bytes, &L1, &L2, &F2, nullptr, NB))) return rc;
+    const uint64_t counters = (256 + (uint64_t)(fanout + F2) * 8 + 255) / 256 * 256;
+    if ((rc = mgv_fill_u64((uint64_t *)ws, counters / 8, 0, (mgv_stream_t)s))) return rc;
+    DeviceState *ds = current_state();
+    const uint32_t cus = (ds && ds->cus > 0) ? (uint32_t)ds->cus : 256u;
+    const uint64_t tiles = (n + kFillTile - 1) / kFillTile;
+    const uint32_t grid1 = (uint32_t)(tiles < 2ull * cus ? tiles : 2ull * cus);
+    if (L1.F) {
+        hipLaunchKernelGGL((k_fill_scatter<false, kMaxFanout>), dim3(grid1), dim3(kFillBlock), 0, s, L1, L1, keys,
+                           vals, pred, bound, n);
+        if ((rc = check_launch("k_fill_scatter (level 1)"))) return rc;
+        hipLaunchKernelGGL((k_fill_scatter<true, kMaxFanoutWide>), dim3(grid1), dim3(kFillBlock), 0, s, L2, L1,
+                           nullptr, nullptr, nullptr, 0, 0ull);
+        if ((rc = check_launch("k_fill_scatter (level 2)"))) return rc;
+    } else {
+        hipLaunchKernelGGL((k_fill_scatter<false, kMaxFanoutWide>), dim3(grid1), dim3(kFillBlock), 0, s, L2, L2,
+                           keys, vals, pred, bound, n);
+        if ((rc = check_launch("k_fill_scatter"))) return rc;
+    }
+    static thread_local int cached = 0;
+    if (!cached) cached = persistent_blocks(k_lds_probe<NB, Acc>, kLdsBlock);
+    const uint32_t grid = LC.grid ? LC.grid : (uint32_t)cached;
+    LdsProbeArgs a = {};
+    a.slots = slots;
+    a.nbuckets = nbuckets;
+    a.slice = nbuckets / F2;
+    a.nparts = F2;
+    a.fill = L2.fill;
+    a.cap = L2.cap;
+    a.fcol[0] = L2.col[0];
+    a.fcol[1] = L2.col[1];
+    a.next = (uint32_t *)ws;
+    hipLaunchKernelGGL((k_lds_probe<NB, Acc>), dim3(grid), dim3(kLdsBlock), 0, s, a, aa);
+    if ((rc = check_launch("k_lds_probe"))) return rc;
+    // rows whose partition was full (none for a uniform hash partition)
+    hipLaunchKernelGGL(k_probe_hashed<Acc>, dim3(cus), dim3(256), 0, s, slots, nbuckets, L2.col[0] + L2.ostart,
+                       L2.col[1] + L2.ostart, L2.ofill, L2.ocap, aa);
+    return check_launch("k_probe_hashed");
+}
+
+// Slices of the group-by probe: its LDS group table needs half the slice.
+constexpr uint32_t kGroupSlice = 1024;
+constexpr uint32_t kLdsMaxGroups = 4096;
+
 // The probe of mgv_hashjoin_probe_lds on workspace ws (both entry points).
 static int probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
                      const int32_t *keys, const int32_t *vals, uint64_t n, uint32_t fanout, void *ws,
@@ -533,6 +1046,11 @@ static int probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pr
     if (!slots || !keys || !res || nbuckets == 0) return fail(MGV_E_INVALID, "probe_lds: null argument");
     if (n == 0) return MGV_OK;
     if (n >= (1ull << 31)) return fail(MGV_E_INVALID, "probe_lds: n must be < 2^31");
+    if (!(LC.reserved & MGV_LAUNCH_COUNTED_PART)) {
+        LdsAccArgs aa = {res, nullptr, nullptr, 0};
+        return probe_lds_fill<kSliceMax, AccSum>(slots, nbuckets, pred, bound, keys, vals, n, fanout, ws, ws_bytes,
+                                                 aa, LC, stream_of(st));
+    }
     PartLayout L1, L2;
     uint32_t F2 = 0;
     int rc = lds_plan(n, nbuckets, fanout, ws, ws_bytes, &L1, &L2, &F2, nullptr);
@@ -551,10 +1069,17 @@ static int probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pr
     uint32_t *next = (uint32_t *)ws;
     if ((rc = mgv_fill_u32(next, 1, 0, (mgv_stream_t)s))) return rc;
     static thread_local int cached = 0;
-    if (!cached) cached = persistent_blocks(k_lds_probe, kLdsBlock);
+    if (!cached) cached = persistent_blocks(k_lds_probe<kSliceMax, AccSum>, kLdsBlock);
     const uint32_t grid = LC.grid ? LC.grid : (uint32_t)cached;
-    LdsProbeArgs a = {slots, nbuckets, nbuckets / F2, L2, next, res};
-    hipLaunchKernelGGL(k_lds_probe, dim3(grid), dim3(kLdsBlock), 0, s, a);
+    LdsProbeArgs a = {};
+    a.slots = slots;
+    a.nbuckets = nbuckets;
+    a.slice = nbuckets / F2;
+    a.L = L2;
+    a.nparts = L2.F;
+    a.next = next;
+    LdsAccArgs aa = {res, nullptr, nullptr, 0};
+    hipLaunchKernelGGL((k_lds_probe<kSliceMax, AccSum>), dim3(grid), dim3(kLdsBlock), 0, s, a, aa);
     return check_launch("k_lds_probe");
 }
 
@@ -583,8 +1108,7 @@ extern "C" {
 
 int mgv_probe_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes) {
     if (!bytes || nbuckets == 0) return fail(MGV_E_INVALID, "mgv_probe_lds_workspace_size: bad argument");
-    uint32_t F2 = 0;
-    return lds_plan(n, nbuckets, fanout, nullptr, 0, nullptr, nullptr, &F2, bytes);
+    return probe_ws_need(n, nbuckets, fanout, bytes);
 }
 
 int mgv_hashjoin_probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
@@ -602,15 +1126,35 @@ int mgv_hashjoin_probe_lds_noalloc(const uint64_t *slots, uint32_t nbuckets, con
     if (!ds->probe_scratch) return fail(MGV_E_NOT_INIT, "probe_lds_noalloc: scratch array not resolved");
     if (!slots || !keys || !res || nbuckets == 0) return fail(MGV_E_INVALID, "probe_lds_noalloc: null argument");
     if (n == 0) return MGV_OK;
-    uint32_t F2 = 0;
     uint64_t need = 0;
-    int rc = lds_plan(n, nbuckets, fanout, nullptr, 0, nullptr, nullptr, &F2, &need);
+    int rc = probe_ws_need(n, nbuckets, fanout, &need);
     if (rc) return rc;
     if (need > kProbeScratchBytes)
         return fail(MGV_E_CAPACITY, "probe_lds_noalloc: %llu rows need %llu bytes of scratch, the array holds %llu",
                     (unsigned long long)n, (unsigned long long)need, (unsigned long long)kProbeScratchBytes);
     return probe_lds(slots, nbuckets, pred, bound, keys, vals, n, fanout, ds->probe_scratch, kProbeScratchBytes,
                      res, LC, st);
+}
+
+int mgv_probe_groupby_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes) {
+    if (!bytes || nbuckets == 0) return fail(MGV_E_INVALID, "mgv_probe_groupby_lds_workspace_size: bad argument");
+    uint32_t F2 = 0;
+    return fill_probe_plan(n, nbuckets, fanout, nullptr, 0, nullptr, nullptr, &F2, bytes, kGroupSlice);
+}
+
+int mgv_hashjoin_probe_groupby_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pred, int32_t bound,
+                                   const int32_t *keys, const int32_t *vals, uint64_t n, uint64_t *gcnt, int64_t *gsum,
+                                   uint32_t ngroups, uint32_t fanout, void *ws, uint64_t ws_bytes, mgv_launch LC,
+                                   mgv_stream_t st) {
+    if (!current_state()) return fail(MGV_E_NOT_INIT, "mgv_init was not called");
+    if (!slots || !keys || !vals || !gcnt || !gsum || nbuckets == 0 || ngroups == 0)
+        return fail(MGV_E_INVALID, "probe_groupby_lds: bad argument");
+    if (ngroups > kLdsMaxGroups) return fail(MGV_E_INVALID, "probe_groupby_lds: more than %u groups", kLdsMaxGroups);
+    if (n == 0) return MGV_OK;
+    if (n >= (1ull << 31)) return fail(MGV_E_INVALID, "probe_groupby_lds: n must be < 2^31");
+    LdsAccArgs aa = {nullptr, gcnt, gsum, ngroups};
+    return probe_lds_fill<kGroupSlice, AccGroup<kLdsMaxGroups>>(slots, nbuckets, pred, bound, keys, vals, n, fanout,
+                                                                ws, ws_bytes, aa, LC, stream_of(st));
 }
 
 int mgv_build_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes) {

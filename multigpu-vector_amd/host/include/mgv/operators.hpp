@@ -42,8 +42,10 @@
 #include <condition_variable>
 #include <deque>
 #include <functional>
+#include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <variant>
 #include <vector>
@@ -120,23 +122,38 @@ public:
     d_operator_t *parent;
     std::shared_ptr<mgv_table_state> table;
     d_operator_t *builder;
-    // log_table_size is accepted for signature compatibility: the table is
-    // sized for max_size keys at load factor 0.5.
+    // The table holds max_size keys at load factor 0.5 and has at least
+    // 2^log_table_size slots (the reference's 2^log_table_size chain heads,
+    // hashjoin.cu:63-71); log_table_size must be in [0, 31].
     hashjoin(d_operator_t *parent_, int32_t log_table_size, int32_t max_size, const launch_conf &conf);
     ~hashjoin();   // destroys the builder
 };
 
+// Hands a chain's results to its host parent (gpu_to_cpu.cu:7-184).  Rows
+// (select -> gpu_to_cpu) stream through a small pool of this operator's own
+// output buffers in pinned host memory: kernels append them to the current
+// pool buffer, swap in a fresh one when it is full, and publish each full
+// buffer on a ring; the catcher thread copies it into a pinned host buffer
+// of the buffer manager, hands the pool buffer straight back (no kernel, no
+// copy engine), and passes the manager's buffer to the parent, which owns it
+// from then on.  A scalar result (aggregation / join count) arrives on the
+// same ring.
 template <size_t warp_size = WARPSIZE, size_t size = 64, typename T = buffer_t *>
 class gpu_to_cpu {
 public:
     h_operator_t *parent;
     int device;
-    mgv_channel *channel = nullptr;
+    static constexpr uint32_t kOutBuffers = 8, kOutRows = 1u << 20;
+    mgv_pool *out_pool = nullptr;           // kOutBuffers pinned host buffers of kOutRows rows
+    mgv_ring *ring = nullptr;
+    mgv_stream_out *rows_out = nullptr;
     std::thread catcher;
-    std::atomic<bool> eof{false};
-    mgv_result last{};
-    std::vector<int32_t> rows;          // materialised rows (select -> gpu_to_cpu)
-    std::vector<buffer_t> out;          // host buffers handed to the parent
+    std::atomic<bool> stop{false};          // ends a catcher whose chain failed
+    mgv_result last{};                      // count = rows handed over, or the scalar result
+    uint64_t buffers = 0, dropped = 0;      // of the last rows stream
+    std::string failure;                    // set by the catcher
+    std::vector<int32_t> rows;              // scalar result row
+    std::vector<buffer_t> out;              // scalar buffer handed to the parent
 
     gpu_to_cpu(h_operator_t *parent_, int device_);
     ~gpu_to_cpu();
@@ -213,8 +230,6 @@ private:
     std::shared_ptr<mgv_table_state> table;
     gpu_to_cpu_t *sink = nullptr;
     mgv_result *res = nullptr;           // device
-    int32_t *out = nullptr;              // device, select materialisation
-    uint64_t out_cap = 0, out_upper = 0;
     mgv_stream_t last_stream = nullptr;
     bool opened = false;
     // a union_all further up the chain: this chain shares its result slot
@@ -226,12 +241,21 @@ private:
     uint64_t *cursors = nullptr;         // device, [dest]
     uint64_t stage_cap = 0, stage_upper = 0;
     int stage_cols = 0;
+    // join vectors large enough for the partitioned LDS paths: their
+    // workspace, one per stream the chain is fed on
+    static constexpr uint64_t kLdsRows = 1ull << 20, kLdsBuildRows = 1ull << 21;
+    static constexpr uint64_t kLdsTableBytes = 64ull << 20;
+    static constexpr uint32_t kLdsFanout = 32;
+    std::mutex ws_mu;
+    std::map<mgv_stream_t, std::pair<void *, uint64_t>> ws;
 
     void resolve();
     void do_open();
     void do_close();
     void flush_split(mgv_stream_t s);
     uint32_t split_dests() const;
+    void *workspace(mgv_stream_t s, uint64_t bytes);
+    bool lds_join(uint64_t n, bool build, uint64_t *need) const;
 };
 
 // ------------------------------------------------------------ host operators

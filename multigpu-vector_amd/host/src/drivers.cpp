@@ -132,6 +132,18 @@ int mgv_ops_shutdown(void) {
     return guarded("mgv_ops_shutdown", [&] { buffer_manager<int32_t>::destroy(); });
 }
 
+int mgv_ops_pool_stats(uint32_t *device_free, uint32_t *device_cached, uint32_t *device_total, uint32_t *host_free) {
+    return guarded("mgv_ops_pool_stats", [&] {
+        require_manager();
+        uint32_t avail = 0, prod = 0, stacked = 0;
+        buffer_manager<int32_t>::device_pool()->stats(&avail, &prod, &stacked);
+        if (device_free) *device_free = stacked;
+        if (device_cached) *device_cached = buffer_manager<int32_t>::device_cached();
+        if (device_total) *device_total = (uint32_t)buffer_manager<int32_t>::device_pool()->size();
+        if (host_free) *host_free = buffer_manager<int32_t>::host_available();
+    });
+}
+
 int mgv_ops_sum_select(const int32_t *pred, const int32_t *val, uint64_t n, int32_t thres, uint32_t vector_rows,
                        int device, uint32_t grid, uint32_t block, int32_t *dst, int64_t *sum) {
     return guarded("mgv_ops_sum_select", [&] {

@@ -13,10 +13,17 @@
 template <>
 hashjoin<WARPSIZE>::hashjoin(d_operator_t *parent_, int32_t log_table_size, int32_t max_size, const launch_conf &conf)
     : parent(parent_), table(std::make_shared<mgv_table_state>()) {
-    (void)log_table_size;
     if (max_size < 0) throw mgv::error(MGV_E_INVALID, "hashjoin: negative max_size");
+    if (log_table_size < 0 || log_table_size > 31) throw mgv::error(MGV_E_INVALID, "hashjoin: log_table_size in [0, 31]");
     uint64_t bytes = 0;
     gpu(mgv_ht_plan((uint64_t)max_size, 0.5, &table->nbuckets, &bytes));
+    // at least 2^log_table_size slots
+    const uint64_t min_buckets = ((1ull << log_table_size) + MGV_SLOTS_PER_BUCKET - 1) / MGV_SLOTS_PER_BUCKET;
+    if (min_buckets > table->nbuckets) {
+        if (min_buckets > 0xffffffffull) throw mgv::error(MGV_E_CAPACITY, "hashjoin: table too large");
+        table->nbuckets = (uint32_t)min_buckets;
+        bytes = min_buckets * MGV_SLOTS_PER_BUCKET * sizeof(uint64_t);
+    }
     void *p = nullptr;
     gpu(mgv_malloc(&p, bytes));
     table->slots = (uint64_t *)p;
@@ -35,14 +42,27 @@ hashjoin<WARPSIZE>::~hashjoin() {
 
 template <>
 gpu_to_cpu<WARPSIZE, 64, buffer_t *>::gpu_to_cpu(h_operator_t *parent_, int device_) : parent(parent_), device(device_) {
-    gpu(mgv_channel_create(1, &channel));
+    if (!buffer_manager<int32_t>::initialized())
+        throw mgv::error(MGV_E_NOT_INIT, "gpu_to_cpu: buffer_manager<int32_t>::init was not called");
+    try {
+        gpu(mgv_pool_create_host(kOutBuffers, kOutRows, &out_pool));
+        // a slot for every pool buffer, the end-of-stream entry and a scalar result
+        gpu(mgv_ring_create(kOutBuffers + 4, &ring));
+        gpu(mgv_stream_out_create(out_pool, ring, &rows_out));
+    } catch (...) {
+        mgv_ring_destroy(ring);
+        mgv_pool_destroy(out_pool);
+        throw;
+    }
 }
 
 template <>
 gpu_to_cpu<WARPSIZE, 64, buffer_t *>::~gpu_to_cpu() {
-    eof = true;
+    stop = true;
     if (catcher.joinable()) catcher.join();
-    mgv_channel_destroy(channel);
+    mgv_stream_out_destroy(rows_out);
+    mgv_ring_destroy(ring);
+    mgv_pool_destroy(out_pool);
 }
 
 // ------------------------------------------------------------------ split / union_all
@@ -203,42 +223,61 @@ void d_operator_t::do_open() {
     if (sink) {
         // before_open: open the host parent and start the catcher
         sink->parent->open();
-        sink->eof = false;
+        sink->stop = false;
+        sink->failure.clear();
+        sink->last = mgv_result{};
+        sink->buffers = sink->dropped = 0;
         gpu_to_cpu_t *s = sink;
         const bool rows = kind == SELECT_MAT;
         const bool scalar_sum = sum;
+        if (rows) {
+            gpu(mgv_stream_out_open(s->rows_out, nullptr));
+            gpu(mgv_stream_synchronize(nullptr));
+        }
         s->catcher = std::thread([s, rows, scalar_sum]() {
-            mgv_result r;
-            uint32_t seq = 0;
-            for (;;) {
-                if (mgv_channel_poll(s->channel, 0, &r, &seq) == 0) {
-                    s->last = r;
-                    s->out.clear();
-                    if (rows) {
-                        // the selected rows, one host buffer per capacity-sized slice
-                        const size_t total = s->rows.size();
-                        const size_t cap = buffer_t::capacity();
-                        s->out.reserve(total / cap + 1);
-                        for (size_t off = 0; off < total || off == 0; off += cap) {
-                            buffer_t b(s->rows.data() + off, -1);
-                            b.cnt = (uint32_t)std::min(cap, total - off);
-                            s->out.push_back(b);
-                            if (total == 0) break;
-                        }
-                    } else {
-                        const int64_t v = scalar_sum ? r.sum : (int64_t)r.count;
+            try {
+                mgv_ring_entry e;
+                for (;;) {
+                    const int rc = mgv_ring_poll(s->ring, &e);
+                    if (rc == 1) {
+                        if (s->stop.load()) break;
+                        std::this_thread::sleep_for(std::chrono::microseconds(20));
+                        continue;
+                    }
+                    gpu(rc);
+                    if (!rows) {   // the scalar result (aggregation / join count)
+                        s->last = e.res;
+                        const int64_t v = scalar_sum ? e.res.sum : (int64_t)e.res.count;
                         s->rows.assign(1, (int32_t)(uint32_t)(uint64_t)v);
-                        buffer_t b(s->rows.data(), -1);
+                        s->out.assign(1, buffer_t(s->rows.data(), -1));
+                        buffer_t &b = s->out[0];
                         b.cnt = 1;
                         b.scalar = true;
                         b.value64 = v;
-                        s->out.push_back(b);
+                        s->parent->consume(&b);
+                        break;
                     }
-                    for (buffer_t &b : s->out) s->parent->consume(&b);
-                    return;
+                    if (e.id == MGV_RING_EOF) {   // at_close: totals of the stream
+                        s->last = mgv_result{};
+                        s->last.count = e.res.count;
+                        s->dropped = e.res.overflow;
+                        break;
+                    }
+                    // a full pool buffer (pinned host memory): copy its rows
+                    // into a host buffer of the manager, hand the pool buffer
+                    // back, pass the rows on
+                    if (e.value > gpu_to_cpu_t::kOutRows) throw mgv::error(MGV_E_CAPACITY, "gpu_to_cpu: bad buffer");
+                    int32_t *data = nullptr;
+                    gpu(mgv_pool_buffer(s->out_pool, (int32_t)e.id, &data, nullptr));
+                    buffer_t *h = buffer_manager<int32_t>::get_buffer();
+                    memcpy(h->data, data, (size_t)e.value * sizeof(int32_t));
+                    gpu(mgv_pool_release_host(s->out_pool, (int32_t)e.id));
+                    h->cnt = (uint32_t)e.value;
+                    ++s->buffers;
+                    s->parent->consume(h);
                 }
-                if (s->eof.load()) return;
-                std::this_thread::sleep_for(std::chrono::microseconds(20));
+            } catch (const std::exception &ex) {
+                s->failure = ex.what();
             }
         });
     }
@@ -251,7 +290,6 @@ void d_operator_t::do_open() {
         }
         gpu(mgv_result_clear(res, 1, nullptr));
     }
-    out_upper = 0;
     gpu(mgv_stream_synchronize(nullptr));
     opened = true;
 }
@@ -274,41 +312,41 @@ void d_operator_t::consume(buffer_t *v, mgv_stream_t s) {
         case SELECT_SUM:
             gpu(mgv_select_sum(c[0], nc > 1 ? c[1] : c[0], n, bound, res, L, s));
             break;
-        case SELECT_MAT: {
-            if (acc->out_upper + n > acc->out_cap) {   // grow the output (rare: doubling)
-                // every stream: the children of a union_all write from several
-                gpu(mgv_device_synchronize());
-                uint64_t cap = acc->out_cap ? acc->out_cap : (1u << 20);
-                while (cap < acc->out_upper + n) cap *= 2;
-                void *p = nullptr;
-                gpu(mgv_malloc(&p, cap * sizeof(int32_t)));
-                if (acc->out) {
-                    gpu(mgv_memcpy(p, acc->out, acc->out_cap * sizeof(int32_t)));
-                    gpu(mgv_free(acc->out));
-                }
-                acc->out = (int32_t *)p;
-                acc->out_cap = cap;
-            }
-            acc->out_upper += n;
-            gpu(mgv_select_materialize(c[0], nc > 1 ? c[1] : c[0], n, bound, acc->out, acc->out_cap, res, L, s));
+        case SELECT_MAT:
+            // gpu_to_cpu::consume_warp: the survivors stream into the sink's
+            // rotating pool buffers (gpu_to_cpu.cu:61-105)
+            gpu(mgv_select_stream(c[0], nc > 1 ? c[1] : c[0], n, bound, sink->rows_out, L, s));
             break;
-        }
         case PROBE:
-            gpu(mgv_hashjoin_probe(table->slots, table->nbuckets, nullptr, 0, c[0], nullptr, n, 0, 0, res, L, s));
-            break;
         case SELECT_PROBE: {
-            const int32_t *keys = nc > 1 ? c[1] : c[0];
-            gpu(mgv_hashjoin_probe(table->slots, table->nbuckets, c[0], bound, keys, nullptr, n, 0, 0, res, L, s));
+            const int32_t *pred = kind == SELECT_PROBE ? c[0] : nullptr;
+            const int32_t *keys = kind == SELECT_PROBE && nc > 1 ? c[1] : c[0];
+            uint64_t need = 0;
+            if (lds_join(n, false, &need)) {   // a large vector against a large table: through LDS slices
+                void *w = workspace(s, need);
+                gpu(mgv_hashjoin_probe_lds(table->slots, table->nbuckets, pred, bound, keys, nullptr, n, kLdsFanout, w,
+                                           need, res, mgv_launch{0, 1024, -1, MGV_LAUNCH_SCATTER(2u)}, s));
+            } else {
+                gpu(mgv_hashjoin_probe(table->slots, table->nbuckets, pred, bound, keys, nullptr, n, 0, 0, res, L, s));
+            }
             break;
         }
         case BUILD:
-            gpu(mgv_hashjoin_build(table->slots, table->nbuckets, c[0], nc > 1 ? c[1] : nullptr,
-                                   nc > 1 ? MGV_PAYLOAD_COLUMN : MGV_PAYLOAD_NONE, nullptr, 0, n, 0, nullptr, L, s));
-            break;
         case SELECT_BUILD: {
-            const int32_t *keys = nc > 1 ? c[1] : c[0];
-            gpu(mgv_hashjoin_build(table->slots, table->nbuckets, keys, nc > 2 ? c[2] : nullptr,
-                                   nc > 2 ? MGV_PAYLOAD_COLUMN : MGV_PAYLOAD_NONE, c[0], bound, n, 0, nullptr, L, s));
+            const int32_t *pred = kind == SELECT_BUILD ? c[0] : nullptr;
+            const int k = kind == SELECT_BUILD ? 1 : 0;   // first carried column: the key
+            const int32_t *keys = nc > k ? c[k] : c[0];
+            const int32_t *pay = nc > k + 1 ? c[k + 1] : nullptr;
+            const int mode = pay ? MGV_PAYLOAD_COLUMN : MGV_PAYLOAD_NONE;
+            uint64_t need = 0;
+            if (lds_join(n, true, &need)) {
+                void *w = workspace(s, need);
+                gpu(mgv_hashjoin_build_lds(table->slots, table->nbuckets, keys, pay, mode, pred, bound, n, 0, kLdsFanout,
+                                           w, need, nullptr, mgv_launch{0, 1024, -1, MGV_LAUNCH_SCATTER(2u)}, s));
+            } else {
+                gpu(mgv_hashjoin_build(table->slots, table->nbuckets, keys, pay, mode, pred, bound, n, 0, nullptr, L,
+                                       s));
+            }
             break;
         }
         case SPLIT: {
@@ -342,6 +380,34 @@ void d_operator_t::consume(buffer_t *v, mgv_stream_t s) {
         default:
             throw mgv::error(MGV_E_INVALID, "d_operator_t::consume: operator is not the head of a chain");
     }
+}
+
+// Vectors of at least kLdsRows rows against a table of at least kLdsTableBytes
+// (far larger than the L2) are built and probed through LDS table slices
+// (mgv_hashjoin_build_lds / mgv_hashjoin_probe_lds) when the table's buckets
+// split into slices; the results are those of the direct kernels.
+bool d_operator_t::lds_join(uint64_t n, bool build, uint64_t *need) const {
+    if (n < (build ? kLdsBuildRows : kLdsRows) || (uint64_t)table->nbuckets * 64 < kLdsTableBytes) return false;
+    return (build ? mgv_build_lds_workspace_size(n, table->nbuckets, kLdsFanout, need)
+                  : mgv_probe_lds_workspace_size(n, table->nbuckets, kLdsFanout, need)) == MGV_OK;
+}
+
+// The chain's workspace for stream s, at least `bytes`: reused vector after
+// vector in stream order, replaced (after the stream's work on it is done)
+// when a larger one is needed.
+void *d_operator_t::workspace(mgv_stream_t s, uint64_t bytes) {
+    std::lock_guard<std::mutex> g(ws_mu);
+    auto &w = ws[s];
+    if (w.second < bytes) {
+        if (w.first) {
+            gpu(mgv_stream_synchronize(s));
+            gpu(mgv_free(w.first));
+            w = {nullptr, 0};
+        }
+        gpu(mgv_malloc(&w.first, bytes));
+        w.second = bytes;
+    }
+    return w.first;
 }
 
 // Hands the staged rows of every destination to its parent as vectors of at
@@ -392,26 +458,24 @@ void d_operator_t::do_close() {
     gpu(mgv_stream_synchronize(last_stream));
     gpu(mgv_device_synchronize());
     if (sink) {
-        // at_close: hand the result (and the selected rows) to the host
-        if (kind == SELECT_MAT) {
-            mgv_result r;
-            gpu(mgv_memcpy(&r, res, sizeof(r)));
-            const uint64_t n = r.count < out_cap ? r.count : out_cap;
-            sink->rows.resize(n);
-            if (n) gpu(mgv_memcpy(sink->rows.data(), out, n * sizeof(int32_t)));
-        }
-        gpu(mgv_channel_emit(sink->channel, 0, res, 1, nullptr));
+        // at_close: the end of the rows stream, or the scalar result
+        if (kind == SELECT_MAT) gpu(mgv_stream_out_close(sink->rows_out, nullptr));
+        else gpu(mgv_ring_emit(sink->ring, 0, 0, res, nullptr));
         gpu(mgv_stream_synchronize(nullptr));
         // after_close: join the catcher, close the host parent
         if (sink->catcher.joinable()) sink->catcher.join();
+        if (!sink->failure.empty()) throw mgv::error(MGV_E_INVALID, "gpu_to_cpu: " + sink->failure);
+        if (sink->dropped)
+            throw mgv::error(MGV_E_CAPACITY, "gpu_to_cpu: " + std::to_string(sink->dropped) +
+                                                 " rows found no free device buffer in time");
         sink->parent->close();
     }
     opened = false;
 }
 
 d_operator_t::~d_operator_t() {
+    for (auto &w : ws) mgv_free(w.second.first);
     mgv_free(res);
-    mgv_free(out);
     mgv_free(stage);
     mgv_free(cursors);
 }

@@ -3,8 +3,11 @@
 // One host thread drives the loop: it releases requests at their arrival
 // time, admits waiting requests FIFO while their HBM footprint fits the free
 // budget, packs the admitted requests into one batch per free stream slot
-// (fused persistent launches, as mgv_batch_run), and retires batches whose
-// completion event has fired, returning their memory.  The reference has no
+// (fused persistent launches, as mgv_batch_run), and takes completed
+// requests off a device-written completion ring (each batch ends with a
+// kernel that publishes every request's {id, result slot, result}),
+// retiring a batch and returning its memory once all its requests are off
+// the ring.  The reference has no
 // serving layer; the per-query plan it would run is the select -> hashjoin ->
 // aggregation chain of src/main9.cu:260-367 and the result hand-off of
 // gpu_to_cpu (src/operators/gpu_to_cpu.cu:42-58,149-165).
@@ -95,13 +98,14 @@ constexpr uint32_t kLdsBuildFanout = 32;
 
 struct Slot {
     mgv_stream_t stream = nullptr;
-    mgv_event_t done = nullptr;
     bool busy = false;
+    uint32_t pending = 0;             // requests of the batch not yet off the ring
     uint32_t batch = 0;
     std::vector<Admitted> q;
+    std::vector<mgv_result> ring_res;   // the batch's results as they came off the ring
     uint32_t nbuild = 0, nmain = 0;
     uint64_t nbitems = 0, nmitems = 0;
-    // device: [build queries | main queries | build items | main items], results, counters
+    // device: [build queries | main queries | request ids | build items | main items], results, counters
     uint8_t *d_desc = nullptr;
     mgv_result *d_res = nullptr;
     uint32_t *d_counter = nullptr;
@@ -120,8 +124,12 @@ struct Server {
     Arena arena;
     uint8_t *d_arena = nullptr;
     std::vector<Slot> slots;
+    std::vector<uint32_t> slot_of;    // request -> slot running it
+    mgv_ring *ring = nullptr;         // completion ring, a slot per request that can be in flight
+    uint64_t ring_results = 0;
     uint64_t items_cap = 0;
     size_t desc_bytes = 0;
+    size_t ids_at = 0, items_at = 0;  // offsets in the descriptor staging
 
     Server(const mgv_serve_request *r, uint32_t n_, const mgv_serve_config &c, mgv_serve_result *o)
         : R(r), n(n_), cfg(c), out(o), arena(c.budget_bytes) {}
@@ -135,9 +143,9 @@ struct Server {
             mgv_host_free(s.h_desc);
             mgv_host_free(s.h_res);
             mgv_host_free(s.h_groups);
-            mgv_event_destroy(s.done);
             mgv_stream_destroy(s.stream);
         }
+        mgv_ring_destroy(ring);
         mgv_free(d_arena);
     }
 
@@ -185,8 +193,11 @@ struct Server {
         std::sort(per.rbegin(), per.rend());
         for (uint32_t i = 0; i < per.size() && i < cfg.max_batch; ++i) items_cap += per[i];
         items_cap = std::max<uint64_t>(items_cap, 1);
-        desc_bytes = align_up(2 * (size_t)cfg.max_batch * sizeof(mgv_query), 256) +
-                     (size_t)items_cap * sizeof(mgv_work_item);
+        ids_at = align_up(2 * (size_t)cfg.max_batch * sizeof(mgv_query), 256);
+        items_at = align_up(ids_at + (size_t)cfg.max_batch * sizeof(uint64_t), 256);
+        desc_bytes = items_at + (size_t)items_cap * sizeof(mgv_work_item);
+        slot_of.assign(n, 0);
+        gpu(mgv_ring_create(cfg.max_inflight * cfg.max_batch + 2, &ring));
         void *p = nullptr;
         gpu(mgv_malloc(&p, std::max<uint64_t>(cfg.budget_bytes, 256)));
         d_arena = (uint8_t *)p;
@@ -195,7 +206,6 @@ struct Server {
         slots.resize(cfg.max_inflight);
         for (Slot &s : slots) {
             gpu(mgv_stream_create(&s.stream));
-            gpu(mgv_event_create(&s.done));
             gpu(mgv_malloc(&p, desc_bytes));
             s.d_desc = (uint8_t *)p;
             gpu(mgv_malloc(&p, 2 * (size_t)cfg.max_batch * sizeof(mgv_result)));
@@ -218,8 +228,9 @@ struct Server {
     void issue(Slot &s) {
         mgv_query *bq = (mgv_query *)s.h_desc;
         mgv_query *mq = bq + cfg.max_batch;
-        const size_t items_at = align_up(2 * (size_t)cfg.max_batch * sizeof(mgv_query), 256);
+        uint64_t *ids = (uint64_t *)(s.h_desc + ids_at);
         mgv_work_item *items = (mgv_work_item *)(s.h_desc + items_at);
+        for (size_t i = 0; i < s.q.size(); ++i) ids[i] = s.q[i].req;
         s.nbuild = s.nmain = 0;
         uint32_t join = 0;   // build result slot: max_batch + the join's index in the batch
         for (size_t i = 0; i < s.q.size(); ++i) {
@@ -300,7 +311,8 @@ struct Server {
                                         r.ngroups, g + garr, a.ws, s.d_res + i, mgv_launch{0, 1024, -1, 0},
                                         s.stream));
         }
-        gpu(mgv_memcpy_async(s.h_res, s.d_res, 2 * (size_t)cfg.max_batch * sizeof(mgv_result), s.stream));
+        gpu(mgv_memcpy_async(s.h_res, s.d_res + cfg.max_batch, (size_t)cfg.max_batch * sizeof(mgv_result),
+                             s.stream));
         for (size_t i = 0; i < s.q.size(); ++i) {
             const Admitted &a = s.q[i];
             const mgv_serve_request &r = R[a.req];
@@ -308,18 +320,36 @@ struct Server {
                 gpu(mgv_memcpy_async(s.h_groups + 2 * a.group_stage, d_arena + a.off, (size_t)r.ngroups * 16,
                                      s.stream));
         }
-        gpu(mgv_event_record(s.done, s.stream));
+        // gpu_to_cpu: every request's {id, result slot, result} on the ring,
+        // after the copies above (stream order)
+        gpu(mgv_ring_emit_results(ring, (const uint64_t *)(s.d_desc + ids_at), s.d_res, (uint32_t)s.q.size(),
+                                  s.stream));
+        s.pending = (uint32_t)s.q.size();
         s.busy = true;
     }
 
-    void retire(Slot &s, double now) {
+    // A request's completion came off the ring: its result, and the batch's
+    // retirement once it was the batch's last.
+    void complete(const mgv_ring_entry &e, double now) {
+        if (e.id >= n) throw mgv::error(MGV_E_INVALID, "serve: completion of an unknown request");
+        Slot &s = slots[slot_of[e.id]];
+        if (!s.busy || e.value >= s.q.size() || s.q[e.value].req != e.id)
+            throw mgv::error(MGV_E_INVALID, "serve: completion does not match its batch");
+        s.ring_res.resize(s.q.size());
+        s.ring_res[e.value] = e.res;
+        out[e.id].done_s = now;
+        ++ring_results;
+        if (--s.pending == 0) retire(s);
+    }
+
+    void retire(Slot &s) {
         uint32_t b = 0;
         for (size_t i = 0; i < s.q.size(); ++i) {
             const Admitted &a = s.q[i];
             const mgv_serve_request &r = R[a.req];
             mgv_serve_result &o = out[a.req];
-            const mgv_result &m = s.h_res[i];
-            o.done_s = now;
+            const mgv_result &m = s.ring_res[i];
+            const double now = o.done_s;
             o.latency_s = now - r.arrival_s;
             o.batch = s.batch;
             o.footprint = align_up(a.bytes, 256);
@@ -327,7 +357,7 @@ struct Server {
                 o.count = m.count;
                 o.sum = m.sum;
                 o.digest = 0;
-                o.errors = (uint32_t)s.h_res[cfg.max_batch + b].errors;
+                o.errors = (uint32_t)s.h_res[b].errors;
                 ++b;
             } else {
                 const uint64_t *cnt = s.h_groups + 2 * a.group_stage;
@@ -361,16 +391,14 @@ struct Server {
         while (done < n) {
             double now = now_s();
             while (next < n && R[next].arrival_s <= now) waiting.push_back(next++);
-            // retire finished batches
-            for (Slot &s : slots) {
-                if (!s.busy) continue;
-                const int q = mgv_event_query(s.done);
-                if (q == 0) {
-                    done += (uint32_t)s.q.size();
-                    retire(s, now_s());
-                } else if (q != 1) {
-                    gpu(q);
-                }
+            // completed requests off the ring; a batch retires with its last
+            for (;;) {
+                mgv_ring_entry e;
+                const int q = mgv_ring_poll(ring, &e);
+                if (q == 1) break;
+                gpu(q);
+                complete(e, now_s());
+                ++done;
             }
             // admit into free slots
             bool issued = false;
@@ -389,6 +417,7 @@ struct Server {
                     uint64_t off = 0;
                     if (!arena.alloc(bytes, &off)) break;   // FIFO: wait for memory
                     s.q.push_back(Admitted{i, off, bytes, nb, gstage, ws});
+                    slot_of[i] = (uint32_t)(&s - slots.data());
                     out[i].admit_s = now_s();
                     items += it;
                     gstage += gs;
@@ -434,6 +463,7 @@ struct Server {
             st->max_inflight_seen = max_inflight_seen;
             st->peak_bytes = arena.peak();
             st->mean_batch = batches ? (double)batched_queries / batches : 0;
+            st->ring_results = ring_results;
         }
     }
 };

@@ -54,6 +54,13 @@ class Result(C.Structure):
         return {k: getattr(self, k) for k in ("count", "sum", "digest_sum", "digest_xor", "errors", "overflow")}
 
 
+class RingEntry(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("value", C.c_uint64), ("res", Result)]
+
+
+RING_EOF = (1 << 64) - 1
+
+
 class DeviceProps(C.Structure):
     _fields_ = [
         ("name", C.c_char * 64),
@@ -165,12 +172,26 @@ SIGNATURES = {
     "mgv_channel_destroy": [VP],
     "mgv_channel_emit": [VP, U32, VP, U32, VP],
     "mgv_channel_poll": [VP, U32, VP, C.POINTER(U32)],
+    "mgv_ring_create": [U32, C.POINTER(VP)],
+    "mgv_ring_destroy": [VP],
+    "mgv_ring_emit": [VP, U64, U64, VP, VP],
+    "mgv_ring_emit_results": [VP, VP, VP, U32, VP],
+    "mgv_ring_poll": [VP, VP],
+    "mgv_pool_release_host": [VP, I32],
+    "mgv_pool_create_host": [U32, U32, C.POINTER(VP)],
+    "mgv_stream_out_create": [VP, VP, C.POINTER(VP)],
+    "mgv_stream_out_destroy": [VP],
+    "mgv_stream_out_open": [VP, VP],
+    "mgv_stream_out_close": [VP, VP],
+    "mgv_select_stream": [VP, VP, U64, I32, VP, Launch, VP],
     "mgv_groupby_workspace_size": [U64, U32, C.POINTER(U64)],
     "mgv_probe_lds_workspace_size": [U64, U32, U32, C.POINTER(U64)],
     "mgv_build_lds_workspace_size": [U64, U32, U32, C.POINTER(U64)],
     "mgv_hashjoin_build_lds": [VP, U32, VP, VP, I32, VP, I32, U64, I64, U32, VP, U64, VP, Launch, VP],
     "mgv_hashjoin_probe_lds": [VP, U32, VP, I32, VP, VP, U64, U32, VP, U64, VP, Launch, VP],
     "mgv_hashjoin_probe_lds_noalloc": [VP, U32, VP, I32, VP, VP, U64, U32, VP, Launch, VP],
+    "mgv_probe_groupby_lds_workspace_size": [U64, U32, U32, C.POINTER(U64)],
+    "mgv_hashjoin_probe_groupby_lds": [VP, U32, VP, I32, VP, VP, U64, VP, VP, U32, U32, VP, U64, Launch, VP],
     "mgv_groupby_partitioned": [VP, VP, U64, VP, VP, U32, VP, U64, VP, Launch, VP],
     "mgv_split": [VP, I32, VP, U32, U64, U32, VP, U64, VP, Launch, VP],
 }
@@ -232,7 +253,7 @@ class ServeStats(C.Structure):
     _fields_ = [
         ("elapsed_s", C.c_double), ("queries_per_s", C.c_double), ("p50_s", C.c_double), ("p99_s", C.c_double),
         ("max_s", C.c_double), ("mean_s", C.c_double), ("batches", C.c_uint32), ("max_inflight_seen", C.c_uint32),
-        ("peak_bytes", C.c_uint64), ("mean_batch", C.c_double),
+        ("peak_bytes", C.c_uint64), ("mean_batch", C.c_double), ("ring_results", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -243,6 +264,7 @@ OPS_PATH = os.path.join(HERE, "lib", "libmgv_ops.so")
 OPS_SIGNATURES = {
     "mgv_ops_init": [U32, U32, U32],
     "mgv_ops_shutdown": [],
+    "mgv_ops_pool_stats": [C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U32)],
     "mgv_ops_last_error": [],
     "mgv_ops_sum_select": [VP, VP, U64, I32, U32, C.c_int, U32, U32, VP, C.POINTER(I64)],
     "mgv_ops_select": [VP, VP, U64, I32, C.c_int, U32, C.c_int, U32, U32, VP, U64, C.POINTER(U64)],

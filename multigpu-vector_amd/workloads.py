@@ -120,6 +120,11 @@ class JoinBatch:
 
     def _drop_plans(self):
         for plan in self._plans.values():
+            if isinstance(plan, dict):   # routed plan
+                dq, di, _ = plan["small"]
+                for b in [dq, di] + plan["ws"] + plan["streams"] + plan["events"]:
+                    b.close()
+                continue
             for dq, di, _ in plan:
                 dq.close()
                 di.close()
@@ -160,6 +165,91 @@ class JoinBatch:
         self._plan(vector_rows)
         mgv.call("mgv_graph_begin_capture", stream)
         self.issue_fused(vector_rows, launch, stream)
+        g = C.c_void_p()
+        mgv.call("mgv_graph_end_capture", stream, C.byref(g))
+        return g
+
+    # -------------------------------------------------------------- routed path
+    def lds_fanout(self, x: JoinQuery) -> int:
+        """Fan-out that cuts a query's table into slices of <= 2048 buckets in
+        one partitioning level (0 when its bucket count does not split)."""
+        f = 2
+        while x.nbuckets // f > 2048:
+            f *= 2
+        return f if x.nbuckets % f == 0 and f <= 2048 else 0
+
+    def _routed(self, vector_rows: int, threshold: int, nstreams: int):
+        """Plan of the routed batch: the fused items of the small probes
+        (P < threshold) and the large probes' LDS calls; cached."""
+        key = ("routed", vector_rows, threshold, nstreams)
+        if key in self._plans:
+            return self._plans[key]
+        big = [i for i, x in enumerate(self.q) if x.P >= threshold and self.lds_fanout(x)]
+        small = [i for i in range(self.nq) if i not in set(big)]
+        sq = (mgv.Query * max(1, len(small)))()
+        for j, i in enumerate(small):
+            sq[j] = self.probe_q[i]
+        n = C.c_uint64()
+        mgv.call("mgv_batch_plan", sq, len(small), vector_rows, None, 0, C.byref(n))
+        items = (mgv.WorkItem * max(1, n.value))()
+        if small:
+            mgv.call("mgv_batch_plan", sq, len(small), vector_rows, items, n.value, C.byref(n))
+        dq = mgv.DeviceBuffer(C.sizeof(sq)).upload(np.frombuffer(bytes(sq), dtype=np.uint8))
+        di = mgv.DeviceBuffer(max(16, C.sizeof(items))).upload(np.frombuffer(bytes(items), dtype=np.uint8))
+        need = 0
+        for i in big:
+            x = self.q[i]
+            b = C.c_uint64()
+            mgv.call("mgv_probe_lds_workspace_size", x.P, x.nbuckets, self.lds_fanout(x), C.byref(b))
+            need = max(need, b.value)
+        ws = [mgv.DeviceBuffer(max(256, need)) for _ in range(nstreams)] if big else []
+        streams = [mgv.Stream() for _ in range(nstreams)] if big else []
+        events = [mgv.Event() for _ in range(nstreams + 1)] if big else []
+        plan = {"small": (dq, di, n.value if small else 0), "big": big, "ws": ws, "streams": streams,
+                "events": events}
+        self._plans[key] = plan
+        return plan
+
+    def issue_routed(self, vector_rows: int, launch: mgv.Launch, stream, threshold: int = 1 << 20,
+                     nstreams: int = 4):
+        """clear + the fused build launch; then the small probes in one fused
+        launch on `stream` while the large ones (P >= threshold, tables that
+        split into LDS slices) run through mgv_hashjoin_probe_lds on nstreams
+        side streams (forked and joined with events, so the whole batch can be
+        captured into one graph).  Results land in the same slots as the fused
+        path's."""
+        (bq, bi, bn), _ = self._plan(vector_rows)
+        plan = self._routed(vector_rows, threshold, nstreams)
+        self.clear(stream)
+        mgv.call("mgv_batch_run", bq.vp, bi.vp, bn, self.counter.vp, launch, stream)
+        ev = plan["events"]
+        if plan["big"]:
+            ev[0].record(stream)
+            for sd in plan["streams"]:
+                mgv.call("mgv_stream_wait_event", sd.vp, C.c_void_p(ev[0].ptr))
+            lds_launch = mgv.Launch(0, 1024, -1, 2 << 8)
+            for j, i in enumerate(plan["big"]):
+                x = self.q[i]
+                k = j % len(plan["streams"])
+                ws, sd = plan["ws"][k], plan["streams"][k]
+                r4 = 4 * x.r_off
+                mgv.call("mgv_hashjoin_probe_lds", self.tables.ptr + x.t_off, x.nbuckets, self.rc.at(r4), x.bound,
+                         self.rk.at(r4), self.rv.at(r4), x.P, self.lds_fanout(x), ws.vp, ws.nbytes,
+                         self.res.slot(self.nq + i), lds_launch, sd.vp)
+        dq, di, pn = plan["small"]
+        if pn:
+            mgv.call("mgv_batch_run", dq.vp, di.vp, pn, self.counter.at(128), launch, stream)
+        if plan["big"]:
+            for k, sd in enumerate(plan["streams"]):
+                ev[k + 1].record(sd.vp)
+                mgv.call("mgv_stream_wait_event", stream, C.c_void_p(ev[k + 1].ptr))
+
+    def capture_routed(self, vector_rows: int, launch: mgv.Launch, stream, threshold: int = 1 << 20,
+                       nstreams: int = 4) -> C.c_void_p:
+        self._plan(vector_rows)
+        self._routed(vector_rows, threshold, nstreams)
+        mgv.call("mgv_graph_begin_capture", stream)
+        self.issue_routed(vector_rows, launch, stream, threshold, nstreams)
         g = C.c_void_p()
         mgv.call("mgv_graph_end_capture", stream, C.byref(g))
         return g
@@ -214,8 +304,12 @@ class ServeTables:
 
     Join pairs: for every b in build_logs, S_b has 2^b unique keys (a seeded
     permutation of 0..2^b-1) with payload w in [0, 2^16), and R_b has
-    min(16 * 2^b, probe_cap) rows {c uniform in [0, 2^31), k uniform in
-    [0, 2^(b+1)) (about 50% match), v in [0, 2^16)}.
+    min(16 * 2^b, probe_cap) rows {c uniform in [0, 2^31), k, v in
+    [0, 2^16)}.  With join_zipf_s set (the select+join+sum shape of the
+    skewed join workload), k follows Zipf(join_zipf_s) over the 2^b build
+    keys, ranks mapped through a seeded permutation, so every row matches and
+    a few keys take most rows; without it k is uniform in [0, 2^(b+1))
+    (about 50% match).
     Group-by tables: G_m has group_rows[m] rows {k ~ Zipf(zipf_s) over
     ngroups ranks mapped through a seeded permutation, v in [0, 2^16)}.
     """
@@ -223,7 +317,7 @@ class ServeTables:
     CFG = 5
 
     def __init__(self, build_logs=range(16, 27), probe_cap=1 << 29, group_rows=(1 << 30,) * 5,
-                 ngroups=1 << 16, zipf_s=0.8, stream=None):
+                 ngroups=1 << 16, zipf_s=0.8, join_zipf_s=None, stream=None):
         sd = mgv.seed
         self.pairs = []
         self.groups = []
@@ -239,7 +333,14 @@ class ServeTables:
             mgv.call("mgv_gen_perm", t["sk"].vp, B, B, sd(self.CFG, 0), j, stream)
             mgv.call("mgv_gen_bits", t["sw"].vp, B, sd(self.CFG, 1), j, 16, stream)
             mgv.call("mgv_gen_bits", t["rc"].vp, P, sd(self.CFG, 2), j, 31, stream)
-            mgv.call("mgv_gen_range", t["rk"].vp, P, sd(self.CFG, 3), j, 2 * B, stream)
+            if join_zipf_s is None:
+                mgv.call("mgv_gen_range", t["rk"].vp, P, sd(self.CFG, 3), j, 2 * B, stream)
+            else:
+                zt = C.c_void_p()
+                mgv.call("mgv_zipf_table_create", C.byref(zt), B, float(join_zipf_s))
+                mgv.call("mgv_gen_zipf", t["rk"].vp, P, sd(self.CFG, 3), j, zt, B, sd(self.CFG, 8), j, stream)
+                mgv.call("mgv_stream_synchronize", stream)
+                mgv.call("mgv_free", zt)
             mgv.call("mgv_gen_bits", t["rv"].vp, P, sd(self.CFG, 4), j, 16, stream)
             self.pairs.append(t)
         table = C.c_void_p()

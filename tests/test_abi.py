@@ -83,8 +83,18 @@ def test_ops_calls_fail_cleanly_before_init():
     assert "mgv_ops_init" in mgv.load_ops().mgv_ops_last_error().decode()
 
 
-def test_serve_struct_sizes_match_header():
-    # sizeof of mgv_serve_request / _result / _config / _stats as gcc lays out include/mgv_ops.h
-    assert (C.sizeof(mgv.ServeRequest), C.sizeof(mgv.ServeResult), C.sizeof(mgv.ServeConfig),
-            C.sizeof(mgv.ServeStats)) == (80, 64, 48, 72)
-    assert C.sizeof(mgv.Query) == 96 and C.sizeof(mgv.WorkItem) == 16
+def test_struct_layouts_match_the_headers(tmp_path):
+    """sizeof of every struct the binding mirrors, as gcc lays out
+    include/mgv.h and include/mgv_ops.h, equals the ctypes layout."""
+    import subprocess
+    structs = {"mgv_serve_request": mgv.ServeRequest, "mgv_serve_result": mgv.ServeResult,
+               "mgv_serve_config": mgv.ServeConfig, "mgv_serve_stats": mgv.ServeStats, "mgv_query": mgv.Query,
+               "mgv_work_item": mgv.WorkItem, "mgv_result": mgv.Result, "mgv_launch": mgv.Launch,
+               "mgv_ring_entry": mgv.RingEntry, "mgv_device_props": mgv.DeviceProps}
+    src = tmp_path / "sizes.c"
+    src.write_text('#include <stdio.h>\n#include "mgv_ops.h"\nint main(void) {\n' +
+                   "".join(f'  printf("%zu\\n", sizeof({k}));\n' for k in structs) + "  return 0;\n}\n")
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == [C.sizeof(t) for t in structs.values()], dict(zip(structs, got))

@@ -282,3 +282,47 @@ def test_result_channel(gpu):
     assert (r.count, r.sum, seq.value) == (500, sum(range(500)), 77)
     assert mgv.load().mgv_channel_poll(ch, 2, C.byref(r), C.byref(seq)) == 1
     gpu.call("mgv_channel_destroy", ch)
+
+
+def test_routed_batch_equals_fused_and_replays(gpu, orc):
+    """A C4-shaped batch whose large probes are routed through LDS table
+    slices on side streams (forked and joined inside one hipGraph) gives
+    every query the fused batch's result, on every replay, and the oracle's
+    for the queries the oracle checks."""
+    import workloads
+    shapes = workloads.c4_shapes(40, seed=0x4C, lo=12.0, hi=20.0)
+    st = mgv.Stream()
+    batch = workloads.JoinBatch(shapes, 0.25, st.vp)
+    st.sync()
+    L = mgv.launch(0, 256, 0)
+    g = batch.capture_fused(4096, L, st.vp)
+    gpu.call("mgv_graph_launch", g, st.vp)
+    st.sync()
+    fused = batch.results()
+    gpu.call("mgv_graph_destroy", g)
+    assert int(fused["build_errors"].sum()) == 0
+    thr = 1 << 16
+    big = [i for i, x in enumerate(shapes) if x.P >= thr and batch.lds_fanout(x)]
+    assert 4 <= len(big) < len(shapes)
+    g = batch.capture_routed(4096, L, st.vp, threshold=thr, nstreams=3)
+    for rep in range(3):
+        gpu.call("mgv_graph_launch", g, st.vp)
+        st.sync()
+        r = batch.results()
+        assert np.array_equal(r["count"], fused["count"]) and np.array_equal(r["sum"], fused["sum"]), rep
+        batch.res.clear(st.vp)   # eager work between replays
+        st.sync()
+    gpu.call("mgv_graph_destroy", g)
+    # the same routing issued eagerly
+    batch.issue_routed(4096, L, st.vp, threshold=thr, nstreams=3)
+    st.sync()
+    r = batch.results()
+    assert np.array_equal(r["count"], fused["count"]) and np.array_equal(r["sum"], fused["sum"])
+    for i in big[:3] + [j for j in range(len(shapes)) if j not in big][:3]:
+        x, h = shapes[i], batch.host_columns(i)
+        tb = orc.table(h["sk"], h["sw"])
+        want = orc.probe(tb, h["k"], h["v"], pred=h["c"], bound=x.bound)
+        orc.free(tb)
+        assert (int(r["count"][i]), int(r["sum"][i])) == (want["count"], want["sum"]), i
+    batch.close()
+    st.close()

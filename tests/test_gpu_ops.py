@@ -205,6 +205,37 @@ def test_select_pipelines_with_union_all_cpu(ops, orc, npipelines, aggregate, wh
     del keep
 
 
+def pool_stats():
+    v = [C.c_uint32() for _ in range(4)]
+    mgv.ops_call("mgv_ops_pool_stats", *[C.byref(x) for x in v])
+    return [x.value for x in v]
+
+
+def test_select_plan_streams_more_rows_than_the_device_pool(ops, orc):
+    """select -> gpu_to_cpu with more survivors than the manager's device pool
+    holds at once (8 buffers of 4 Mi rows): the kernels rotate the pool's
+    buffers (acquire_buffer_blocked_to, src/buffer_manager.cuh:147-176), the
+    catcher hands each full one back, and at the end every device buffer is
+    back in the pool or the manager's cache."""
+    n = (1 << 26) + 4097
+    a = orc.gen_bits(n, orc.seed(1, 0), 50, 20)
+    b = orc.gen_bits(n, orc.seed(1, 1), 50, 31)
+    bound = int(0.52 * 2**31)
+    want = np.sort(a[b <= bound])
+    free0, cached0, total, _ = pool_stats()
+    assert len(want) > total * (4 << 20)
+    (pb, pa), keep = cols(0, b, a)
+    dst = np.full(n, -7, np.int32)
+    cnt = C.c_uint64()
+    mgv.ops_call("mgv_ops_select", pb, pa, n, bound, 1, 1 << 20, 0, 128, 1024, host_ptr(dst), n, C.byref(cnt))
+    assert cnt.value == len(want)
+    assert np.array_equal(np.sort(dst[:cnt.value]), want)
+    assert np.all(dst[cnt.value:] == -7)
+    free, cached, total2, _ = pool_stats()
+    assert total2 == total and free + cached == total, (free, cached, total)
+    del keep
+
+
 def test_select_plan_rejects_small_dst(ops):
     x = dev(np.arange(1024, dtype=np.int32))
     dst = np.zeros(10, np.int32)
@@ -238,6 +269,37 @@ def test_count_join_plan(ops, orc, where, with_select):
     orc.free(t)
     assert cnt.value == want["count"]
     assert dst[0] == want["count32"]
+    del keep
+
+
+@pytest.mark.parametrize("with_select", [False, True])
+def test_count_join_plan_large_tables_go_through_lds_slices(ops, orc, with_select):
+    """create_count_join_1G_pipeline at 2^24 build keys (a 256 MiB table) and
+    2^26 probe rows in 4 Mi-row vectors: the drop-in operators build and
+    probe those vectors through LDS table slices; the count is the closed
+    form #{L < 2^24} (keys a permutation of 0..2^24-1), with the selects
+    applied to both sides in the second case."""
+    nb, npr = 1 << 24, 1 << 26
+    R = orc.gen_perm(nb, nb, orc.seed(2, 0), 6)
+    L = orc.gen_bits(npr, orc.seed(2, 2), 6, 25)
+    bound = (1 << 30) - 1
+    (dR, dL), keep = cols(0, R, L)
+    if with_select:
+        pr = orc.gen_bits(nb, orc.seed(3, 0), 6, 31)
+        pl = orc.gen_bits(npr, orc.seed(3, 1), 6, 31)
+        (dpr, dpl), keep2 = cols(0, pr, pl)
+        inv = np.empty(nb, np.int64)
+        inv[R] = np.arange(nb)
+        hit = L < nb
+        ok = np.zeros(npr, bool)
+        ok[hit] = pr[inv[L[hit]]] <= bound
+        want = int((ok & (pl <= bound)).sum())
+    else:
+        dpr = dpl = None
+        want = int((L < nb).sum())
+    cnt = C.c_uint64()
+    mgv.ops_call("mgv_ops_count_join", dR, dpr, nb, dL, dpl, npr, bound, 4 << 20, 0, 256, 1024, None, C.byref(cnt))
+    assert cnt.value == want
     del keep
 
 

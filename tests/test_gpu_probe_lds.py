@@ -30,7 +30,7 @@ def build(keys, payload, lf):
     return ht
 
 
-def probe_lds(ht, rk, rv, pred=None, bound=0, fanout=64, grid=0):
+def probe_lds(ht, rk, rv, pred=None, bound=0, fanout=64, grid=0, flags=0):
     n = len(rk)
     need = C.c_uint64()
     mgv.call("mgv_probe_lds_workspace_size", n, ht.nbuckets, fanout, C.byref(need))
@@ -40,7 +40,7 @@ def probe_lds(ht, rk, rv, pred=None, bound=0, fanout=64, grid=0):
     dp = dev(pred) if pred is not None else None
     res = mgv.ResultBuffer()
     mgv.call("mgv_hashjoin_probe_lds", ht.buf.vp, ht.nbuckets, dp.vp if dp else None, bound, dk.vp,
-             dv.vp if dv else None, n, fanout, ws.vp, ws.nbytes, res.vp, mgv.launch(grid, 1024), None)
+             dv.vp if dv else None, n, fanout, ws.vp, ws.nbytes, res.vp, mgv.Launch(grid, 1024, -1, flags), None)
     mgv.sync()
     r = res.read()
     return r.count, r.sum
@@ -71,6 +71,53 @@ def test_lds_probe_matches_oracle(gpu, orc, nb_keys, n, fanout, lf):
     want = orc.probe(t, rk, rv)
     orc.free(t)
     assert got == (want["count"], want["sum"]) == probe_direct(ht, rk, rv)
+
+
+COUNTED = 1 << 12   # MGV_LAUNCH_COUNTED_PART
+
+
+@pytest.mark.parametrize("nb_keys,n,fanout", [(1 << 16, 1 << 20, 64), (1 << 18, 3 << 20, 32), (50000, 123457, 4),
+                                              (1 << 16, 1 << 20, 2048)])
+def test_lds_probe_fill_and_counted_partitionings_agree(gpu, orc, nb_keys, n, fanout):
+    """The default count-free partitioning (fixed-capacity regions filled by
+    per-tile reservations) and the counted one give the oracle's result, with
+    two levels and with one (fan-out 2048 on 2^13 buckets of 8 slots)."""
+    sk = orc.gen_perm(nb_keys, nb_keys, orc.seed(2, 0), 12)
+    sw = orc.gen_bits(nb_keys, orc.seed(2, 1), 12, 16)
+    rk = orc.gen_range(n, orc.seed(2, 2), 12, 2 * nb_keys)
+    rv = orc.gen_bits(n, orc.seed(2, 3), 12, 16)
+    ht = build(sk, sw, 0.5 if fanout != 2048 else 0.25)
+    if fanout == 2048:
+        assert ht.nbuckets >= 2048 and ht.nbuckets % 2048 == 0
+    t = orc.table(sk, sw)
+    want = orc.probe(t, rk, rv)
+    orc.free(t)
+    got = probe_lds(ht, rk, rv, fanout=fanout)
+    assert got == (want["count"], want["sum"]) == probe_lds(ht, rk, rv, fanout=fanout, flags=COUNTED)
+
+
+@pytest.mark.parametrize("kind", ["one_key", "zipf"])
+def test_lds_probe_skewed_keys_overflow_their_regions(gpu, orc, kind):
+    """Skewed probe keys overfill their partitions' regions: the excess rows
+    go to the overflow areas (level 1 into level 2, level 2 to the hashed
+    global-table probe) and the result still equals the oracle's."""
+    nb_keys, n = 1 << 16, 1 << 20
+    sk = orc.gen_perm(nb_keys, nb_keys, orc.seed(2, 0), 13)
+    sw = orc.gen_bits(nb_keys, orc.seed(2, 1), 13, 16)
+    rv = orc.gen_bits(n, orc.seed(2, 3), 13, 16)
+    if kind == "one_key":
+        rk = orc.gen_range(n, orc.seed(2, 2), 13, 2 * nb_keys)
+        rk[: (3 * n) // 5] = 12345          # 60 % of the rows on one key (one partition)
+    else:
+        T = orc.zipf_cdf(nb_keys, 1.2)
+        rk = orc.gen_zipf(n, orc.seed(2, 2), 13, T)
+    ht = build(sk, sw, 0.5)
+    t = orc.table(sk, sw)
+    want = orc.probe(t, rk, rv)
+    orc.free(t)
+    for fanout in (32, 256):
+        assert probe_lds(ht, rk, rv, fanout=fanout) == (want["count"], want["sum"])
+    assert probe_direct(ht, rk, rv) == (want["count"], want["sum"])
 
 
 def test_lds_probe_with_filter_and_without_values(gpu, orc):

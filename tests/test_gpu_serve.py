@@ -1,7 +1,8 @@
 """GPU tests of the continuous-batching serving loop (mgv_serve_run): each
 request's result is the same alone (one request per batch, one batch at a
 time) and in the mix (Poisson arrivals, batches of many requests, several
-batches in flight), and equals the oracle; admission respects the HBM
+batches in flight), and equals the oracle; every result reaches the host
+through the device-written completion ring; admission respects the HBM
 budget."""
 from __future__ import annotations
 
@@ -62,6 +63,8 @@ def test_serve_mix_equals_alone_equals_oracle(tables, orc):
     assert all(mix[i].latency_s > 0 and mix[i].done_s >= mix[i].admit_s for i in range(n))
     assert st.p99_s >= st.p50_s > 0 and st.queries_per_s > 0
     assert st.batches < n                      # requests were batched together
+    # every result reached the host through the device-written completion ring
+    assert st.ring_results == n and st1.ring_results == n
 
 
 def test_serve_budget_limits_admission(tables, orc):

@@ -16,7 +16,7 @@ import mgv
 
 pytestmark = pytest.mark.gpu
 
-SCRATCH = 32 << 20   # MGV_PROBE_NOALLOC_SCRATCH_BYTES
+SCRATCH = 64 << 20   # MGV_PROBE_NOALLOC_SCRATCH_BYTES
 
 
 def dev(a):
