@@ -470,13 +470,16 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a, LdsAccA
 // ------------------------------------------------------------------ fill partitioning
 
 constexpr int kFillBlock = 1024;
-constexpr int kFillTile = 8192;
+constexpr int kFillTile = 8192;        // rows per tile of a scatter into up to 2048 partitions
+constexpr int kFillTileWide = 16384;   // ... into up to 512 (the staging takes 128 KiB of LDS)
+constexpr int kFillBlocksPerCU = 2;    // grid of the scatters: blocks per CU
 
-// One tile of rows of a fill scatter's input: the rows of segment `seg`
-// starting at `r0` (below `r1`).
+// One tile of rows of a fill scatter's input: the rows of its segment from
+// r0 (below r1), whose partitions are [base, base + S).
 struct FillTile {
     const int32_t *k, *v, *pred;
     uint64_t r0, r1;
+    uint32_t base;
 };
 
 // Scatters rows into a FillLayout by the top bits of their hash, tile by
@@ -485,47 +488,52 @@ struct FillTile {
 // (the part of a run past the region's capacity goes to the overflow area),
 // and the staged rows are written run by run.  Input: FROM_UP = false, n
 // rows of key (hashed here with fmix32), val and an optional filter pred <=
-// bound; FROM_UP = true, the regions and the overflow area of an upstream
-// FillLayout U (rows already hashed), whose partitions are coarser: a tile
-// then spans F / U.F partitions, so the runs are long.  The next tile's
-// columns load while the current one is sorted; the write step issues a
-// fixed number of stores per thread (lanes past the tile's last staged row
-// repeat its store; an empty tile stores into the padding after the
-// regions), so the compiler can wait for the loads without the stores.
-template <bool FROM_UP, uint32_t MAXF>
+// bound, and a tile spans all F partitions; FROM_UP = true, the regions of
+// an upstream FillLayout U (rows already hashed; its overflow area is probed
+// as it is and not passed on), whose partitions are coarser: region a holds
+// the rows whose top log2(U.F) hash bits are a, so its tiles span only the
+// S = F / U.F partitions [a S, a S + S), the per-tile arrays hold S entries
+// (MAXS bounds S) and the runs are long.  The next tile's columns load while
+// the current one is written; the write step issues a fixed number of stores
+// per thread (lanes past the tile's last staged row repeat its store; an
+// empty tile stores into the padding after the regions), so the compiler
+// can wait for the loads without the stores.  The next tile's loads are
+// issued after the run reservations (whose returned offsets are waited for
+// in issue order with every earlier memory operation) and the staging.
+template <bool FROM_UP, uint32_t MAXS, int TILE>
 __global__ __launch_bounds__(kFillBlock) void k_fill_scatter(FillLayout O, FillLayout U, const int32_t *key,
                                                              const int32_t *val, const int32_t *pred, int32_t bound,
                                                              uint64_t n) {
-    constexpr int BLOCK = kFillBlock, TILE = kFillTile;
+    constexpr int BLOCK = kFillBlock;
     constexpr int Q = TILE / (4 * BLOCK);
     constexpr int R = TILE / BLOCK;
-    constexpr int PER = MAXF > (uint32_t)BLOCK ? (int)(MAXF / BLOCK) : 1;
+    constexpr int PER = MAXS > (uint32_t)BLOCK ? (int)(MAXS / BLOCK) : 1;
     static_assert(Q >= 1 && TILE % (4 * BLOCK) == 0, "fill scatter shape");
-    __shared__ unsigned long long dbase[MAXF];   // slot i of partition p -> region row dbase[p] + i
-    __shared__ unsigned long long obase[MAXF];   // slot i past rlim[p] -> overflow row obase[p] + i
-    __shared__ uint32_t rlim[MAXF];              // slots below rlim[p] fit the region
-    __shared__ uint32_t tcnt[MAXF];
-    __shared__ uint16_t toff[MAXF];
+    __shared__ unsigned long long dbase[MAXS];   // slot i of local partition q -> region row dbase[q] + i
+    __shared__ unsigned long long obase[MAXS];   // slot i past rlim[q] -> overflow row obase[q] + i
+    __shared__ uint32_t rlim[MAXS];              // slots below rlim[q] fit the region
+    __shared__ uint32_t tcnt[MAXS];
+    __shared__ uint16_t toff[MAXS];
     __shared__ uint32_t wtot[BLOCK / kWave];
     __shared__ uint32_t s_total;
-    __shared__ uint32_t s_tiles[BLOCK + 1];      // FROM_UP: cumulative tiles per upstream segment
+    __shared__ uint32_t s_tiles[FROM_UP ? BLOCK + 1 : 1];   // cumulative tiles per upstream region
+    __shared__ uint32_t s_len[FROM_UP ? BLOCK : 1];         // rows of each upstream region
     __shared__ int32_t sk[TILE];
     __shared__ int32_t sv[TILE];
     const uint32_t F = O.F;
+    const uint32_t S = FROM_UP ? F / U.F : F;   // partitions a tile spans
     const uint32_t t = threadIdx.x;
-    const uint32_t per = (F + BLOCK - 1) / BLOCK;
-    for (uint32_t p = t; p < F; p += BLOCK) tcnt[p] = 0;
+    const uint32_t per = (S + BLOCK - 1) / BLOCK;
+    for (uint32_t q = t; q < S; q += BLOCK) tcnt[q] = 0;
     // the tiles of the input
     uint32_t ntiles;
     if (FROM_UP) {
-        const uint32_t nseg = U.F + 1;   // regions, then the overflow area (nseg <= BLOCK)
+        const uint32_t nseg = U.F;   // <= BLOCK
         uint64_t len = 0;
-        if (t < U.F) {
+        if (t < nseg) {
             const unsigned long long f = U.fill[t];
             len = f < U.cap ? f : U.cap;
-        } else if (t == U.F) {
-            const unsigned long long f = *U.ofill;
-            len = f < U.ocap ? f : U.ocap;
+            s_len[t] = (uint32_t)len;
         }
         const uint32_t nt = t < nseg ? (uint32_t)((len + TILE - 1) / TILE) : 0u;
         const uint32_t before = scan_block_excl<BLOCK>(nt, wtot);
@@ -539,32 +547,25 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_scatter(FillLayout O, FillL
     auto tile_of = [&](uint32_t ti) {
         FillTile ft;
         if (FROM_UP) {
-            // the segment s with s_tiles[s] <= ti < s_tiles[s + 1]
-            uint32_t a = 0, b = U.F;
+            // the region a with s_tiles[a] <= ti < s_tiles[a + 1]
+            uint32_t a = 0, b = U.F - 1;
             while (a < b) {
                 const uint32_t mid = (a + b + 1) / 2;
                 if (s_tiles[mid] <= ti) a = mid; else b = mid - 1;
             }
-            const uint64_t off = (uint64_t)(ti - s_tiles[a]) * TILE;
-            if (a < U.F) {
-                const unsigned long long f = U.fill[a];
-                ft.k = U.col[0] + (uint64_t)a * U.cap;
-                ft.v = U.col[1] + (uint64_t)a * U.cap;
-                ft.r1 = f < U.cap ? f : U.cap;
-            } else {
-                const unsigned long long f = *U.ofill;
-                ft.k = U.col[0] + U.ostart;
-                ft.v = U.col[1] + U.ostart;
-                ft.r1 = f < U.ocap ? f : U.ocap;
-            }
+            ft.k = U.col[0] + (uint64_t)a * U.cap;
+            ft.v = U.col[1] + (uint64_t)a * U.cap;
+            ft.r1 = s_len[a];
             ft.pred = nullptr;
-            ft.r0 = off;
+            ft.r0 = (uint64_t)(ti - s_tiles[a]) * TILE;
+            ft.base = a * S;
         } else {
             ft.k = key;
             ft.v = val;
             ft.pred = pred;
             ft.r0 = (uint64_t)ti * TILE;
             ft.r1 = n;
+            ft.base = 0;
         }
         if (ft.r1 > ft.r0 + TILE) ft.r1 = ft.r0 + TILE;
         return ft;
@@ -586,9 +587,11 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_scatter(FillLayout O, FillL
     };
     uint32_t ti = blockIdx.x;
     bool has_pred = false;
+    uint32_t nbase = 0;   // first partition of the loaded tile
     if (ti < ntiles) {
         const FillTile ft = tile_of(ti);
         has_pred = ft.pred != nullptr;
+        nbase = ft.base;
         load_tile(ft);
     }
 #pragma unroll
@@ -599,6 +602,7 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_scatter(FillLayout O, FillL
     const uint64_t dummy = (uint64_t)F * O.cap + 32;   // padding row: stores of an empty tile
     for (; ti < ntiles; ti += gridDim.x) {
         // this tile's rows as hashes, with their masks
+        const uint32_t base = nbase;
         v4i ch[Q], cv[Q];
         uint32_t cm[Q];
 #pragma unroll
@@ -617,7 +621,6 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_scatter(FillLayout O, FillL
                             (int32_t)fmix32((uint32_t)nk[q].z), (int32_t)fmix32((uint32_t)nk[q].w)};
             }
         }
-        if (ti + gridDim.x < ntiles) load_tile(tile_of(ti + gridDim.x));
         // 1. rank every row within its partition
         uint32_t rank[Q][4];
 #pragma unroll
@@ -625,7 +628,7 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_scatter(FillLayout O, FillL
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const bool ok = (cm[q] >> j) & 1u;
-                rank[q][j] = ok ? atomicAdd(&tcnt[part_of_hash(q_get(ch[q], j), F)], 1u) : 0u;
+                rank[q][j] = ok ? atomicAdd(&tcnt[part_of_hash(q_get(ch[q], j), F) - base], 1u) : 0u;
             }
         __syncthreads();
         // 2. owners: staging offsets, run reservations, overflow
@@ -633,30 +636,31 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_scatter(FillLayout O, FillL
         uint32_t mine = 0;
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
-            const uint32_t p = t * per + e;
-            c[e] = (e < (int)per && p < F) ? tcnt[p] : 0u;
+            const uint32_t q = t * per + e;
+            c[e] = (e < (int)per && q < S) ? tcnt[q] : 0u;
             mine += c[e];
         }
         uint32_t start = scan_block_excl<BLOCK>(mine, wtot);
         if (t == BLOCK - 1) s_total = start + mine;
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
-            const uint32_t p = t * per + e;
-            if (e < (int)per && p < F) {
-                toff[p] = (uint16_t)start;
+            const uint32_t q = t * per + e;
+            if (e < (int)per && q < S) {
+                toff[q] = (uint16_t)start;
                 if (c[e]) {
+                    const uint32_t p = base + q;
                     const unsigned long long off = atomic_add_global(&O.fill[p], (unsigned long long)c[e]);
                     const uint64_t room = off < O.cap ? O.cap - off : 0;
                     const uint32_t fit = room < c[e] ? (uint32_t)room : c[e];
-                    dbase[p] = (uint64_t)p * O.cap + off - start;
-                    rlim[p] = start + fit;
+                    dbase[q] = (uint64_t)p * O.cap + off - start;
+                    rlim[q] = start + fit;
                     if (fit < c[e]) {
                         const unsigned long long o =
                             atomic_add_global(O.ofill, (unsigned long long)(c[e] - fit));
-                        obase[p] = O.ostart + o - (start + fit);
+                        obase[q] = O.ostart + o - (start + fit);
                     }
                 }
-                tcnt[p] = 0;
+                tcnt[q] = 0;
                 start += c[e];
             }
         }
@@ -668,7 +672,7 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_scatter(FillLayout O, FillL
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const bool ok = (cm[q] >> j) & 1u;
-                pos[q][j] = toff[ok ? part_of_hash(q_get(ch[q], j), F) : 0u] + rank[q][j];
+                pos[q][j] = toff[ok ? part_of_hash(q_get(ch[q], j), F) - base : 0u] + rank[q][j];
             }
 #pragma unroll
         for (int q = 0; q < Q; ++q)
@@ -678,10 +682,19 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_scatter(FillLayout O, FillL
                     sk[pos[q][j]] = q_get(ch[q], j);
                     sv[pos[q][j]] = q_get(cv[q], j);
                 }
+        // the next tile's loads: issued after the reservations (whose returns
+        // would otherwise wait for them) and after this tile's rows are
+        // staged (their registers are free again), in flight while the tile
+        // is written
+        if (ti + gridDim.x < ntiles) {
+            const FillTile ft = tile_of(ti + gridDim.x);
+            nbase = ft.base;
+            load_tile(ft);
+        }
         __syncthreads();
         // 4. write the runs: exactly R stores per column and thread
         const uint32_t total = s_total;
-        constexpr int G = R < 8 ? R : 8;
+        constexpr int G = R < 8 ? R : (R > 8 ? 4 : 8);   // stores in flight per group: fewer for the wide tile
 #pragma unroll
         for (int r0 = 0; r0 < R; r0 += G) {
             int32_t k[G], v[G];
@@ -695,8 +708,8 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_scatter(FillLayout O, FillL
             uint64_t dst[G];
 #pragma unroll
             for (int r = 0; r < G; ++r) {
-                const uint32_t p = total ? part_of_hash(k[r], F) : 0u;
-                dst[r] = !total ? dummy : (ii[r] < rlim[p] ? dbase[p] : obase[p]) + ii[r];
+                const uint32_t q = total ? part_of_hash(k[r], F) - base : 0u;
+                dst[r] = !total ? dummy : (ii[r] < rlim[q] ? dbase[q] : obase[q]) + ii[r];
             }
 #pragma unroll
             for (int r = 0; r < G; ++r) {
@@ -708,23 +721,31 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_scatter(FillLayout O, FillL
     }
 }
 
-// Probes hashed rows against the global table (the overflow area of a fill
-// partitioning, rows whose partition was full): one row per lane, its chain
-// followed bucket by bucket as in mgv_hashjoin_probe.
+// Rows of a fill partitioning's overflow area (rows whose region was full):
+// `*nrows` rows (at most ncap) of hashes H and values V.
+struct HashedRows {
+    const int32_t *H, *V;
+    const unsigned long long *nrows;
+    uint64_t ncap;
+};
+
+// Probes hashed rows against the global table: the overflow areas of both
+// levels of a fill partitioning, one row per lane, its chain followed bucket
+// by bucket as in mgv_hashjoin_probe.
 template <class Acc>
-__global__ __launch_bounds__(256) void k_probe_hashed(const uint64_t *slots, uint32_t nb, const int32_t *H,
-                                                      const int32_t *V, const unsigned long long *nrows, uint64_t ncap,
+__global__ __launch_bounds__(256) void k_probe_hashed(const uint64_t *slots, uint32_t nb, HashedRows A, HashedRows B,
                                                       LdsAccArgs aa) {
     __shared__ __align__(16) typename Acc::Smem s_acc;
     Acc acc;
     acc.open(&s_acc, aa);
-    const unsigned long long f = *nrows;
-    const uint64_t n = f < ncap ? f : ncap;
+    const unsigned long long fa = A.nrows ? *A.nrows : 0ull, fb = B.nrows ? *B.nrows : 0ull;
+    const uint64_t na = fa < A.ncap ? fa : A.ncap;
+    const uint64_t n = na + (fb < B.ncap ? fb : B.ncap);
     for (uint64_t i0 = (uint64_t)blockIdx.x * 256; i0 < n; i0 += (uint64_t)gridDim.x * 256) {
         const uint64_t i = i0 + threadIdx.x;
         if (i < n) {
-            const uint32_t h = (uint32_t)H[i];
-            const int32_t v = V[i];
+            const uint32_t h = (uint32_t)(i < na ? A.H[i] : B.H[i - na]);
+            const int32_t v = i < na ? A.V[i] : B.V[i - na];
             const uint32_t home = bucket_of_hash(h, nb);
             uint32_t b = home;
             for (;;) {
@@ -1000,18 +1021,34 @@ static int probe_lds_fill(const uint64_t *slots, uint32_t nbuckets, const int32_
     if ((rc = mgv_fill_u64((uint64_t *)ws, counters / 8, 0, (mgv_stream_t)s))) return rc;
     DeviceState *ds = current_state();
     const uint32_t cus = (ds && ds->cus > 0) ? (uint32_t)ds->cus : 256u;
-    const uint64_t tiles = (n + kFillTile - 1) / kFillTile;
-    const uint32_t grid1 = (uint32_t)(tiles < 2ull * cus ? tiles : 2ull * cus);
+    const uint64_t gmax = (uint64_t)kFillBlocksPerCU * cus;
+    // blocks for a scatter of up to `rows` rows in tiles of `tile` (plus one
+    // partial tile per upstream region)
+    auto grid_for = [&](uint64_t rows, int tile, uint32_t regions) {
+        const uint64_t t = (rows + tile - 1) / tile + regions;
+        return (uint32_t)(t < gmax ? t : gmax);
+    };
     if (L1.F) {
-        hipLaunchKernelGGL((k_fill_scatter<false, kMaxFanout>), dim3(grid1), dim3(kFillBlock), 0, s, L1, L1, keys,
-                           vals, pred, bound, n);
+        hipLaunchKernelGGL((k_fill_scatter<false, kMaxFanout, kFillTileWide>), dim3(grid_for(n, kFillTileWide, 0)),
+                           dim3(kFillBlock), 0, s, L1, L1, keys, vals, pred, bound, n);
         if ((rc = check_launch("k_fill_scatter (level 1)"))) return rc;
-        hipLaunchKernelGGL((k_fill_scatter<true, kMaxFanoutWide>), dim3(grid1), dim3(kFillBlock), 0, s, L2, L1,
-                           nullptr, nullptr, nullptr, 0, 0ull);
+        // a level-2 tile spans F2 / F1 partitions: small per-tile arrays leave
+        // LDS for the wide tile
+        const uint32_t S = F2 / fanout;
+        if (S <= 64)
+            hipLaunchKernelGGL((k_fill_scatter<true, 64, kFillTileWide>), dim3(grid_for(n, kFillTileWide, fanout)),
+                               dim3(kFillBlock), 0, s, L2, L1, nullptr, nullptr, nullptr, 0, 0ull);
+        else if (S <= 256)
+            hipLaunchKernelGGL((k_fill_scatter<true, 256, kFillTileWide>), dim3(grid_for(n, kFillTileWide, fanout)),
+                               dim3(kFillBlock), 0, s, L2, L1, nullptr, nullptr, nullptr, 0, 0ull);
+        else
+            hipLaunchKernelGGL((k_fill_scatter<true, kMaxFanoutWide / 2, kFillTile>),
+                               dim3(grid_for(n, kFillTile, fanout)), dim3(kFillBlock), 0, s, L2, L1, nullptr, nullptr,
+                               nullptr, 0, 0ull);
         if ((rc = check_launch("k_fill_scatter (level 2)"))) return rc;
     } else {
-        hipLaunchKernelGGL((k_fill_scatter<false, kMaxFanoutWide>), dim3(grid1), dim3(kFillBlock), 0, s, L2, L2,
-                           keys, vals, pred, bound, n);
+        hipLaunchKernelGGL((k_fill_scatter<false, kMaxFanoutWide, kFillTile>), dim3(grid_for(n, kFillTile, 0)),
+                           dim3(kFillBlock), 0, s, L2, L2, keys, vals, pred, bound, n);
         if ((rc = check_launch("k_fill_scatter"))) return rc;
     }
     static thread_local int cached = 0;
@@ -1029,9 +1066,12 @@ static int probe_lds_fill(const uint64_t *slots, uint32_t nbuckets, const int32_
     a.next = (uint32_t *)ws;
     hipLaunchKernelGGL((k_lds_probe<NB, Acc>), dim3(grid), dim3(kLdsBlock), 0, s, a, aa);
     if ((rc = check_launch("k_lds_probe"))) return rc;
-    // rows whose partition was full (none for a uniform hash partition)
-    hipLaunchKernelGGL(k_probe_hashed<Acc>, dim3(cus), dim3(256), 0, s, slots, nbuckets, L2.col[0] + L2.ostart,
-                       L2.col[1] + L2.ostart, L2.ofill, L2.ocap, aa);
+    // rows whose region was full at either level (none for a uniform hash
+    // partition)
+    const HashedRows o2 = {L2.col[0] + L2.ostart, L2.col[1] + L2.ostart, L2.ofill, L2.ocap};
+    const HashedRows o1 = L1.F ? HashedRows{L1.col[0] + L1.ostart, L1.col[1] + L1.ostart, L1.ofill, L1.ocap}
+                               : HashedRows{nullptr, nullptr, nullptr, 0};
+    hipLaunchKernelGGL(k_probe_hashed<Acc>, dim3(cus), dim3(256), 0, s, slots, nbuckets, o2, o1, aa);
     return check_launch("k_probe_hashed");
 }
 

@@ -97,11 +97,13 @@ def test_lds_probe_fill_and_counted_partitionings_agree(gpu, orc, nb_keys, n, fa
 
 
 @pytest.mark.parametrize("kind", ["one_key", "zipf"])
-def test_lds_probe_skewed_keys_overflow_their_regions(gpu, orc, kind):
+@pytest.mark.parametrize("nb_keys,n", [(1 << 16, 1 << 20), (1 << 20, 1 << 21)])
+def test_lds_probe_skewed_keys_overflow_their_regions(gpu, orc, kind, nb_keys, n):
     """Skewed probe keys overfill their partitions' regions: the excess rows
-    go to the overflow areas (level 1 into level 2, level 2 to the hashed
-    global-table probe) and the result still equals the oracle's."""
-    nb_keys, n = 1 << 16, 1 << 20
+    go to the overflow areas, which the hashed global-table probe reads, and
+    the result still equals the oracle's.  2^16 keys (2^14 buckets) take one
+    level of partitions at both fan-outs; 2^20 keys (2^18 buckets) take two
+    at fan-out 32 (level-1 overflow and level-2 overflow) and one at 256."""
     sk = orc.gen_perm(nb_keys, nb_keys, orc.seed(2, 0), 13)
     sw = orc.gen_bits(nb_keys, orc.seed(2, 1), 13, 16)
     rv = orc.gen_bits(n, orc.seed(2, 3), 13, 16)
