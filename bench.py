@@ -29,6 +29,7 @@ import mgv  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
 MGV_LAUNCH_COUNTED_PART = 1 << 12
+MGV_LAUNCH_KEY_SLICES = 1 << 13
 
 
 def log(*a):
@@ -142,12 +143,13 @@ def leg_c2(args, st):
     def make_probe(mode, fanout):
         """mode "direct" (one pass over R), "part" (one-level partitioned,
         fan-out F), "lds" (two-level count-free partitioning, slices probed
-        from LDS) or "ldsc" (the same with counted partitionings)."""
+        from LDS), "ldsc" (the same with counted partitionings) or "ldsk"
+        (count-free, slices held as 32-bit key hashes instead of slot tags)."""
         nonlocal ws
         need = C.c_uint64()
         if mode == "part":
             mgv.call("mgv_partition_workspace_size", nr, fanout, 2, C.byref(need))
-        elif mode in ("lds", "ldsc"):
+        elif mode in ("lds", "ldsc", "ldsk"):
             mgv.call("mgv_probe_lds_workspace_size", nr, ht.nbuckets, fanout, C.byref(need))
         if need.value and (ws is None or ws.nbytes < need.value):
             if ws is not None:
@@ -156,8 +158,9 @@ def leg_c2(args, st):
         if mode == "part":
             return lambda: mgv.call("mgv_hashjoin_probe_partitioned", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp,
                                     nr, 0, 0, fanout, ws.vp, ws.nbytes, res.vp, L, st.vp)
-        if mode in ("lds", "ldsc"):
-            Lm = Llds if mode == "lds" else mgv.Launch(0, 1024, -1, Llds.reserved | MGV_LAUNCH_COUNTED_PART)
+        if mode in ("lds", "ldsc", "ldsk"):
+            flag = {"lds": 0, "ldsc": MGV_LAUNCH_COUNTED_PART, "ldsk": MGV_LAUNCH_KEY_SLICES}[mode]
+            Lm = mgv.Launch(0, 1024, -1, Llds.reserved | flag)
             return lambda: mgv.call("mgv_hashjoin_probe_lds", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp, nr,
                                     fanout, ws.vp, ws.nbytes, res.vp, Lm, st.vp)
         return lambda: mgv.call("mgv_hashjoin_probe", ht.buf.vp, ht.nbuckets, None, 0, rk.vp, rv.vp, nr, 0, 0,
@@ -167,7 +170,7 @@ def leg_c2(args, st):
     if args.probe != "auto":
         candidates = [(args.probe, args.fanout if args.fanout > 0 else (1 if args.probe == "direct" else 64))]
     else:
-        candidates = [("direct", 1), ("part", 64), ("ldsc", 32), ("lds", 32), ("lds", 64)]
+        candidates = [("direct", 1), ("part", 64), ("ldsc", 32), ("ldsk", 32), ("lds", 32), ("lds", 64)]
         if "mode" in t2 and (t2["mode"], int(t2["fanout"])) not in candidates:
             candidates.append((t2["mode"], int(t2["fanout"])))
     sweep, sweep_res = {}, {}
@@ -941,7 +944,7 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="probe workgroup size (0 = tuned, else 256)")
     ap.add_argument("--coop", type=int, default=-1, help="4-lane cooperative probe (-1 = tuned)")
     ap.add_argument("--load-factor", type=float, default=0.5)
-    ap.add_argument("--probe", default="auto", choices=["auto", "direct", "part", "lds", "ldsc"],
+    ap.add_argument("--probe", default="auto", choices=["auto", "direct", "part", "lds", "ldsc", "ldsk"],
                     help="probe path (auto = time the candidates, keep the fastest)")
     ap.add_argument("--fanout", type=int, default=-1, help="fan-out of the partitioned paths (with --probe)")
     ap.add_argument("--scatter", type=int, default=-1, help="scatter workgroup shape MGV_SCATTER_* (-1 = tuned)")

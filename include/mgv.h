@@ -137,6 +137,11 @@ typedef struct mgv_launch {
  * exact partition sizes instead of the default count-free partitioning into
  * fixed-capacity regions (same results; kept for comparison). */
 #define MGV_LAUNCH_COUNTED_PART (1u << 12)
+/* Launch flag of mgv_hashjoin_probe_lds / mgv_hashjoin_probe_groupby_lds:
+ * hold the slices as 32-bit key hashes instead of the 16-bit slot tags used
+ * by default for power-of-two tables of at least 2^18 buckets (same
+ * results; kept for comparison). */
+#define MGV_LAUNCH_KEY_SLICES (1u << 13)
 
 /* ------------------------------------------------------------ datagen */
 
@@ -463,7 +468,11 @@ int mgv_hashjoin_build_lds(uint64_t *slots, uint32_t nbuckets, const int32_t *ke
  * 32 bytes per row with their overflow areas).  The table must come from
  * mgv_hashjoin_build or a batch build (mgv_batch_run), which fill each
  * bucket's slots in order: the LDS copy of a slice keeps per bucket only
- * the keys, the payloads and the number of filled slots.  Replaces
+ * the keys, the payloads and the number of filled slots.  For a table of
+ * 2^B >= 2^18 buckets the copy holds per slot a 16-bit tag instead of the
+ * key: the slot's chain offset (B - 16 bits) and the low 32 - B bits of its
+ * key's hash, which together with the bucket identify the key exactly
+ * (chains longer than the offset field continue in global memory).  Replaces
  * hashjoin::consume_warp (src/operators/hashjoin.cu:135-182) fused with
  * aggregation (src/operators/aggregation.cu:27-66). */
 int mgv_probe_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes);

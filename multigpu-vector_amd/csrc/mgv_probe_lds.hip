@@ -37,6 +37,25 @@ constexpr int kLdsBlock = 1024;
 constexpr int kAhead = 1;
 constexpr uint64_t kQuadStride = 4ull * kLdsBlock;   // rows between a lane's quads
 
+// Tagged slices (tables of nb = 2^B buckets, B >= 18): a bucket's home is
+// the top B bits of a key's hash, so a slot of bucket b holding a key of
+// hash hk is identified exactly by its chain offset j = b - (hk >> L)
+// (L = 32 - B) and the low L bits of hk.  With J = B - 16 bits for j, the
+// tag (j << L) | (hk & (2^L - 1)) fits 16 bits while j <= 2^J - 2; a row of
+// hash h at bucket b computes its own tag with j = b - (h >> L), and a slot
+// hits exactly when the tags are equal.  No row computes a tag whose j bits
+// are all ones: 0xFFFF marks an empty slot and 0xFFFE a slot whose j does
+// not fit (a row reaches it only at j > 2^J - 2 and continues in global
+// memory).
+struct TagGeom {
+    uint32_t L;         // low hash bits in a tag
+    uint32_t lowmask;   // 2^L - 1
+    uint32_t jmax;      // largest chain offset a tag holds: 2^(16 - L) - 2
+};
+
+constexpr uint32_t kTagEmpty = 0xFFFFu;
+constexpr uint32_t kTagFar = 0xFFFEu;
+
 struct LdsProbeArgs {
     const uint64_t *slots;
     uint32_t nbuckets;
@@ -48,6 +67,7 @@ struct LdsProbeArgs {
     uint64_t cap;
     const int32_t *fcol[2];  // its hash and value columns
     uint32_t *next;          // partition counter (zeroed before the launch)
+    TagGeom geom;            // tagged slices only
 };
 
 // Count-free partitioning: F regions of `cap` rows each, filled by
@@ -97,6 +117,8 @@ __device__ __forceinline__ void bucket_scan(const v4u (&q)[4], uint32_t h_key, u
 
 __device__ __forceinline__ bool bucket_full(const v4u (&q)[4]) { return !(q[3].z == ~0u && q[3].w == ~0u); }
 
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
 // A table slice in LDS is stored keys first: per bucket its 8 keys (hashed,
 // in two 16-byte quads), its 8 payloads, and its fill count c.  The build fills a
 // bucket's slots strictly in order (a slot is claimed only after every lower
@@ -113,6 +135,18 @@ struct SliceT {
     v4u key[(NB + kSliceExtra) * 2];
     uint32_t pay[(NB + kSliceExtra) * 8];
     uint8_t cnt[NB + kSliceExtra];
+
+    // one bucket's keys and fill count, as read for a probe
+    struct Bucket {
+        v4u k0, k1;
+        uint32_t c;
+    };
+    __device__ __forceinline__ Bucket fetch(uint32_t idx) const;
+    // slot mask of the hits of the row of hash h at chain offset j
+    __device__ __forceinline__ static uint32_t hits(const Bucket &b, uint32_t h, uint32_t j, const TagGeom &g);
+    __device__ __forceinline__ static bool full(const Bucket &b) { return b.c == 8; }
+    // whether a row at chain offset j is probed in LDS (always)
+    __device__ __forceinline__ static bool in_range(uint32_t, const TagGeom &) { return true; }
 };
 
 // LDS quad of key quad w (slots 4w..4w+3) of slice bucket idx.  Buckets' key
@@ -128,6 +162,47 @@ __device__ __forceinline__ uint32_t slice_hits(const v4u &k0, const v4u &k1, uin
                        (uint32_t)(k1.z == key) << 6 | (uint32_t)(k1.w == key) << 7;
     return m & ((1u << c) - 1u);
 }
+
+template <uint32_t NB>
+__device__ __forceinline__ typename SliceT<NB>::Bucket SliceT<NB>::fetch(uint32_t idx) const {
+    return Bucket{key[key_at(idx, 0)], key[key_at(idx, 1)], cnt[idx]};
+}
+
+template <uint32_t NB>
+__device__ __forceinline__ uint32_t SliceT<NB>::hits(const Bucket &b, uint32_t h, uint32_t, const TagGeom &) {
+    return slice_hits(b.k0, b.k1, b.c, h);
+}
+
+// A tagged slice: per bucket its 8 slot tags in 16 bytes (dword w holds
+// slot w in its low half and slot w + 4 in its high half) and its 8
+// payloads.  The build fills a bucket's slots in order and never frees one,
+// so the bucket is full exactly when slot 7 is not empty.
+template <uint32_t NB>
+struct SliceTag {
+    static constexpr uint32_t kBuckets = NB;
+    v4u tag[NB + kSliceExtra];
+    uint32_t pay[(NB + kSliceExtra) * 8];
+
+    struct Bucket {
+        v4u t;
+    };
+    __device__ __forceinline__ Bucket fetch(uint32_t idx) const { return Bucket{tag[idx]}; }
+    // the row's tag against the 8 slot tags: xor, then per 16-bit half
+    // 1 - x saturated at 0 (1 exactly when the tags are equal)
+    __device__ __forceinline__ static uint32_t hits(const Bucket &b, uint32_t h, uint32_t j, const TagGeom &g) {
+        const uint32_t r = (j << g.L) | (h & g.lowmask);
+        const uint32_t R = r | (r << 16);
+        const u16x2 one = {1, 1};
+        const uint32_t d0 = __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(one, __builtin_bit_cast(u16x2, b.t.x ^ R)));
+        const uint32_t d1 = __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(one, __builtin_bit_cast(u16x2, b.t.y ^ R)));
+        const uint32_t d2 = __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(one, __builtin_bit_cast(u16x2, b.t.z ^ R)));
+        const uint32_t d3 = __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(one, __builtin_bit_cast(u16x2, b.t.w ^ R)));
+        const uint32_t m = d0 | d1 << 1 | d2 << 2 | d3 << 3;   // slot w at bit w, slot w + 4 at bit 16 + w
+        return (m & 0xFu) | ((m >> 12) & 0xF0u);
+    }
+    __device__ __forceinline__ static bool full(const Bucket &b) { return b.t.w < (kTagEmpty << 16); }
+    __device__ __forceinline__ static bool in_range(uint32_t j, const TagGeom &g) { return j <= g.jmax; }
+};
 
 __device__ __forceinline__ uint32_t lowest_slot(uint32_t hm) { return hm ? (uint32_t)__builtin_ctz(hm) : 0u; }
 
@@ -261,11 +336,12 @@ __device__ __forceinline__ uint32_t queue_push(WaveQueue &Q, uint32_t tail, bool
 
 // Probes up to 64 queued entries (one per lane) and re-queues the chains
 // that go on; returns the new head and, through tail, the new tail.  A
-// bucket inside the loaded range is read from LDS, any other from the table.
+// bucket inside the loaded range is read from LDS (for a tagged slice: while
+// the chain offset fits a tag), any other from the table.
 template <class S, class Acc>
 __device__ __forceinline__ uint32_t queue_round(WaveQueue &Q, uint32_t head, uint32_t &tail, const S &sl,
                                                 const uint64_t *slots, uint32_t nb, uint32_t lo, uint32_t nload,
-                                                Acc &acc) {
+                                                const TagGeom &g, Acc &acc) {
     const uint32_t avail = tail - head;
     const uint32_t take = avail < kWave ? avail : kWave;
     const uint32_t l = lane_id();
@@ -279,44 +355,46 @@ __device__ __forceinline__ uint32_t queue_round(WaveQueue &Q, uint32_t head, uin
         bkt = Q.bkt[pos];
     }
     wave_sync();   // every lane has read its entry before slots are reused
+    const uint32_t home = bucket_of_hash(key, nb);
     const uint32_t idx = bkt - lo;
-    const bool in_lds = mine && idx < nload;
+    const uint32_t j = bkt - home;   // the chain offset (huge for a chain that wrapped: never in LDS)
+    const bool in_lds = mine && idx < nload && S::in_range(j, g);
     bool full = false;
     uint32_t hm = 0, ii = 0;
     if (in_lds) {
-        const v4u k0 = sl.key[key_at(idx, 0)], k1 = sl.key[key_at(idx, 1)];
-        const uint32_t c = sl.cnt[idx];
-        hm = slice_hits(k0, k1, c, key);
+        const typename S::Bucket B = sl.fetch(idx);
+        hm = S::hits(B, key, j, g);
         ii = idx;
-        full = c == 8;
+        full = S::full(B);
     }
     acc.lds(sl, ii, hm, (int32_t)sl.pay[ii * 8 + lowest_slot(hm)], val);
     if (mine && !in_lds) {
-        const v4u *g = reinterpret_cast<const v4u *>(slots + (uint64_t)bkt * 8);
+        const v4u *gb = reinterpret_cast<const v4u *>(slots + (uint64_t)bkt * 8);
         v4u q[4];
 #pragma unroll
-        for (int w = 0; w < 4; ++w) q[w] = load_global(g + w);
+        for (int w = 0; w < 4; ++w) q[w] = load_global(gb + w);
         acc.bucket(q, key, val);
         full = bucket_full(q);
     }
     // a chain ends at a bucket with an empty slot, or after visiting all nb
     // buckets once (when its next bucket is its home bucket again)
     const uint32_t nxt = (bkt + 1 == nb) ? 0 : bkt + 1;
-    const bool more = mine && full && nxt != bucket_of_hash(key, nb);
+    const bool more = mine && full && nxt != home;
     head += take;
     tail = queue_push(Q, tail, more, key, val, nxt);
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)head);
 }
 
-// A lane's four rows: the four home buckets' keys and counts are read from
-// LDS together (12 reads in flight) and compared, then the four lowest-hit
-// payloads are read together; rows whose bucket is full continue at the next
-// bucket, and rows whose home bucket lies outside the slice start at their
-// home bucket, through the wave's queue.
+// A lane's four rows: the four home buckets are read from LDS together and
+// compared, then the four lowest-hit payloads are read together; rows whose
+// bucket is full continue at the next bucket, and rows whose home bucket
+// lies outside the slice start at their home bucket, through the wave's
+// queue.
 template <class S, class Acc>
 __device__ __forceinline__ void lds_probe_quad(const v4i &k, const v4i &v, uint32_t m, const S &sl,
                                                const uint64_t *slots, uint32_t nb, uint32_t lo, uint32_t nload,
-                                               Acc &acc, WaveQueue &Q, uint32_t &head, uint32_t &tail) {
+                                               const TagGeom &g, Acc &acc, WaveQueue &Q, uint32_t &head,
+                                               uint32_t &tail) {
     uint32_t b[4], ii[4];
     bool ins[4];
 #pragma unroll
@@ -326,18 +404,13 @@ __device__ __forceinline__ void lds_probe_quad(const v4i &k, const v4i &v, uint3
         ins[j] = ((m >> j) & 1u) && idx < nload;
         ii[j] = ins[j] ? idx : 0u;
     }
-    v4u k0[4], k1[4];
-    uint32_t c[4];
+    typename S::Bucket B[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        k0[j] = sl.key[key_at(ii[j], 0)];
-        k1[j] = sl.key[key_at(ii[j], 1)];
-        c[j] = sl.cnt[ii[j]];
-    }
+    for (int j = 0; j < 4; ++j) B[j] = sl.fetch(ii[j]);
     uint32_t hm[4];
     int32_t p1[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) hm[j] = ins[j] ? slice_hits(k0[j], k1[j], c[j], (uint32_t)q_get(k, j)) : 0u;
+    for (int j = 0; j < 4; ++j) hm[j] = ins[j] ? S::hits(B[j], (uint32_t)q_get(k, j), 0u, g) : 0u;
 #pragma unroll
     for (int j = 0; j < 4; ++j) p1[j] = (int32_t)sl.pay[ii[j] * 8 + lowest_slot(hm[j])];
 #pragma unroll
@@ -345,11 +418,11 @@ __device__ __forceinline__ void lds_probe_quad(const v4i &k, const v4i &v, uint3
         acc.lds(sl, ii[j], hm[j], p1[j], q_get(v, j));
         const bool live = (m >> j) & 1u;
         const uint32_t from = ins[j] ? ((b[j] + 1 == nb) ? 0 : b[j] + 1) : b[j];
-        const bool go = live && (!ins[j] || (c[j] == 8 && from != b[j]));
+        const bool go = live && (!ins[j] || (S::full(B[j]) && from != b[j]));
         tail = queue_push(Q, tail, go, (uint32_t)q_get(k, j), q_get(v, j), from);
         // fewer than kWave entries were queued before this push and at most
         // kWave were added, so the queue never holds more than kQueue - 1
-        while (tail - head >= kWave) head = queue_round(Q, head, tail, sl, slots, nb, lo, nload, acc);
+        while (tail - head >= kWave) head = queue_round(Q, head, tail, sl, slots, nb, lo, nload, g, acc);
     }
 }
 
@@ -369,8 +442,9 @@ __device__ __forceinline__ uint32_t load_rows4(const int32_t *K, const int32_t *
 // Copies slice buckets [lo, lo + nload) of the table into the slice, one
 // bucket per thread: keys, payloads and the fill count (the first empty
 // slot, or 8).
-template <class S>
-__device__ __forceinline__ void load_slice(S &sl, const uint64_t *slots, uint32_t lo, uint32_t nload) {
+template <uint32_t NB>
+__device__ __forceinline__ void load_slice(SliceT<NB> &sl, const uint64_t *slots, uint32_t lo, uint32_t nload,
+                                           uint32_t, const TagGeom &) {
     for (uint32_t i = threadIdx.x; i < nload; i += kLdsBlock) {
         const v4u *g = reinterpret_cast<const v4u *>(slots + (uint64_t)(lo + i) * 8);
         const v4u q0 = load_global(g), q1 = load_global(g + 1), q2 = load_global(g + 2), q3 = load_global(g + 3);
@@ -392,9 +466,43 @@ __device__ __forceinline__ void load_slice(S &sl, const uint64_t *slots, uint32_
     }
 }
 
-template <uint32_t NB, class Acc>
+// The tagged copy: slot s < c of global bucket b with key k gets the tag of
+// fmix32(k) at chain offset j = (b - home) mod nb (kTagFar when j does not
+// fit), slots from c on kTagEmpty.
+template <uint32_t NB>
+__device__ __forceinline__ void load_slice(SliceTag<NB> &sl, const uint64_t *slots, uint32_t lo, uint32_t nload,
+                                           uint32_t nb, const TagGeom &tg) {
+    for (uint32_t i = threadIdx.x; i < nload; i += kLdsBlock) {
+        const uint32_t b = lo + i;
+        const v4u *g = reinterpret_cast<const v4u *>(slots + (uint64_t)b * 8);
+        const v4u q0 = load_global(g), q1 = load_global(g + 1), q2 = load_global(g + 2), q3 = load_global(g + 3);
+        const v4u qs[4] = {q0, q1, q2, q3};
+        uint32_t c = 8;
+#pragma unroll
+        for (int s = 7; s >= 0; --s) {
+            const v4u &q = qs[s >> 1];
+            const bool empty = (s & 1) ? (q.z == ~0u && q.w == ~0u) : (q.x == ~0u && q.y == ~0u);
+            c = empty ? (uint32_t)s : c;
+        }
+        uint32_t t[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const v4u &q = qs[s >> 1];
+            const uint32_t hk = fmix32((s & 1) ? q.z : q.x);
+            const uint32_t j = (b - (hk >> tg.L)) & (nb - 1u);
+            const uint32_t tag = j <= tg.jmax ? (j << tg.L) | (hk & tg.lowmask) : kTagFar;
+            t[s] = (uint32_t)s < c ? tag : kTagEmpty;
+        }
+        sl.tag[i] = v4u{t[0] | t[4] << 16, t[1] | t[5] << 16, t[2] | t[6] << 16, t[3] | t[7] << 16};
+        v4u *pay4 = reinterpret_cast<v4u *>(sl.pay);
+        pay4[i * 2] = v4u{q0.y, q0.w, q1.y, q1.w};
+        pay4[i * 2 + 1] = v4u{q2.y, q2.w, q3.y, q3.w};
+    }
+}
+
+template <class Slice, class Acc>
 __global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a, LdsAccArgs aa) {
-    __shared__ __align__(16) SliceT<NB> S;
+    __shared__ __align__(16) Slice S;
     __shared__ WaveQueue s_queue[kLdsBlock / kWave];
     __shared__ __align__(16) typename Acc::Smem s_acc;
     __shared__ uint32_t s_part;
@@ -432,7 +540,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a, LdsAccA
             const uint64_t ix = i0 + (uint64_t)x * kQuadStride;
             m[x] = load_rows4(K, V, ix < r1 ? ix : r0, ix < r1 ? r1 : r0, k[x], v[x]);
         }
-        load_slice(S, a.slots, lo, nload);
+        load_slice(S, a.slots, lo, nload, nb, a.geom);
         __syncthreads();
         // r0 is a multiple of 64 rows, so quads are 16-byte aligned
         uint32_t head = 0, tail = 0;   // the wave's queue (wave-uniform counters)
@@ -451,7 +559,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a, LdsAccA
             }
 #pragma unroll
             for (int x = 0; x < kAhead; ++x)
-                lds_probe_quad(k[x], v[x], m[x], S, a.slots, nb, lo, nload, acc, Q, head, tail);
+                lds_probe_quad(k[x], v[x], m[x], S, a.slots, nb, lo, nload, a.geom, acc, Q, head, tail);
 #pragma unroll
             for (int x = 0; x < kAhead; ++x) {
                 k[x] = kn[x];
@@ -461,7 +569,7 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_probe(LdsProbeArgs a, LdsAccA
             i0 = i1;
         }
         // the rest of the queue, before the slice is replaced
-        while (tail != head) head = queue_round(Q, head, tail, S, a.slots, nb, lo, nload, acc);
+        while (tail != head) head = queue_round(Q, head, tail, S, a.slots, nb, lo, nload, a.geom, acc);
         __syncthreads();
     }
     acc.close(aa);
@@ -1000,6 +1108,18 @@ static int probe_ws_need(uint64_t n, uint32_t nb, uint32_t F1, uint64_t *need) {
     return MGV_OK;
 }
 
+// Tag geometry of a table of nbuckets = 2^B buckets, 18 <= B <= 31 (false
+// for any other table, which keeps key slices).
+static bool tag_geometry(uint32_t nbuckets, TagGeom *g) {
+    if (nbuckets & (nbuckets - 1)) return false;
+    const uint32_t B = (uint32_t)__builtin_ctz(nbuckets);
+    if (B < 18 || B > 31) return false;
+    g->L = 32 - B;
+    g->lowmask = (1u << g->L) - 1u;
+    g->jmax = (1u << (B - 16)) - 2u;
+    return true;
+}
+
 // The fill-partitioned probe: no count passes; level 1 by fmix32(key) into
 // fanout regions, level 2 by the same hash bits into the slices' regions,
 // then the LDS probe per slice (slices of at most NB buckets) and the hashed
@@ -1051,9 +1171,6 @@ static int probe_lds_fill(const uint64_t *slots, uint32_t nbuckets, const int32_
                            dim3(kFillBlock), 0, s, L2, L2, keys, vals, pred, bound, n);
         if ((rc = check_launch("k_fill_scatter"))) return rc;
     }
-    static thread_local int cached = 0;
-    if (!cached) cached = persistent_blocks(k_lds_probe<NB, Acc>, kLdsBlock);
-    const uint32_t grid = LC.grid ? LC.grid : (uint32_t)cached;
     LdsProbeArgs a = {};
     a.slots = slots;
     a.nbuckets = nbuckets;
@@ -1064,7 +1181,17 @@ static int probe_lds_fill(const uint64_t *slots, uint32_t nbuckets, const int32_
     a.fcol[0] = L2.col[0];
     a.fcol[1] = L2.col[1];
     a.next = (uint32_t *)ws;
-    hipLaunchKernelGGL((k_lds_probe<NB, Acc>), dim3(grid), dim3(kLdsBlock), 0, s, a, aa);
+    if (!(LC.reserved & MGV_LAUNCH_KEY_SLICES) && tag_geometry(nbuckets, &a.geom)) {
+        static thread_local int cached = 0;
+        if (!cached) cached = persistent_blocks(k_lds_probe<SliceTag<NB>, Acc>, kLdsBlock);
+        hipLaunchKernelGGL((k_lds_probe<SliceTag<NB>, Acc>), dim3(LC.grid ? LC.grid : (uint32_t)cached),
+                           dim3(kLdsBlock), 0, s, a, aa);
+    } else {
+        static thread_local int cached = 0;
+        if (!cached) cached = persistent_blocks(k_lds_probe<SliceT<NB>, Acc>, kLdsBlock);
+        hipLaunchKernelGGL((k_lds_probe<SliceT<NB>, Acc>), dim3(LC.grid ? LC.grid : (uint32_t)cached),
+                           dim3(kLdsBlock), 0, s, a, aa);
+    }
     if ((rc = check_launch("k_lds_probe"))) return rc;
     // rows whose region was full at either level (none for a uniform hash
     // partition)
@@ -1109,7 +1236,7 @@ static int probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pr
     uint32_t *next = (uint32_t *)ws;
     if ((rc = mgv_fill_u32(next, 1, 0, (mgv_stream_t)s))) return rc;
     static thread_local int cached = 0;
-    if (!cached) cached = persistent_blocks(k_lds_probe<kSliceMax, AccSum>, kLdsBlock);
+    if (!cached) cached = persistent_blocks(k_lds_probe<SliceT<kSliceMax>, AccSum>, kLdsBlock);
     const uint32_t grid = LC.grid ? LC.grid : (uint32_t)cached;
     LdsProbeArgs a = {};
     a.slots = slots;
@@ -1119,7 +1246,7 @@ static int probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pr
     a.nparts = L2.F;
     a.next = next;
     LdsAccArgs aa = {res, nullptr, nullptr, 0};
-    hipLaunchKernelGGL((k_lds_probe<kSliceMax, AccSum>), dim3(grid), dim3(kLdsBlock), 0, s, a, aa);
+    hipLaunchKernelGGL((k_lds_probe<SliceT<kSliceMax>, AccSum>), dim3(grid), dim3(kLdsBlock), 0, s, a, aa);
     return check_launch("k_lds_probe");
 }
 

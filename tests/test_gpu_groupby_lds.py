@@ -42,7 +42,7 @@ def read(g, ngroups):
     return g.download(np.uint64, ngroups), g.download(np.int64, ngroups, 8 * ngroups)
 
 
-def probe_groupby_lds(ht, rk, rv, pred, bound, ngroups, fanout=32):
+def probe_groupby_lds(ht, rk, rv, pred, bound, ngroups, fanout=32, flags=2 << 8):
     n = len(rk)
     need = C.c_uint64()
     mgv.call("mgv_probe_groupby_lds_workspace_size", n, ht.nbuckets, fanout, C.byref(need))
@@ -51,7 +51,7 @@ def probe_groupby_lds(ht, rk, rv, pred, bound, ngroups, fanout=32):
     dp = dev(pred) if pred is not None else None
     g = groups(ngroups)
     mgv.call("mgv_hashjoin_probe_groupby_lds", ht.buf.vp, ht.nbuckets, dp.vp if dp else None, bound, dk.vp, dv.vp, n,
-             g.vp, g.at(8 * ngroups), ngroups, fanout, ws.vp, ws.nbytes, mgv.Launch(0, 1024, -1, 2 << 8), None)
+             g.vp, g.at(8 * ngroups), ngroups, fanout, ws.vp, ws.nbytes, mgv.Launch(0, 1024, -1, flags), None)
     return read(g, ngroups)
 
 
@@ -85,6 +85,9 @@ def test_lds_probe_groupby_matches_oracle(gpu, orc, nb_keys, n, ngroups, zipf, f
         wc, ws = orc.probe_groupby(t, rk, rv, c, bound, ngroups)
         gc, gs = probe_groupby_lds(ht, rk, rv, c, bound, ngroups, fanout)
         assert np.array_equal(gc, wc) and np.array_equal(gs, ws)
+        if ht.nbuckets >= 1 << 18:   # tagged slices by default: the key slices agree
+            kc, ks = probe_groupby_lds(ht, rk, rv, c, bound, ngroups, fanout, flags=(2 << 8) | (1 << 13))
+            assert np.array_equal(kc, wc) and np.array_equal(ks, ws)
         dc, ds = probe_groupby_direct(ht, rk, rv, c, bound, ngroups)
         assert np.array_equal(dc, wc) and np.array_equal(ds, ws)
     orc.free(t)

@@ -155,6 +155,40 @@ def test_lds_probe_duplicates_long_chains_and_wraparound(gpu, orc):
         assert probe_lds(ht, rk, rv, fanout=fanout) == (want["count"], want["sum"]) == probe_direct(ht, rk, rv)
 
 
+KEY_SLICES = 1 << 13   # MGV_LAUNCH_KEY_SLICES
+
+
+@pytest.mark.parametrize("nb_keys,lf,key_range,n,fanout", [
+    (1 << 20, 0.5, 0, 1 << 20, 32),             # 2^18 buckets: tags of 14 hash bits + 2 offset bits
+    (1887436, 0.9, 200000, 1 << 20, 16),        # 2^18 buckets at load factor 0.9, ~9 copies per key
+    (1 << 21, 0.5, 0, 1 << 21, 2),              # 2^19 buckets, offsets up to 6
+    (1 << 23, 0.5, 3 << 20, 1 << 21, 64),       # 2^21 buckets, 2.7 copies per key
+])
+def test_tagged_slices_equal_key_slices_and_oracle(gpu, orc, nb_keys, lf, key_range, n, fanout):
+    """Power-of-two tables of >= 2^18 buckets are probed through 16-bit slot
+    tags (chain offset + low hash bits): the result equals the key slices'
+    (MGV_LAUNCH_KEY_SLICES), the direct probe's and the oracle's, also at
+    load factor 0.9 with duplicates, where chains run past the offsets a tag
+    holds and continue in global memory."""
+    if key_range:
+        sk = orc.gen_range(nb_keys, orc.seed(2, 10), 14, key_range)
+    else:
+        sk = orc.gen_perm(nb_keys, nb_keys, orc.seed(2, 10), 14)
+    sw = orc.gen_bits(nb_keys, orc.seed(2, 11), 14, 16)
+    rk = orc.gen_range(n, orc.seed(2, 12), 14, int(1.1 * (key_range or nb_keys)))
+    rv = orc.gen_bits(n, orc.seed(2, 13), 14, 16)
+    ht = build(sk, sw, lf)
+    assert ht.nbuckets & (ht.nbuckets - 1) == 0 and ht.nbuckets >= 1 << 18
+    t = orc.table(sk, sw)
+    want = orc.probe(t, rk, rv)
+    orc.free(t)
+    got = probe_lds(ht, rk, rv, fanout=fanout)
+    assert got == (want["count"], want["sum"])
+    assert probe_lds(ht, rk, rv, fanout=fanout, flags=KEY_SLICES) == got
+    assert probe_direct(ht, rk, rv) == got
+    ht.buf.close()
+
+
 def test_lds_probe_key_minus_one(gpu, orc):
     nb_keys, n = 8192, 50000
     sk = orc.gen_perm(nb_keys, nb_keys, orc.seed(2, 0), 9) - 1          # includes -1 (payload != -1)
