@@ -147,6 +147,10 @@ struct SliceT {
     __device__ __forceinline__ static bool full(const Bucket &b) { return b.c == 8; }
     // whether a row at chain offset j is probed in LDS (always)
     __device__ __forceinline__ static bool in_range(uint32_t, const TagGeom &) { return true; }
+    // the bucket after b, and whether a chain from b may go on to it (not
+    // when it is b itself: a one-bucket table)
+    __device__ __forceinline__ static uint32_t next(uint32_t b, uint32_t nb) { return b + 1 == nb ? 0 : b + 1; }
+    __device__ __forceinline__ static bool moves(uint32_t b, uint32_t nb) { return nb > 1; }
 };
 
 // LDS quad of key quad w (slots 4w..4w+3) of slice bucket idx.  Buckets' key
@@ -197,11 +201,14 @@ struct SliceTag {
         const uint32_t d1 = __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(one, __builtin_bit_cast(u16x2, b.t.y ^ R)));
         const uint32_t d2 = __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(one, __builtin_bit_cast(u16x2, b.t.z ^ R)));
         const uint32_t d3 = __builtin_bit_cast(uint32_t, __builtin_elementwise_sub_sat(one, __builtin_bit_cast(u16x2, b.t.w ^ R)));
-        const uint32_t m = d0 | d1 << 1 | d2 << 2 | d3 << 3;   // slot w at bit w, slot w + 4 at bit 16 + w
-        return (m & 0xFu) | ((m >> 12) & 0xF0u);
+        const uint32_t m = ((d3 << 1 | d2) << 1 | d1) << 1 | d0;   // slot w at bit w, slot w + 4 at bit 16 + w
+        return (m & 0xFu) | (m >> 12);                                // bits 12-15 of m are 0
     }
     __device__ __forceinline__ static bool full(const Bucket &b) { return b.t.w < (kTagEmpty << 16); }
     __device__ __forceinline__ static bool in_range(uint32_t j, const TagGeom &g) { return j <= g.jmax; }
+    // nb is a power of two >= 2^18
+    __device__ __forceinline__ static uint32_t next(uint32_t b, uint32_t nb) { return (b + 1) & (nb - 1); }
+    __device__ __forceinline__ static bool moves(uint32_t, uint32_t) { return true; }
 };
 
 __device__ __forceinline__ uint32_t lowest_slot(uint32_t hm) { return hm ? (uint32_t)__builtin_ctz(hm) : 0u; }
@@ -225,14 +232,15 @@ struct AccSum {
     // hit mask hm of LDS bucket idx (p1 = payload of its lowest hit)
     template <class S>
     __device__ __forceinline__ void lds(const S &sl, uint32_t idx, uint32_t hm, int32_t p1, int32_t val) {
+        // branch-free for the common rows (at most one hit): count, the row
+        // value once per hit, the lowest hit's payload
         const uint32_t h = (uint32_t)__builtin_popcount(hm);
-        int64_t t = h ? (int64_t)p1 : 0;
+        cnt += h;
+        sum += (uint64_t)((int64_t)h * (int64_t)val + (int64_t)(hm ? p1 : 0));
         if (ballot(h > 1)) {
+            int64_t t = 0;
             for (uint32_t m = hm & (hm - 1); m; m &= m - 1) t += (int64_t)(int32_t)sl.pay[idx * 8 + __builtin_ctz(m)];
-        }
-        if (h) {
-            cnt += h;
-            sum += (uint64_t)(t + (int64_t)h * (int64_t)val);
+            sum += (uint64_t)t;
         }
     }
     // a global-memory bucket of a row whose hash is h_key
@@ -326,7 +334,9 @@ __device__ __forceinline__ uint32_t queue_push(WaveQueue &Q, uint32_t tail, bool
                                                uint32_t bkt) {
     const uint64_t bal = ballot(want);
     if (want) {
-        const uint32_t pos = (tail + mbcnt(bal)) & (kQueue - 1);
+        const uint32_t pos =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, tail)) &
+            (kQueue - 1);
         Q.key[pos] = key;
         Q.val[pos] = val;
         Q.bkt[pos] = bkt;
@@ -407,18 +417,25 @@ __device__ __forceinline__ void lds_probe_quad(const v4i &k, const v4i &v, uint3
     typename S::Bucket B[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) B[j] = sl.fetch(ii[j]);
+    // every lane compares (a lane without a row reads bucket 0), then masks
     uint32_t hm[4];
     int32_t p1[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) hm[j] = ins[j] ? S::hits(B[j], (uint32_t)q_get(k, j), 0u, g) : 0u;
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t x = S::hits(B[j], (uint32_t)q_get(k, j), 0u, g);
+        hm[j] = ins[j] ? x : 0u;
+    }
+    // the lowest hit's payload (slot 7 when there is none: read, not used)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) p1[j] = (int32_t)sl.pay[ii[j] * 8 + lowest_slot(hm[j])];
+    for (int j = 0; j < 4; ++j) p1[j] = (int32_t)sl.pay[ii[j] * 8 + (uint32_t)__builtin_ctz(hm[j] | 0x80u)];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         acc.lds(sl, ii[j], hm[j], p1[j], q_get(v, j));
         const bool live = (m >> j) & 1u;
-        const uint32_t from = ins[j] ? ((b[j] + 1 == nb) ? 0 : b[j] + 1) : b[j];
-        const bool go = live && (!ins[j] || (S::full(B[j]) && from != b[j]));
+        const uint32_t from = ins[j] ? S::next(b[j], nb) : b[j];
+        // a row in the slice goes on when its bucket is full, any other
+        // live row starts at its home bucket (bitwise: lane-mask operations)
+        const bool go = (ins[j] & S::full(B[j]) & S::moves(b[j], nb)) | (!ins[j] & live);
         tail = queue_push(Q, tail, go, (uint32_t)q_get(k, j), q_get(v, j), from);
         // fewer than kWave entries were queued before this push and at most
         // kWave were added, so the queue never holds more than kQueue - 1
