@@ -550,6 +550,31 @@ def leg_f1(args, st):
     return out
 
 
+# ------------------------------------------------------------------ copy
+
+def leg_copy(args, st):
+    """Plain device-to-device copies (mgv_memcpy_async) of 1, 2 and 4 GiB:
+    the read+write bandwidth that a one-pass scatter can reach (each of the
+    probe's partitionings reads and writes its rows once), and the
+    calibration of rocprofv3's FETCH_SIZE / WRITE_SIZE counters (a copy of B
+    bytes reads B and writes B)."""
+    out = {}
+    for nbytes in (1 << 30, 2 << 30, 4 << 30):
+        a, b = mgv.DeviceBuffer(nbytes), mgv.DeviceBuffer(nbytes)
+        mgv.call("mgv_fill_u32", a.vp, nbytes // 4, 7, st.vp)
+        t = Timer(st.vp)
+        for _ in range(6):
+            t.around(lambda: mgv.call("mgv_memcpy_async", b.vp, a.vp, nbytes, st.vp))
+        st.sync()
+        ms = min(t.durations_ms()[1:])
+        out[f"{nbytes >> 30}GiB"] = {"ms": ms, "gbps_read_write": 2 * nbytes / ms / 1e6}
+        a.close()
+        b.close()
+    log("copy: " + ", ".join(f"{k} {v['ms']:.3f} ms ({v['gbps_read_write']:.0f} GB/s read+write)"
+                             for k, v in out.items()))
+    return out
+
+
 # ------------------------------------------------------------------ split
 
 def leg_f3(args, st):
@@ -948,7 +973,7 @@ def main():
                     help="probe path (auto = time the candidates, keep the fastest)")
     ap.add_argument("--fanout", type=int, default=-1, help="fan-out of the partitioned paths (with --probe)")
     ap.add_argument("--scatter", type=int, default=-1, help="scatter workgroup shape MGV_SCATTER_* (-1 = tuned)")
-    ap.add_argument("--legs", default="c1,c3,c4,c5,f3,f1")
+    ap.add_argument("--legs", default="c1,c3,c4,c5,copy,f3,f1")
     ap.add_argument("--c5-requests", type=int, default=300)
     ap.add_argument("--c4-reps", type=int, default=5)
     ap.add_argument("--cpu", type=int, default=1)
@@ -975,6 +1000,8 @@ def main():
     if "c4" in legs:
         mgv.load_ops()
         extra["c4"] = leg_c4(args, st)
+    if "copy" in legs:
+        extra["copy"] = leg_copy(args, st)
     if "f3" in legs:
         extra["f3"] = leg_f3(args, st)
     if "f1" in legs:
