@@ -821,13 +821,15 @@ def leg_c4(args, st):
     # routed: the large probes through LDS table slices, in the same graph
     thr = int(t.get("route_threshold", 1 << 20))
     nst = int(t.get("route_streams", 4))
-    g = batch.capture_routed(V, L, st.vp, thr, nst)
+    fan = int(t.get("route_fanout", 0))
+    fan = fan if fan > 1 else 0   # 1 = the tuner kept the fused form: time the one-level LDS route
+    g = batch.capture_routed(V, L, st.vp, thr, nst, fan)
     timed(run_graph, 2)
     rw, re_ = timed(run_graph, args.c4_reps)
     routed = batch.results()
     routed_same = bool(np.array_equal(routed["count"], fused["count"]) and
                        np.array_equal(routed["sum"], fused["sum"]))
-    nbig = len(batch._routed(V, thr, nst)["big"])
+    nbig = len(batch._routed(V, thr, nst, fan)["big"])
     log(f"C4 routed ({nbig} queries >= {thr} rows through LDS slices on {nst} streams): {min(rw) * 1e3:.2f} ms "
         f"wall, equal to fused: {routed_same}")
     plain_out = {}
@@ -871,7 +873,7 @@ def leg_c4(args, st):
                     "frac": algo / (min(best_e) / 1e3) / 1e9 / HBM_PEAK_GBPS, "work_items": batch.items(V)},
         "fused_only": {"wall_ms": min(bw) * 1e3, "gpu_ms": min(be), "queries_per_s": fused_qps},
         "routed": {"wall_ms": min(rw) * 1e3, "gpu_ms": min(re_), "queries_per_s": routed_qps,
-                   "lds_queries": nbig, "threshold_rows": thr, "side_streams": nst},
+                   "lds_queries": nbig, "threshold_rows": thr, "side_streams": nst, "fanout": fan or "one level"},
         "plain": plain_out,
         "speedup_vs_best_plain": max(fused_qps, routed_qps) / best_plain["queries_per_s"],
         "parity": {"fused_equals_plain": all(v["results_equal_fused"] for v in plain_out.values()),

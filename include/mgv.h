@@ -395,7 +395,8 @@ int mgv_batch_plan(const mgv_query *queries, uint32_t nqueries, uint32_t vector_
  * each wave claims a vector with one atomic on *counter (reset by this call
  * with a fill kernel, so the call can be captured into a graph) and runs its
  * query's chain.  Builds must be in an earlier call than the probes that
- * use their tables.  Replaces one launch_consume_pipeline per vector
+ * use their tables.  launch.block: 64, 128, 256, 512 or 1024 threads
+ * (0 = 256).  Replaces one launch_consume_pipeline per vector
  * (src/operators/exchange.cu:124-194,196-291) with one launch per batch. */
 int mgv_batch_run(const mgv_query *queries, const mgv_work_item *items, uint64_t nitems, uint32_t *counter,
                   mgv_launch launch, mgv_stream_t s);

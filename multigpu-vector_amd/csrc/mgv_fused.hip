@@ -169,10 +169,12 @@ static int fused_c(const mgv_query *Q, const mgv_work_item *items, uint64_t nite
                    const mgv_launch &L, hipStream_t s) {
     switch (L.block) {
         case 64: return fused_b<64, C>(Q, items, nitems, counter, L, s);
+        case 128: return fused_b<128, C>(Q, items, nitems, counter, L, s);
         case 0:
         case 256: return fused_b<256, C>(Q, items, nitems, counter, L, s);
+        case 512: return fused_b<512, C>(Q, items, nitems, counter, L, s);
         case 1024: return fused_b<1024, C>(Q, items, nitems, counter, L, s);
-        default: return fail(MGV_E_INVALID, "block %u not in {64,256,1024}", L.block);
+        default: return fail(MGV_E_INVALID, "block %u not in {64,128,256,512,1024}", L.block);
     }
 }
 

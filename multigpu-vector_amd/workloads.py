@@ -170,21 +170,28 @@ class JoinBatch:
         return g
 
     # -------------------------------------------------------------- routed path
-    def lds_fanout(self, x: JoinQuery) -> int:
-        """Fan-out that cuts a query's table into slices of <= 2048 buckets in
-        one partitioning level (0 when its bucket count does not split)."""
+    def lds_fanout(self, x: JoinQuery, want: int = 0) -> int:
+        """Level-1 fan-out of a query's LDS probe: `want` when the call takes
+        it for this table (one level, or two when the table needs more
+        slices), else the fan-out that cuts the table into slices of <= 2048
+        buckets in one level (0 when its bucket count does not split)."""
+        if want:
+            b = C.c_uint64()
+            if mgv.load().mgv_probe_lds_workspace_size(C.c_uint64(x.P), C.c_uint32(x.nbuckets), C.c_uint32(want),
+                                                       C.byref(b)) == 0:
+                return want
         f = 2
         while x.nbuckets // f > 2048:
             f *= 2
         return f if x.nbuckets % f == 0 and f <= 2048 else 0
 
-    def _routed(self, vector_rows: int, threshold: int, nstreams: int):
+    def _routed(self, vector_rows: int, threshold: int, nstreams: int, fanout: int = 0):
         """Plan of the routed batch: the fused items of the small probes
         (P < threshold) and the large probes' LDS calls; cached."""
-        key = ("routed", vector_rows, threshold, nstreams)
+        key = ("routed", vector_rows, threshold, nstreams, fanout)
         if key in self._plans:
             return self._plans[key]
-        big = [i for i, x in enumerate(self.q) if x.P >= threshold and self.lds_fanout(x)]
+        big = [i for i, x in enumerate(self.q) if x.P >= threshold and self.lds_fanout(x, fanout)]
         small = [i for i in range(self.nq) if i not in set(big)]
         sq = (mgv.Query * max(1, len(small)))()
         for j, i in enumerate(small):
@@ -197,29 +204,31 @@ class JoinBatch:
         dq = mgv.DeviceBuffer(C.sizeof(sq)).upload(np.frombuffer(bytes(sq), dtype=np.uint8))
         di = mgv.DeviceBuffer(max(16, C.sizeof(items))).upload(np.frombuffer(bytes(items), dtype=np.uint8))
         need = 0
+        fan = {i: self.lds_fanout(self.q[i], fanout) for i in big}
         for i in big:
             x = self.q[i]
             b = C.c_uint64()
-            mgv.call("mgv_probe_lds_workspace_size", x.P, x.nbuckets, self.lds_fanout(x), C.byref(b))
+            mgv.call("mgv_probe_lds_workspace_size", x.P, x.nbuckets, fan[i], C.byref(b))
             need = max(need, b.value)
         ws = [mgv.DeviceBuffer(max(256, need)) for _ in range(nstreams)] if big else []
         streams = [mgv.Stream() for _ in range(nstreams)] if big else []
         events = [mgv.Event() for _ in range(nstreams + 1)] if big else []
-        plan = {"small": (dq, di, n.value if small else 0), "big": big, "ws": ws, "streams": streams,
+        plan = {"small": (dq, di, n.value if small else 0), "big": big, "fan": fan, "ws": ws, "streams": streams,
                 "events": events}
         self._plans[key] = plan
         return plan
 
     def issue_routed(self, vector_rows: int, launch: mgv.Launch, stream, threshold: int = 1 << 20,
-                     nstreams: int = 4):
+                     nstreams: int = 4, fanout: int = 0):
         """clear + the fused build launch; then the small probes in one fused
         launch on `stream` while the large ones (P >= threshold, tables that
-        split into LDS slices) run through mgv_hashjoin_probe_lds on nstreams
-        side streams (forked and joined with events, so the whole batch can be
-        captured into one graph).  Results land in the same slots as the fused
-        path's."""
+        split into LDS slices) run through mgv_hashjoin_probe_lds (level-1
+        fan-out `fanout` where the table takes it, see lds_fanout) on
+        nstreams side streams (forked and joined with events, so the whole
+        batch can be captured into one graph).  Results land in the same
+        slots as the fused path's."""
         (bq, bi, bn), _ = self._plan(vector_rows)
-        plan = self._routed(vector_rows, threshold, nstreams)
+        plan = self._routed(vector_rows, threshold, nstreams, fanout)
         self.clear(stream)
         mgv.call("mgv_batch_run", bq.vp, bi.vp, bn, self.counter.vp, launch, stream)
         ev = plan["events"]
@@ -234,7 +243,7 @@ class JoinBatch:
                 ws, sd = plan["ws"][k], plan["streams"][k]
                 r4 = 4 * x.r_off
                 mgv.call("mgv_hashjoin_probe_lds", self.tables.ptr + x.t_off, x.nbuckets, self.rc.at(r4), x.bound,
-                         self.rk.at(r4), self.rv.at(r4), x.P, self.lds_fanout(x), ws.vp, ws.nbytes,
+                         self.rk.at(r4), self.rv.at(r4), x.P, plan["fan"][i], ws.vp, ws.nbytes,
                          self.res.slot(self.nq + i), lds_launch, sd.vp)
         dq, di, pn = plan["small"]
         if pn:
@@ -245,11 +254,11 @@ class JoinBatch:
                 mgv.call("mgv_stream_wait_event", stream, C.c_void_p(ev[k + 1].ptr))
 
     def capture_routed(self, vector_rows: int, launch: mgv.Launch, stream, threshold: int = 1 << 20,
-                       nstreams: int = 4) -> C.c_void_p:
+                       nstreams: int = 4, fanout: int = 0) -> C.c_void_p:
         self._plan(vector_rows)
-        self._routed(vector_rows, threshold, nstreams)
+        self._routed(vector_rows, threshold, nstreams, fanout)
         mgv.call("mgv_graph_begin_capture", stream)
-        self.issue_routed(vector_rows, launch, stream, threshold, nstreams)
+        self.issue_routed(vector_rows, launch, stream, threshold, nstreams, fanout)
         g = C.c_void_p()
         mgv.call("mgv_graph_end_capture", stream, C.byref(g))
         return g

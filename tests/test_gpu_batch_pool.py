@@ -70,7 +70,8 @@ def run_batch(gpu, qs, vector_rows=1024, compact=1, block=256):
     return [res.read(nq + i) for i in range(nq)], [res.read(i) for i in range(nq)], tables
 
 
-@pytest.mark.parametrize("vector_rows,compact,block", [(256, 1, 256), (1024, 0, 64), (4096, 1, 1024)])
+@pytest.mark.parametrize("vector_rows,compact,block", [(256, 1, 256), (1024, 0, 64), (4096, 1, 1024), (2048, 1, 128),
+                                                      (4096, 0, 512)])
 def test_fused_batch_matches_oracle(gpu, orc, vector_rows, compact, block):
     qs = make_queries(orc, 24)
     probes, builds, _ = run_batch(gpu, qs, vector_rows, compact, block)
