@@ -3,20 +3,24 @@
 // The partitioned probe keeps a partition's table slice in the XCD's L2, but
 // every probe row still costs one L1->L2 request, and those requests' latency
 // and the L1's limit on outstanding misses bound it.  This path partitions
-// the probe rows twice by the top bits of fmix32(key) -- the bits that also
-// pick the bucket --:
-//   level 1: F1 partitions (default 64), packed without gaps;
-//   level 2: F2 = nbuckets / S partitions of the level-1 output, where a
-//            slice of S <= 2048 buckets (128 KiB) fits in LDS; because the
-//            level-1 output is already ordered by the top log2(F1) bits, a
-//            level-2 tile spreads over only F2 / F1 partitions, so the second
-//            scatter writes long runs.
+// the probe rows by the top bits of fmix32(key) -- the bits that also pick
+// the bucket -- into F2 = nbuckets / S partitions, where a slice of S <= 2048
+// buckets fits in LDS:
+//   level 1: F1 regions of fixed capacity, filled without count passes (one
+//            atomic reservation per tile and partition; rows past a full
+//            region go to an overflow area);
+//   level 2: each level-1 region cut into F2 / F1 regions the same way (a
+//            level-1 region holds one value of the top log2(F1) hash bits,
+//            so a level-2 tile spans only F2 / F1 partitions and its runs
+//            stay long); one level when nbuckets / F1 already is a slice.
 // Then one block per CU claims level-2 partitions: it copies the partition's
 // table slice (plus a few following buckets for chains that run past it)
-// into LDS and probes the partition's rows from LDS; a chain that leaves the
-// loaded range continues in global memory.  Semantics are those of
-// mgv_hashjoin_probe: COUNT of matching (row, slot) pairs and SUM(val +
-// payload), exact in 64 bits.  Replaces hashjoin::consume_warp
+// into LDS -- as 16-bit slot tags for power-of-two tables of >= 2^18
+// buckets, else as key hashes -- and probes the partition's rows from LDS; a
+// chain that leaves the loaded range continues in global memory, and the
+// overflow areas' rows are probed from the table directly.  Semantics are
+// those of mgv_hashjoin_probe: COUNT of matching (row, slot) pairs and
+// SUM(val + payload), exact in 64 bits.  Replaces hashjoin::consume_warp
 // (src/operators/hashjoin.cu:135-182) with its aggregation parent
 // (src/operators/aggregation.cu:27-66).
 #include <hip/hip_runtime.h>
@@ -119,7 +123,7 @@ __device__ __forceinline__ bool bucket_full(const v4u (&q)[4]) { return !(q[3].z
 
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
-// A table slice in LDS is stored keys first: per bucket its 8 keys (hashed,
+// A key slice in LDS is stored keys first: per bucket its 8 keys (hashed,
 // in two 16-byte quads), its 8 payloads, and its fill count c.  The build fills a
 // bucket's slots strictly in order (a slot is claimed only after every lower
 // slot was seen occupied, and slots are never freed), so the occupied slots
