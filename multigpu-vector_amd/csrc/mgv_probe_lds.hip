@@ -101,17 +101,17 @@ __device__ __forceinline__ uint32_t bucket_of_hash(uint32_t h, uint32_t nb) {
 }
 
 // Scans one global-memory bucket (64 bytes: 8 {key, payload} slots, empty =
-// (-1, -1)) for the key whose hash is h: number of hitting slots and the sum
-// of their payloads.  The empty pattern is excluded from the compare, so key
-// -1 is handled.
-__device__ __forceinline__ void bucket_scan(const v4u (&q)[4], uint32_t h_key, uint32_t &hits, int64_t &psum) {
+// (-1, -1)) for key (recovered from a row's hash with fmix32_inv): number of
+// hitting slots and the sum of their payloads.  The empty pattern is
+// excluded from the compare, so key -1 is handled.
+__device__ __forceinline__ void bucket_scan(const v4u (&q)[4], uint32_t key, uint32_t &hits, int64_t &psum) {
     uint32_t h = 0;
     int64_t t = 0;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
         const uint32_t sk = (s & 1) ? q[s >> 1].z : q[s >> 1].x;
         const uint32_t sp = (s & 1) ? q[s >> 1].w : q[s >> 1].y;
-        const bool hit = fmix32(sk) == h_key && !(sk == ~0u && sp == ~0u);
+        const bool hit = sk == key && !(sk == ~0u && sp == ~0u);
         h += hit ? 1u : 0u;
         t += hit ? (int64_t)(int32_t)sp : 0;
     }
@@ -260,7 +260,7 @@ struct AccSum {
     __device__ __forceinline__ void bucket(const v4u (&q)[4], uint32_t h_key, int32_t val) {
         uint32_t hits = 0;
         int64_t psum = 0;
-        bucket_scan(q, h_key, hits, psum);
+        bucket_scan(q, fmix32_inv(h_key), hits, psum);
         if (hits) {
             cnt += hits;
             sum += (uint64_t)(psum + (int64_t)hits * (int64_t)val);
@@ -309,11 +309,12 @@ struct AccGroup {
         }
     }
     __device__ __forceinline__ void bucket(const v4u (&q)[4], uint32_t h_key, int32_t val) {
+        const uint32_t key = fmix32_inv(h_key);
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             const uint32_t sk = (s & 1) ? q[s >> 1].z : q[s >> 1].x;
             const uint32_t sp = (s & 1) ? q[s >> 1].w : q[s >> 1].y;
-            if (fmix32(sk) == h_key && !(sk == ~0u && sp == ~0u)) add(sp, val);
+            if (sk == key && !(sk == ~0u && sp == ~0u)) add(sp, val);
         }
     }
     // the whole block calls close

@@ -171,6 +171,20 @@ __host__ __device__ __forceinline__ uint32_t fmix32(uint32_t x) {
     return x;
 }
 
+// The inverse of fmix32 (each step undone in reverse order: x ^= x >> 16 is
+// its own inverse, x ^= x >> 13 is undone by x ^= x >> 13 ^ x >> 26, and
+// the multipliers by their inverses mod 2^32, 0xa5cb9243 * 0x85ebca6b =
+// 0x7ed1b41d * 0xc2b2ae35 = 1): recovers a key from its hash, so a hashed
+// row compares raw keys instead of hashing every slot.
+__host__ __device__ __forceinline__ uint32_t fmix32_inv(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7ed1b41du;
+    x ^= (x >> 13) ^ (x >> 26);
+    x *= 0xa5cb9243u;
+    x ^= x >> 16;
+    return x;
+}
+
 // splitmix64 finaliser, used for order-independent join-pair digests.
 __host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
     x ^= x >> 30;

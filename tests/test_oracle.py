@@ -435,3 +435,25 @@ def test_threaded_build_and_groupby_equal_serial(orc):
     assert int(g1[0].sum()) == w1["count"]
     orc.free(t1)
     orc.free(t8)
+
+
+# ----------------------------------------------------------------- fmix32 inverse
+
+def test_fmix32_inverse_constants_undo_the_oracle_fmix32(orc):
+    """The library recovers a key from its hash (mgv_wave.hpp fmix32_inv) with
+    the multiplicative inverses 0xa5cb9243 = 0x85ebca6b^-1 and 0x7ed1b41d =
+    0xc2b2ae35^-1 mod 2^32 and the xor-shift inverses; composed with the
+    oracle's fmix32 (pinned by sklearn's murmurhash3 above) the restatement
+    must give back every input."""
+    assert (0xa5cb9243 * 0x85ebca6b) % 2**32 == 1 and (0x7ed1b41d * 0xc2b2ae35) % 2**32 == 1
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.integers(0, 2**32, 1 << 16, dtype=np.uint64),
+                        np.array([0, 1, 2**31, 2**32 - 1, 0x85ebca6b], dtype=np.uint64)]).astype(np.uint32)
+    h = np.array([orc.L.orc_fmix32(int(v)) for v in x], dtype=np.uint64)
+    M = np.uint64(0xFFFFFFFF)
+    y = h ^ (h >> np.uint64(16))
+    y = (y * np.uint64(0x7ed1b41d)) & M
+    y = y ^ (y >> np.uint64(13)) ^ (y >> np.uint64(26))
+    y = (y * np.uint64(0xa5cb9243)) & M
+    y = y ^ (y >> np.uint64(16))
+    assert np.array_equal(y.astype(np.uint32), x)
