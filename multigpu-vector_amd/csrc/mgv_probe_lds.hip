@@ -266,6 +266,9 @@ struct AccSum {
             sum += (uint64_t)(psum + (int64_t)hits * (int64_t)val);
         }
     }
+    __device__ __forceinline__ void bucket_wave(bool live, const v4u (&q)[4], uint32_t h_key, int32_t val) {
+        if (live) bucket(q, h_key, val);
+    }
     // every wave of the block calls close
     __device__ __forceinline__ void close(const LdsAccArgs &a) {
         cnt = wave_sum_u64(cnt);
@@ -301,13 +304,43 @@ struct AccGroup {
         }
         __syncthreads();
     }
+    // One row per lane with `on`, called by every lane of the wave: when at
+    // least kHotLanes lanes add to the same group (the lowest active lane's),
+    // their values are summed across the wave and added with one atomic
+    // pair -- rows partitioned by key arrive in runs of one hot key, whose
+    // per-lane atomics on one LDS word would serialise the wave.
+    static constexpr uint32_t kHotLanes = 16;
+    __device__ __forceinline__ void add_wave(bool on, uint32_t g, int32_t v) {
+        const bool ok = on && g < ngroups;
+        const uint64_t act = ballot(ok);
+        if (act == 0) return;
+        const uint32_t first = (uint32_t)__builtin_ctzll(act);
+        const uint32_t g0 = (uint32_t)__builtin_amdgcn_readlane((int)g, (int)first);
+        const bool in0 = ok && g == g0;
+        const uint64_t same = ballot(in0);
+        bool mine = ok;
+        if (popc64(same) >= kHotLanes) {
+            const uint64_t t = wave_sum_u64(in0 ? (uint64_t)(int64_t)v : 0ull);
+            if (lane_id() == first) {
+                atomic_add_lds(&sm->cnt[g0], popc64(same));
+                atomic_add_lds(&sm->sum[g0], (unsigned long long)t);
+            }
+            mine = ok && !in0;
+        }
+        if (mine) {
+            atomic_add_lds(&sm->cnt[g], 1u);
+            atomic_add_lds(&sm->sum[g], (unsigned long long)(int64_t)v);
+        }
+    }
+    // called by every lane of the wave
     template <class S>
     __device__ __forceinline__ void lds(const S &sl, uint32_t idx, uint32_t hm, int32_t p1, int32_t val) {
-        if (hm) add((uint32_t)p1, val);
+        add_wave(hm != 0, (uint32_t)p1, val);
         if (ballot(__builtin_popcount(hm) > 1)) {
             for (uint32_t m = hm & (hm - 1); m; m &= m - 1) add(sl.pay[idx * 8 + __builtin_ctz(m)], val);
         }
     }
+    // a global-memory bucket of a row whose hash is h_key (any lane set)
     __device__ __forceinline__ void bucket(const v4u (&q)[4], uint32_t h_key, int32_t val) {
         const uint32_t key = fmix32_inv(h_key);
 #pragma unroll
@@ -315,6 +348,28 @@ struct AccGroup {
             const uint32_t sk = (s & 1) ? q[s >> 1].z : q[s >> 1].x;
             const uint32_t sp = (s & 1) ? q[s >> 1].w : q[s >> 1].y;
             if (sk == key && !(sk == ~0u && sp == ~0u)) add(sp, val);
+        }
+    }
+    // the same, called by every lane of the wave (rows of lanes with
+    // `live`): the first hitting slot through add_wave, any further one (a
+    // duplicate key) per lane
+    __device__ __forceinline__ void bucket_wave(bool live, const v4u (&q)[4], uint32_t h_key, int32_t val) {
+        const uint32_t key = fmix32_inv(h_key);
+        uint32_t hm = 0, p0 = 0;   // hit mask, payload of the lowest hit
+#pragma unroll
+        for (int s = 7; s >= 0; --s) {
+            const uint32_t sk = (s & 1) ? q[s >> 1].z : q[s >> 1].x;
+            const uint32_t sp = (s & 1) ? q[s >> 1].w : q[s >> 1].y;
+            const bool hit = live && sk == key && !(sk == ~0u && sp == ~0u);
+            hm |= (uint32_t)hit << s;
+            p0 = hit ? sp : p0;
+        }
+        add_wave(hm != 0, p0, val);
+        if (ballot((hm & (hm - 1)) != 0)) {
+            for (uint32_t m = hm & (hm - 1); m; m &= m - 1) {
+                const uint32_t s = (uint32_t)__builtin_ctz(m);
+                add((s & 1) ? q[s >> 1].w : q[s >> 1].y, val);
+            }
         }
     }
     // the whole block calls close
@@ -886,7 +941,9 @@ struct HashedRows {
 
 // Probes hashed rows against the global table: the overflow areas of both
 // levels of a fill partitioning, one row per lane, its chain followed bucket
-// by bucket as in mgv_hashjoin_probe.
+// by bucket as in mgv_hashjoin_probe.  The chain loop is wave-uniform (a
+// lane whose chain ended stays in it without a row), so the accumulator can
+// combine the lanes' rows of one group.
 template <class Acc>
 __global__ __launch_bounds__(256) void k_probe_hashed(const uint64_t *slots, uint32_t nb, HashedRows A, HashedRows B,
                                                       LdsAccArgs aa) {
@@ -898,21 +955,22 @@ __global__ __launch_bounds__(256) void k_probe_hashed(const uint64_t *slots, uin
     const uint64_t n = na + (fb < B.ncap ? fb : B.ncap);
     for (uint64_t i0 = (uint64_t)blockIdx.x * 256; i0 < n; i0 += (uint64_t)gridDim.x * 256) {
         const uint64_t i = i0 + threadIdx.x;
-        if (i < n) {
-            const uint32_t h = (uint32_t)(i < na ? A.H[i] : B.H[i - na]);
-            const int32_t v = i < na ? A.V[i] : B.V[i - na];
-            const uint32_t home = bucket_of_hash(h, nb);
-            uint32_t b = home;
-            for (;;) {
+        bool live = i < n;
+        const uint32_t h = live ? (uint32_t)(i < na ? A.H[i] : B.H[i - na]) : 0u;
+        const int32_t v = live ? (i < na ? A.V[i] : B.V[i - na]) : 0;
+        const uint32_t home = bucket_of_hash(h, nb);
+        uint32_t b = home;
+        while (ballot(live)) {
+            v4u q[4] = {};
+            if (live) {
                 const v4u *g = reinterpret_cast<const v4u *>(slots + (uint64_t)b * 8);
-                v4u q[4];
 #pragma unroll
                 for (int w = 0; w < 4; ++w) q[w] = load_global(g + w);
-                acc.bucket(q, h, v);
-                if (!bucket_full(q)) break;
-                b = (b + 1 == nb) ? 0 : b + 1;
-                if (b == home) break;
             }
+            acc.bucket_wave(live, q, h, v);
+            const bool full = bucket_full(q);
+            b = (b + 1 == nb) ? 0 : b + 1;
+            live = live && full && b != home;
         }
     }
     acc.close(aa);
@@ -1245,7 +1303,10 @@ static int probe_lds_fill(const uint64_t *slots, uint32_t nbuckets, const int32_
     const HashedRows o2 = {L2.col[0] + L2.ostart, L2.col[1] + L2.ostart, L2.ofill, L2.ocap};
     const HashedRows o1 = L1.F ? HashedRows{L1.col[0] + L1.ostart, L1.col[1] + L1.ostart, L1.ofill, L1.ocap}
                                : HashedRows{nullptr, nullptr, nullptr, 0};
-    hipLaunchKernelGGL(k_probe_hashed<Acc>, dim3(cus), dim3(256), 0, s, slots, nbuckets, o2, o1, aa);
+    // one row per lane and a dependent bucket load per chain step: eight
+    // blocks per CU keep enough loads in flight when skewed keys fill the
+    // overflow areas (the blocks find no rows otherwise)
+    hipLaunchKernelGGL(k_probe_hashed<Acc>, dim3(8 * cus), dim3(256), 0, s, slots, nbuckets, o2, o1, aa);
     return check_launch("k_probe_hashed");
 }
 

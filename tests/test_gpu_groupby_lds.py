@@ -117,3 +117,29 @@ def test_lds_probe_groupby_rejects_too_many_groups(gpu):
         mgv.call("mgv_hashjoin_probe_groupby_lds", ht.buf.vp, ht.nbuckets, None, 0, x.vp, x.vp, 1024, g.vp,
                  g.at(8 * 8192), 8192, 2, None, 0, mgv.Launch(0, 1024, -1, 0), None)
     assert e.value.code == mgv.E_INVALID
+
+
+@pytest.mark.parametrize("nb_keys", [1 << 16, 1 << 20])
+def test_lds_probe_groupby_one_hot_key(gpu, orc, nb_keys):
+    """60 % of the probe rows on one key: its partition's region and the
+    overflow area are full of that key, so whole waves add to one group (the
+    wave-combined path of the group accumulator) -- in the LDS probe and in
+    the hashed probe of the overflow rows; one level (2^14 buckets) and two
+    (2^18 buckets).  Equal to the oracle and the direct kernel."""
+    n, ngroups = 1 << 21, 1024
+    sk = orc.gen_perm(nb_keys, nb_keys, orc.seed(3, 10), 22)
+    sg = orc.gen_range(nb_keys, orc.seed(3, 11), 22, ngroups)
+    rk = orc.gen_range(n, orc.seed(3, 12), 22, 2 * nb_keys)
+    rk[: (3 * n) // 5] = 4321
+    rv = orc.gen_bits(n, orc.seed(3, 13), 22, 16)
+    c = orc.gen_bits(n, orc.seed(3, 14), 22, 31)
+    ht = build(sk, sg, 0.5)
+    t = orc.table(sk, sg)
+    bound = (3 << 29) - 1     # 75 %
+    wc, ws = orc.probe_groupby(t, rk, rv, c, bound, ngroups)
+    orc.free(t)
+    for fanout in (32, 256):
+        gc, gs = probe_groupby_lds(ht, rk, rv, c, bound, ngroups, fanout)
+        assert np.array_equal(gc, wc) and np.array_equal(gs, ws)
+    dc, ds = probe_groupby_direct(ht, rk, rv, c, bound, ngroups)
+    assert np.array_equal(dc, wc) and np.array_equal(ds, ws)
