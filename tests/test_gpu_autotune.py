@@ -1,8 +1,9 @@
 """GPU test of the autotuner in quick mode on reduced workloads: every
 candidate of every search space must reproduce the first candidate's
 results (the tuner raises otherwise), the C4 space covers the five
-workgroup sizes and the routed fan-outs, and the C2 space every probe form
-(BASELINE configs[3] and the C2 probe paths)."""
+workgroup sizes and the routed fan-outs (the batched 256-query workload's
+space: vector size, workgroup 64..1024, load factor, partition fan-out),
+and the C2 space every probe form."""
 from __future__ import annotations
 
 import pytest
