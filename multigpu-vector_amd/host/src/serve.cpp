@@ -96,6 +96,15 @@ struct Admitted {
 constexpr uint64_t kLdsBuildRows = 1ull << 21;
 constexpr uint32_t kLdsBuildFanout = 32;
 
+// Work items of a request: a wave runs an item's rows one 256-row chunk
+// after another, so a request cut into few large items runs on few waves
+// (a 2^16-row build in one 65536-row item is 256 dependent chunks of
+// compare-and-swaps on one wave).  Each request is therefore cut into at
+// least kSpreadItems items while they hold at least kMinItemRows rows, and
+// into items of at most vector_rows rows.
+constexpr uint64_t kSpreadItems = 4096;
+constexpr uint64_t kMinItemRows = 1024;
+
 struct Slot {
     mgv_stream_t stream = nullptr;
     bool busy = false;
@@ -168,7 +177,31 @@ struct Server {
         return align_up((uint64_t)q.ngroups * 16, 256) + w;
     }
 
-    uint64_t items_of(uint64_t rows) const { return (rows + cfg.vector_rows - 1) / cfg.vector_rows; }
+    // rows per work item of a request of `rows` rows: a multiple of 256 in
+    // [min(kMinItemRows, vector_rows), vector_rows]
+    uint32_t item_rows(uint64_t rows) const {
+        const uint64_t lo = std::min<uint64_t>(kMinItemRows, cfg.vector_rows);
+        const uint64_t want = align_up((rows + kSpreadItems - 1) / kSpreadItems, 256);
+        return (uint32_t)std::min<uint64_t>(cfg.vector_rows, std::max(lo, want));
+    }
+
+    uint64_t items_of(uint64_t rows) const {
+        const uint64_t v = item_rows(rows);
+        return (rows + v - 1) / v;
+    }
+
+    // mgv_batch_plan per query with the query's own item size; the items
+    // name their query's index in qs
+    void plan(const mgv_query *qs, uint32_t nq, mgv_work_item *items, uint64_t cap, uint64_t *nitems) const {
+        uint64_t k = 0;
+        for (uint32_t i = 0; i < nq; ++i) {
+            uint64_t c = 0;
+            gpu(mgv_batch_plan(qs + i, 1, item_rows(qs[i].n), items + k, cap - k, &c));
+            for (uint64_t j = 0; j < c; ++j) items[k + j].query = i;
+            k += c;
+        }
+        *nitems = k;
+    }
 
     void setup() {
         if (cfg.max_inflight < 1 || cfg.max_inflight > 8) throw mgv::error(MGV_E_INVALID, "max_inflight in 1..8");
@@ -269,8 +302,8 @@ struct Server {
             mq[s.nmain++] = m;
         }
         uint64_t nb = 0, nm = 0;
-        if (s.nbuild) gpu(mgv_batch_plan(bq, s.nbuild, cfg.vector_rows, items, items_cap, &nb));
-        if (s.nmain) gpu(mgv_batch_plan(mq, s.nmain, cfg.vector_rows, items + nb, items_cap - nb, &nm));
+        if (s.nbuild) plan(bq, s.nbuild, items, items_cap, &nb);
+        if (s.nmain) plan(mq, s.nmain, items + nb, items_cap - nb, &nm);
         if (nb + nm > items_cap) throw mgv::error(MGV_E_CAPACITY, "serve: work-item staging too small");
         s.nbitems = nb;
         s.nmitems = nm;
