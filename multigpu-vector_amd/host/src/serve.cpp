@@ -102,8 +102,14 @@ constexpr uint32_t kLdsBuildFanout = 32;
 // compare-and-swaps on one wave).  Each request is therefore cut into at
 // least kSpreadItems items while they hold at least kMinItemRows rows, and
 // into items of at most vector_rows rows.
-constexpr uint64_t kSpreadItems = 4096;
+constexpr uint64_t kSpreadItems = 16384;
 constexpr uint64_t kMinItemRows = 1024;
+
+// A batch's results are published together when its last launch ends, so a
+// small request batched with a large one waits for the large one.  A batch
+// takes requests while their rows (probe / group-by rows + build rows) add up
+// to at most kBatchRows; a larger request runs in a batch of its own.
+constexpr uint64_t kBatchRows = 1ull << 26;
 
 struct Slot {
     mgv_stream_t stream = nullptr;
@@ -437,7 +443,7 @@ struct Server {
             bool issued = false;
             for (Slot &s : slots) {
                 if (s.busy || waiting.empty()) continue;
-                uint64_t items = 0, gstage = 0;
+                uint64_t items = 0, gstage = 0, rows = 0;
                 while (!waiting.empty() && s.q.size() < cfg.max_batch) {
                     const uint32_t i = waiting.front();
                     const mgv_serve_request &r = R[i];
@@ -446,7 +452,9 @@ struct Server {
                     const uint64_t bytes = footprint(r, &nb, &ws);
                     const uint64_t it = items_of(r.n) + (r.kind == MGV_SERVE_JOIN_SUM ? items_of(r.bn) : 0);
                     const uint64_t gs = r.kind == MGV_SERVE_GROUPBY ? r.ngroups : 0;
+                    const uint64_t rr = r.n + (r.kind == MGV_SERVE_JOIN_SUM ? r.bn : 0);
                     if (items + it > items_cap || gstage + gs > s.group_cap) break;
+                    if (!s.q.empty() && rows + rr > kBatchRows) break;   // the next batch takes it
                     uint64_t off = 0;
                     if (!arena.alloc(bytes, &off)) break;   // FIFO: wait for memory
                     s.q.push_back(Admitted{i, off, bytes, nb, gstage, ws});
@@ -454,6 +462,7 @@ struct Server {
                     out[i].admit_s = now_s();
                     items += it;
                     gstage += gs;
+                    rows += rr;
                     waiting.pop_front();
                 }
                 if (s.q.empty()) break;   // the head of the queue does not fit yet
