@@ -437,13 +437,17 @@ int mgv_debug_xcc_histogram(uint32_t *out8, uint32_t grid, mgv_stream_t s);
  * mgv_hashjoin_probe_lds, then one block per slice copies the slice into
  * LDS, inserts the slice's rows with LDS compare-and-swap (each bucket's
  * slots claimed in order), and writes it back; rows whose chain runs past
- * their slice are inserted into the global table afterwards.  Existing
+ * their slice, and rows whose partition region was full (duplicate or
+ * skewed keys), are inserted into the global table afterwards.  Existing
  * contents of the table are kept.  No other build may write the same table
- * concurrently.  MGV_PAYLOAD_ROWID runs mgv_hashjoin_build.  Same bucket
- * and fan-out limits as mgv_hashjoin_probe_lds; n < 2^31; the workspace
- * (mgv_build_lds_workspace_size) holds both partitionings and the overflow
- * list (24 bytes per row).  Replaces hashjoin_builder::consume_warp
- * (src/operators/hashjoin.cu:41-60). */
+ * concurrently.  MGV_PAYLOAD_ROWID runs mgv_hashjoin_build.  The
+ * partitionings are count-free by default (as mgv_hashjoin_probe_lds);
+ * MGV_LAUNCH_COUNTED_PART selects the counted ones (same table contents,
+ * at most 2048 slices).  Same bucket and fan-out limits as
+ * mgv_hashjoin_probe_lds; n < 2^31; the workspace
+ * (mgv_build_lds_workspace_size, enough for either partitioning) holds both
+ * partitionings and the overflow list (about 40 bytes per row).  Replaces
+ * hashjoin_builder::consume_warp (src/operators/hashjoin.cu:41-60). */
 int mgv_build_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes);
 int mgv_hashjoin_build_lds(uint64_t *slots, uint32_t nbuckets, const int32_t *keys, const int32_t *payload,
                            int payload_mode, const int32_t *pred, int32_t bound, uint64_t n, int64_t row_base,
@@ -464,7 +468,8 @@ int mgv_hashjoin_build_lds(uint64_t *slots, uint32_t nbuckets, const int32_t *ke
  * nbuckets must split into power-of-two slices of at most 2048 buckets:
  * when nbuckets / fanout is such a slice there is one partitioning level
  * only, otherwise the second level cuts each of the fanout partitions into
- * a power of two of slices (at most 2048 in all).  n < 2^31.  The workspace
+ * a power of two of at most 1024 slices (at most 32768 in all; 2048 with
+ * MGV_LAUNCH_COUNTED_PART).  n < 2^31.  The workspace
  * (size from mgv_probe_lds_workspace_size) holds both partitionings (about
  * 32 bytes per row with their overflow areas).  The table must come from
  * mgv_hashjoin_build or a batch build (mgv_batch_run), which fill each

@@ -1013,7 +1013,13 @@ struct LdsBuildArgs {
     uint64_t *slots;
     uint32_t nbuckets;
     uint32_t slice;          // buckets per level-2 partition
-    PartLayout L;            // level-2 layout (key, payload)
+    PartLayout L;            // level-2 layout (key, payload) of the counted partitioning
+    uint32_t nparts;         // level-2 partitions
+    // fill partitioning: partition p holds rows [p * cap, p * cap + min(fill[p], cap))
+    // of fmix32(key) and payload; null: the counted layout L (keys as they are)
+    const unsigned long long *fill;
+    uint64_t cap;
+    const int32_t *fcol[2];
     uint32_t *next;          // partition counter (zeroed before the launch)
     unsigned long long *ovf_count;
     int32_t *ovf_key;        // rows whose chain leaves their slice
@@ -1055,31 +1061,41 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_build(LdsBuildArgs a) {
     __shared__ __align__(16) unsigned long long tab[kSliceMax * 8];
     __shared__ uint32_t s_part;
     const uint32_t t = threadIdx.x;
-    const int32_t *K = a.L.col[0];
-    const int32_t *V = a.L.col[1];
+    const bool hashed = a.fill != nullptr;
+    const int32_t *K = hashed ? a.fcol[0] : a.L.col[0];
+    const int32_t *V = hashed ? a.fcol[1] : a.L.col[1];
     uint64_t errs = 0;
     for (;;) {
         if (t == 0) s_part = atomicAdd(a.next, 1u);
         __syncthreads();
         const uint32_t p = s_part;
         __syncthreads();
-        if (p >= a.L.F) break;
-        const uint64_t r0 = a.L.pstart[p];
-        const uint64_t r1 = r0 + a.L.ptotal[p];
+        if (p >= a.nparts) break;
+        uint64_t r0, r1;
+        if (hashed) {
+            r0 = (uint64_t)p * a.cap;
+            const unsigned long long f = a.fill[p];
+            r1 = r0 + (f < a.cap ? f : a.cap);
+        } else {
+            r0 = a.L.pstart[p];
+            r1 = r0 + a.L.ptotal[p];
+        }
         if (r0 == r1) continue;   // an untouched slice is not copied
         const uint32_t lo = p * a.slice;
         v4u *g = reinterpret_cast<v4u *>(a.slots + (uint64_t)lo * 8);
         v4u *l = reinterpret_cast<v4u *>(tab);
         for (uint32_t i = t; i < a.slice * 4; i += kLdsBlock) l[i] = load_global(g + i);
         __syncthreads();
-        // r0 is a multiple of 64 rows; columns are padded past the partition end
+        // r0 is a multiple of 64 rows; columns are padded past the partition
+        // end (counted) or the region runs on to its capacity (fill)
         for (uint64_t i0 = r0 + 4ull * t; i0 < r1; i0 += 4ull * kLdsBlock) {
             const v4i k = load_stream4(K + i0);
             const v4i v = load_stream4(V + i0);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (i0 + j >= r1) break;
-                const uint32_t key = (uint32_t)q_get(k, j), pay = (uint32_t)q_get(v, j);
+                const uint32_t kh = (uint32_t)q_get(k, j), pay = (uint32_t)q_get(v, j);
+                const uint32_t key = hashed ? fmix32_inv(kh) : kh;
                 const uint64_t val = ((uint64_t)pay << 32) | key;
                 if (val == kEmptySlot) {
                     ++errs;
@@ -1087,7 +1103,8 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_build(LdsBuildArgs a) {
                 }
                 // home buckets lie in the slice (bucket / slice = level-2
                 // partition); anything else is finished in the global table
-                const uint32_t idx = bucket_of((int32_t)key, a.nbuckets) - lo;
+                const uint32_t idx =
+                    (hashed ? bucket_of_hash(kh, a.nbuckets) : bucket_of((int32_t)key, a.nbuckets)) - lo;
                 if (idx >= a.slice || !lds_insert(tab, a.slice, idx, val)) {
                     const unsigned long long o = atomic_add_global(a.ovf_count, 1ull);
                     a.ovf_key[o] = (int32_t)key;
@@ -1106,37 +1123,64 @@ __global__ __launch_bounds__(kLdsBlock) void k_lds_build(LdsBuildArgs a) {
 // The overflow rows, inserted into the global table after every slice is
 // written back (their chains continue into the next slices, or wrap around
 // to bucket 0).
-__global__ __launch_bounds__(256) void k_lds_build_overflow(LdsBuildArgs a) {
+// The rows of a fill partitioning's overflow areas (A, B: hashed keys) come
+// after the list.
+__global__ __launch_bounds__(256) void k_lds_build_overflow(LdsBuildArgs a, HashedRows A, HashedRows B) {
     OpBuild<256 / kWave> op;
     op.slots = a.slots;
     op.nbuckets = a.nbuckets;
     op.res = a.res;
     op.consume_open();
-    const uint64_t n = *a.ovf_count;
-    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
-        op.insert((uint32_t)a.ovf_key[i], (uint32_t)a.ovf_pay[i]);
+    const uint64_t nl = *a.ovf_count;
+    const unsigned long long fa = A.nrows ? *A.nrows : 0ull, fb = B.nrows ? *B.nrows : 0ull;
+    const uint64_t na = fa < A.ncap ? fa : A.ncap, nbr = fb < B.ncap ? fb : B.ncap;
+    const uint64_t n = nl + na + nbr;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        if (i < nl) {
+            op.insert((uint32_t)a.ovf_key[i], (uint32_t)a.ovf_pay[i]);
+        } else {
+            const uint64_t j = i - nl;
+            const HashedRows &h = j < na ? A : B;
+            const uint64_t r = j < na ? j : j - na;
+            op.insert(fmix32_inv((uint32_t)h.H[r]), (uint32_t)h.V[r]);
+        }
+    }
     op.consume_close();
 }
 
+// Slices of the count-free (fill) partitionings: level 2 cuts each of the F1
+// level-1 regions into at most kMaxFanoutWide / 2 slices (the largest
+// per-tile array of k_fill_scatter), and at most kMaxSlicesFill in all.  The
+// counted partitioning stops at kMaxFanoutWide slices.
+constexpr uint32_t kMaxSlicesFill = 32768;
+constexpr uint32_t kMaxSlicesPerRegion = kMaxFanoutWide / 2;
+
 // Level-2 fan-out for a table of nb buckets: the smallest power of two
-// F2 >= 2 * F1 that divides nb into slices of at most smax buckets.  When F1
-// alone already cuts the table into such slices (or F1 > kMaxFanout asks for
-// one level), there is one level only: F2 = F1.
-static int lds_fanout(uint32_t nb, uint32_t F1, uint32_t *F2, uint32_t smax = kSliceMax) {
+// F2 >= 2 * F1, at most maxf, that divides nb into slices of at most smax
+// buckets.  When F1 alone already cuts the table into such slices (or
+// F1 > kMaxFanout asks for one level), there is one level only: F2 = F1.
+// Returns false (and sets nothing) when there is no such F2.
+static bool fanout_of(uint32_t nb, uint32_t F1, uint32_t smax, uint32_t maxf, uint32_t *F2) {
     if (F1 > kMaxFanout || (nb % F1 == 0 && nb / F1 <= smax && nb / F1 > 0)) {
-        if (nb % F1 != 0 || nb / F1 > smax || nb / F1 == 0)
-            return fail(MGV_E_INVALID, "LDS probe: one level of %u partitions does not cut %u buckets into slices "
-                        "of <= %u buckets", F1, nb, smax);
+        if (nb % F1 != 0 || nb / F1 > smax || nb / F1 == 0) return false;
         *F2 = F1;
-        return MGV_OK;
+        return true;
     }
-    uint32_t f = 2 * F1;
-    while (f <= kMaxFanoutWide && (nb % f != 0 || nb / f > smax)) f <<= 1;
-    if (f > kMaxFanoutWide || nb % f != 0 || nb / f > smax || nb / f == 0)
-        return fail(MGV_E_INVALID, "LDS probe: %u buckets do not split into <= %u slices of <= %u buckets",
-                    nb, kMaxFanoutWide, smax);
-    *F2 = f;
-    return MGV_OK;
+    uint64_t f = 2ull * F1;
+    while (f <= maxf && (nb % f != 0 || nb / f > smax)) f <<= 1;
+    if (f > maxf || nb % f != 0 || nb / f > smax || nb / f == 0) return false;
+    *F2 = (uint32_t)f;
+    return true;
+}
+
+static int lds_fanout(uint32_t nb, uint32_t F1, uint32_t *F2, uint32_t smax = kSliceMax,
+                      uint32_t maxf = kMaxFanoutWide) {
+    if (fanout_of(nb, F1, smax, maxf, F2)) return MGV_OK;
+    if (F1 > kMaxFanout)
+        return fail(MGV_E_INVALID, "LDS probe: one level of %u partitions does not cut %u buckets into slices "
+                    "of <= %u buckets", F1, nb, smax);
+    return fail(MGV_E_INVALID, "LDS probe: %u buckets do not split into <= %u slices of <= %u buckets", nb, maxf,
+                smax);
 }
 
 // Workspace of the LDS paths: 256 bytes of counters, then the level-1 and
@@ -1168,9 +1212,12 @@ static int fill_probe_plan(uint64_t n, uint32_t nb, uint32_t F1, void *ws, uint6
                            FillLayout *L2, uint32_t *F2, uint64_t *need, uint32_t smax = kSliceMax) {
     if (F1 < 2 || F1 > kMaxFanoutWide || (F1 & (F1 - 1)))
         return fail(MGV_E_INVALID, "LDS probe: level-1 fan-out %u not a power of two in [2,%u]", F1, kMaxFanoutWide);
-    int rc = lds_fanout(nb, F1, F2, smax);
+    int rc = lds_fanout(nb, F1, F2, smax, kMaxSlicesFill);
     if (rc) return rc;
     const bool one = *F2 == F1;
+    if (!one && *F2 / F1 > kMaxSlicesPerRegion)
+        return fail(MGV_E_INVALID, "LDS probe: %u buckets need %u slices per level-1 region (at most %u); raise the "
+                    "fan-out", nb, *F2 / F1, kMaxSlicesPerRegion);
     char *b = (char *)ws;
     unsigned long long *ctr = (unsigned long long *)ws;
     const uint64_t counters = (256 + (uint64_t)(F1 + *F2) * 8 + 255) / 256 * 256;
@@ -1202,13 +1249,49 @@ static int build_plan(uint64_t n, uint32_t nb, uint32_t F1, void *ws, uint64_t w
     return MGV_OK;
 }
 
-// Workspace of either partitioning (a caller's workspace serves both).
+// Fill-partitioned build workspace: the fill probe's (counters, both
+// partitionings of n rows) followed by the chain-overflow list (n keys, n
+// payloads); the list's count is counter word 3.
+static int build_fill_plan(uint64_t n, uint32_t nb, uint32_t F1, void *ws, uint64_t ws_bytes, FillLayout *L1,
+                           FillLayout *L2, uint32_t *F2, int32_t **okey, int32_t **opay, uint64_t *need) {
+    uint64_t base = 0;
+    int rc = fill_probe_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, F2, &base);
+    if (rc) return rc;
+    base = (base + 255) / 256 * 256;
+    const uint64_t ob = (n * 4 + 255) / 256 * 256;
+    if (need) *need = base + 2 * ob;
+    if (!L1) return MGV_OK;
+    if (!ws || ws_bytes < base + 2 * ob) return fail(MGV_E_CAPACITY, "LDS build: workspace too small");
+    if ((rc = fill_probe_plan(n, nb, F1, ws, base, L1, L2, F2, nullptr))) return rc;
+    *okey = (int32_t *)((char *)ws + base);
+    *opay = (int32_t *)((char *)ws + base + ob);
+    return MGV_OK;
+}
+
+// Workspace of either build (the counted one only where its fan-out limit
+// allows it).
+static int build_ws_need(uint64_t n, uint32_t nb, uint32_t F1, uint64_t *need) {
+    uint32_t F2 = 0;
+    uint64_t a = 0, b = 0;
+    int rc = build_fill_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, &F2, nullptr, nullptr, &b);
+    if (rc) return rc;
+    if (fanout_of(nb, F1, kSliceMax, kMaxFanoutWide, &F2) &&
+        (rc = build_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, &F2, nullptr, nullptr, &a)))
+        return rc;
+    *need = a > b ? a : b;
+    return MGV_OK;
+}
+
+// Workspace of either partitioning (a caller's workspace serves both; a
+// table too large for the counted one gets the fill partitioning's).
 static int probe_ws_need(uint64_t n, uint32_t nb, uint32_t F1, uint64_t *need) {
     uint32_t F2 = 0;
     uint64_t a = 0, b = 0;
-    int rc = lds_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, &F2, &a);
+    int rc = fill_probe_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, &F2, &b);
     if (rc) return rc;
-    if ((rc = fill_probe_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, &F2, &b))) return rc;
+    if (F1 >= 2 && F1 <= kMaxFanoutWide && !(F1 & (F1 - 1)) && fanout_of(nb, F1, kSliceMax, kMaxFanoutWide, &F2) &&
+        (rc = lds_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, &F2, &a)))
+        return rc;
     *need = a > b ? a : b;
     return MGV_OK;
 }
@@ -1223,6 +1306,50 @@ static bool tag_geometry(uint32_t nbuckets, TagGeom *g) {
     g->lowmask = (1u << g->L) - 1u;
     g->jmax = (1u << (B - 16)) - 2u;
     return true;
+}
+
+// CUs of the current device (256 if unknown).
+static uint32_t device_cus() {
+    DeviceState *ds = current_state();
+    return (ds && ds->cus > 0) ? (uint32_t)ds->cus : 256u;
+}
+
+// Launches the count-free partitioning of n rows (key hashed with fmix32,
+// val, optional filter pred <= bound) into the layouts of fill_probe_plan:
+// level 1 into L1's F1 regions, then each region into F2 / F1 regions of L2;
+// or, when L1.F == 0, one level straight into L2.
+static int fill_partition(const FillLayout &L1, const FillLayout &L2, uint32_t F1, uint32_t F2, const int32_t *keys,
+                          const int32_t *vals, const int32_t *pred, int32_t bound, uint64_t n, hipStream_t s) {
+    int rc;
+    const uint64_t gmax = (uint64_t)kFillBlocksPerCU * device_cus();
+    // blocks for a scatter of up to `rows` rows in tiles of `tile` (plus one
+    // partial tile per upstream region)
+    auto grid_for = [&](uint64_t rows, int tile, uint32_t regions) {
+        const uint64_t t = (rows + tile - 1) / tile + regions;
+        return (uint32_t)(t < gmax ? t : gmax);
+    };
+    if (L1.F) {
+        hipLaunchKernelGGL((k_fill_scatter<false, kMaxFanout, kFillTileWide>), dim3(grid_for(n, kFillTileWide, 0)),
+                           dim3(kFillBlock), 0, s, L1, L1, keys, vals, pred, bound, n);
+        if ((rc = check_launch("k_fill_scatter (level 1)"))) return rc;
+        // a level-2 tile spans F2 / F1 partitions: small per-tile arrays leave
+        // LDS for the wide tile
+        const uint32_t S = F2 / F1;
+        if (S <= 64)
+            hipLaunchKernelGGL((k_fill_scatter<true, 64, kFillTileWide>), dim3(grid_for(n, kFillTileWide, F1)),
+                               dim3(kFillBlock), 0, s, L2, L1, nullptr, nullptr, nullptr, 0, 0ull);
+        else if (S <= 256)
+            hipLaunchKernelGGL((k_fill_scatter<true, 256, kFillTileWide>), dim3(grid_for(n, kFillTileWide, F1)),
+                               dim3(kFillBlock), 0, s, L2, L1, nullptr, nullptr, nullptr, 0, 0ull);
+        else
+            hipLaunchKernelGGL((k_fill_scatter<true, kMaxSlicesPerRegion, kFillTile>),
+                               dim3(grid_for(n, kFillTile, F1)), dim3(kFillBlock), 0, s, L2, L1, nullptr, nullptr,
+                               nullptr, 0, 0ull);
+        return check_launch("k_fill_scatter (level 2)");
+    }
+    hipLaunchKernelGGL((k_fill_scatter<false, kMaxFanoutWide, kFillTile>), dim3(grid_for(n, kFillTile, 0)),
+                       dim3(kFillBlock), 0, s, L2, L2, keys, vals, pred, bound, n);
+    return check_launch("k_fill_scatter");
 }
 
 // The fill-partitioned probe: no count passes; level 1 by fmix32(key) into
@@ -1244,38 +1371,8 @@ static int probe_lds_fill(const uint64_t *slots, uint32_t nbuckets, const int32_
     if ((rc = fill_probe_plan(n, nbuckets, fanout, ws, ws_bytes, &L1, &L2, &F2, nullptr, NB))) return rc;
     const uint64_t counters = (256 + (uint64_t)(fanout + F2) * 8 + 255) / 256 * 256;
     if ((rc = mgv_fill_u64((uint64_t *)ws, counters / 8, 0, (mgv_stream_t)s))) return rc;
-    DeviceState *ds = current_state();
-    const uint32_t cus = (ds && ds->cus > 0) ? (uint32_t)ds->cus : 256u;
-    const uint64_t gmax = (uint64_t)kFillBlocksPerCU * cus;
-    // blocks for a scatter of up to `rows` rows in tiles of `tile` (plus one
-    // partial tile per upstream region)
-    auto grid_for = [&](uint64_t rows, int tile, uint32_t regions) {
-        const uint64_t t = (rows + tile - 1) / tile + regions;
-        return (uint32_t)(t < gmax ? t : gmax);
-    };
-    if (L1.F) {
-        hipLaunchKernelGGL((k_fill_scatter<false, kMaxFanout, kFillTileWide>), dim3(grid_for(n, kFillTileWide, 0)),
-                           dim3(kFillBlock), 0, s, L1, L1, keys, vals, pred, bound, n);
-        if ((rc = check_launch("k_fill_scatter (level 1)"))) return rc;
-        // a level-2 tile spans F2 / F1 partitions: small per-tile arrays leave
-        // LDS for the wide tile
-        const uint32_t S = F2 / fanout;
-        if (S <= 64)
-            hipLaunchKernelGGL((k_fill_scatter<true, 64, kFillTileWide>), dim3(grid_for(n, kFillTileWide, fanout)),
-                               dim3(kFillBlock), 0, s, L2, L1, nullptr, nullptr, nullptr, 0, 0ull);
-        else if (S <= 256)
-            hipLaunchKernelGGL((k_fill_scatter<true, 256, kFillTileWide>), dim3(grid_for(n, kFillTileWide, fanout)),
-                               dim3(kFillBlock), 0, s, L2, L1, nullptr, nullptr, nullptr, 0, 0ull);
-        else
-            hipLaunchKernelGGL((k_fill_scatter<true, kMaxFanoutWide / 2, kFillTile>),
-                               dim3(grid_for(n, kFillTile, fanout)), dim3(kFillBlock), 0, s, L2, L1, nullptr, nullptr,
-                               nullptr, 0, 0ull);
-        if ((rc = check_launch("k_fill_scatter (level 2)"))) return rc;
-    } else {
-        hipLaunchKernelGGL((k_fill_scatter<false, kMaxFanoutWide, kFillTile>), dim3(grid_for(n, kFillTile, 0)),
-                           dim3(kFillBlock), 0, s, L2, L2, keys, vals, pred, bound, n);
-        if ((rc = check_launch("k_fill_scatter"))) return rc;
-    }
+    if ((rc = fill_partition(L1, L2, fanout, F2, keys, vals, pred, bound, n, s))) return rc;
+    const uint32_t cus = device_cus();
     LdsProbeArgs a = {};
     a.slots = slots;
     a.nbuckets = nbuckets;
@@ -1434,8 +1531,7 @@ int mgv_hashjoin_probe_groupby_lds(const uint64_t *slots, uint32_t nbuckets, con
 
 int mgv_build_lds_workspace_size(uint64_t n, uint32_t nbuckets, uint32_t fanout, uint64_t *bytes) {
     if (!bytes || nbuckets == 0) return fail(MGV_E_INVALID, "mgv_build_lds_workspace_size: bad argument");
-    uint32_t F2 = 0;
-    return build_plan(n, nbuckets, fanout, nullptr, 0, nullptr, nullptr, &F2, nullptr, nullptr, bytes);
+    return build_ws_need(n, nbuckets, fanout, bytes);
 }
 
 int mgv_hashjoin_build_lds(uint64_t *slots, uint32_t nbuckets, const int32_t *keys, const int32_t *payload,
@@ -1453,13 +1549,49 @@ int mgv_hashjoin_build_lds(uint64_t *slots, uint32_t nbuckets, const int32_t *ke
     if (payload_mode == MGV_PAYLOAD_COLUMN && !payload) return fail(MGV_E_INVALID, "build_lds: payload column missing");
     if (payload_mode != MGV_PAYLOAD_COLUMN && payload_mode != MGV_PAYLOAD_NONE)
         return fail(MGV_E_INVALID, "build_lds: bad payload mode %d", payload_mode);
-    PartLayout L1, L2;
-    uint32_t F2 = 0;
-    int32_t *okey = nullptr, *opay = nullptr;
-    int rc = build_plan(n, nbuckets, fanout, ws, ws_bytes, &L1, &L2, &F2, &okey, &opay, nullptr);
-    if (rc) return rc;
     hipStream_t s = stream_of(st);
     const int32_t *pcol = payload_mode == MGV_PAYLOAD_COLUMN ? payload : nullptr;   // NONE: zeros
+    for (const int32_t *c : {pred, keys, pcol})
+        if (c && !aligned16(c)) return fail(MGV_E_ALIGN, "build_lds: column is not 16-byte aligned");
+    uint32_t F2 = 0;
+    int32_t *okey = nullptr, *opay = nullptr;
+    int rc;
+    if (!(LC.reserved & MGV_LAUNCH_COUNTED_PART)) {
+        FillLayout F1L = {}, F2L = {};
+        if ((rc = build_fill_plan(n, nbuckets, fanout, ws, ws_bytes, &F1L, &F2L, &F2, &okey, &opay, nullptr)))
+            return rc;
+        const uint64_t counters = (256 + (uint64_t)(fanout + F2) * 8 + 255) / 256 * 256;
+        if ((rc = mgv_fill_u64((uint64_t *)ws, counters / 8, 0, (mgv_stream_t)s))) return rc;
+        if ((rc = fill_partition(F1L, F2L, fanout, F2, keys, pcol, pred, bound, n, s))) return rc;
+        LdsBuildArgs a = {};
+        a.slots = slots;
+        a.nbuckets = nbuckets;
+        a.slice = nbuckets / F2;
+        a.nparts = F2;
+        a.fill = F2L.fill;
+        a.cap = F2L.cap;
+        a.fcol[0] = F2L.col[0];
+        a.fcol[1] = F2L.col[1];
+        a.next = (uint32_t *)ws;
+        a.ovf_count = (unsigned long long *)ws + 3;
+        a.ovf_key = okey;
+        a.ovf_pay = opay;
+        a.res = res;
+        static thread_local int cached = 0;
+        if (!cached) cached = persistent_blocks(k_lds_build, kLdsBlock);
+        hipLaunchKernelGGL(k_lds_build, dim3(LC.grid ? LC.grid : (uint32_t)cached), dim3(kLdsBlock), 0, s, a);
+        if ((rc = check_launch("k_lds_build"))) return rc;
+        // rows whose region was full at either level (duplicate or skewed
+        // keys), then rows whose chain left their slice
+        const HashedRows o2 = {F2L.col[0] + F2L.ostart, F2L.col[1] + F2L.ostart, F2L.ofill, F2L.ocap};
+        const HashedRows o1 = F1L.F ? HashedRows{F1L.col[0] + F1L.ostart, F1L.col[1] + F1L.ostart, F1L.ofill,
+                                                 F1L.ocap}
+                                    : HashedRows{nullptr, nullptr, nullptr, 0};
+        hipLaunchKernelGGL(k_lds_build_overflow, dim3(4 * device_cus()), dim3(256), 0, s, a, o2, o1);
+        return check_launch("k_lds_build_overflow");
+    }
+    PartLayout L1, L2;
+    if ((rc = build_plan(n, nbuckets, fanout, ws, ws_bytes, &L1, &L2, &F2, &okey, &opay, nullptr))) return rc;
     PartIn in1 = {pred, bound, keys, pcol, 0, 0u, 0u, 0u, nullptr, nullptr};
     if (L1.F == 0) {
         if ((rc = partition_rows(L2, in1, s, scatter_shape(LC)))) return rc;
@@ -1470,12 +1602,23 @@ int mgv_hashjoin_build_lds(uint64_t *slots, uint32_t nbuckets, const int32_t *ke
     }
     uint32_t *ctr = (uint32_t *)ws;   // [0]: partition counter, [2..3]: overflow count
     if ((rc = mgv_fill_u32(ctr, 4, 0, (mgv_stream_t)s))) return rc;
-    LdsBuildArgs a = {slots, nbuckets, nbuckets / F2, L2, ctr, (unsigned long long *)(ctr + 2), okey, opay, res};
+    LdsBuildArgs a = {};
+    a.slots = slots;
+    a.nbuckets = nbuckets;
+    a.slice = nbuckets / F2;
+    a.L = L2;
+    a.nparts = L2.F;
+    a.next = ctr;
+    a.ovf_count = (unsigned long long *)(ctr + 2);
+    a.ovf_key = okey;
+    a.ovf_pay = opay;
+    a.res = res;
     static thread_local int cached = 0;
     if (!cached) cached = persistent_blocks(k_lds_build, kLdsBlock);
     hipLaunchKernelGGL(k_lds_build, dim3(LC.grid ? LC.grid : (uint32_t)cached), dim3(kLdsBlock), 0, s, a);
     if ((rc = check_launch("k_lds_build"))) return rc;
-    hipLaunchKernelGGL(k_lds_build_overflow, dim3(256), dim3(256), 0, s, a);
+    const HashedRows none = {nullptr, nullptr, nullptr, 0};
+    hipLaunchKernelGGL(k_lds_build_overflow, dim3(256), dim3(256), 0, s, a, none, none);
     return check_launch("k_lds_build_overflow");
 }
 

@@ -34,7 +34,10 @@ def build_direct(ht, keys, pay, mode=mgv.MGV_PAYLOAD_COLUMN, pred=None, bound=0)
     return res.read().errors
 
 
-def build_lds(ht, keys, pay, mode=mgv.MGV_PAYLOAD_COLUMN, pred=None, bound=0, fanout=4, shape=2):
+COUNTED = 1 << 12   # MGV_LAUNCH_COUNTED_PART: counted partitionings instead of the count-free ones
+
+
+def build_lds(ht, keys, pay, mode=mgv.MGV_PAYLOAD_COLUMN, pred=None, bound=0, fanout=4, shape=2, flags=0):
     n = len(keys)
     need = C.c_uint64()
     mgv.call("mgv_build_lds_workspace_size", n, ht.nbuckets, fanout, C.byref(need))
@@ -45,7 +48,7 @@ def build_lds(ht, keys, pay, mode=mgv.MGV_PAYLOAD_COLUMN, pred=None, bound=0, fa
     res = mgv.ResultBuffer()
     mgv.call("mgv_hashjoin_build_lds", ht.buf.vp, ht.nbuckets, dk.vp, dp.vp if dp else None, mode,
              dc.vp if dc else None, bound, n, 0, fanout, ws.vp, ws.nbytes, res.vp,
-             mgv.Launch(0, 1024, -1, shape << 8), None)
+             mgv.Launch(0, 1024, -1, (shape << 8) | flags), None)
     mgv.sync()
     return res.read().errors
 
@@ -71,15 +74,16 @@ def probe(ht, rk, rv):
     return r.count, r.sum
 
 
+@pytest.mark.parametrize("flags", [0, COUNTED], ids=["fill", "counted"])
 @pytest.mark.parametrize("n,lf,dup,fanout", [(1 << 18, 0.5, False, 4), (1 << 20, 0.25, False, 16),
                                              (300000, 0.9, False, 2), (300000, 0.9, True, 2),
                                              (120000, 0.75, True, 8)])
-def test_lds_build_matches_direct_build_and_oracle(gpu, orc, n, lf, dup, fanout):
+def test_lds_build_matches_direct_build_and_oracle(gpu, orc, n, lf, dup, fanout, flags):
     keys = orc.gen_range(n, orc.seed(7, 0), 3, n // 40) if dup else orc.gen_perm(n, n, orc.seed(7, 0), 3)
     pay = orc.gen_bits(n, orc.seed(7, 1), 3, 16)
     a, b = mgv.HashTable(n, lf), mgv.HashTable(n, lf)
     assert build_direct(a, keys, pay) == 0
-    assert build_lds(b, keys, pay, fanout=fanout) == 0
+    assert build_lds(b, keys, pay, fanout=fanout, flags=flags) == 0
     assert np.array_equal(occupied(a), occupied(b))
     assert_prefix_filled(b)
     rk = orc.gen_range(4 * n, orc.seed(7, 2), 3, 2 * max(1, int(keys.max()) + 1))
@@ -88,6 +92,30 @@ def test_lds_build_matches_direct_build_and_oracle(gpu, orc, n, lf, dup, fanout)
     want = orc.probe(t, rk, rv)
     orc.free(t)
     assert probe(b, rk, rv) == probe(a, rk, rv) == (want["count"], want["sum"])
+
+
+def test_lds_build_more_slices_than_the_counted_partitioning_takes(gpu, orc):
+    """2^23 buckets (4096 slices of 2048) and 2^24 buckets (8192 slices):
+    beyond the counted partitioning's 2048 slices, so the count-free one
+    builds them (level 2 cutting each level-1 region into 128, then 1024
+    slices); the table equals the direct build's and answers probes as the
+    direct build's does.  Low load factors keep the key columns small."""
+    for nkeys, lf, F1 in (((1 << 22), 0.0625, 32), ((1 << 21), 0.015625, 8)):
+        keys = orc.gen_perm(nkeys, nkeys, orc.seed(7, 9), 9)
+        pay = orc.gen_bits(nkeys, orc.seed(7, 10), 9, 16)
+        a, b = mgv.HashTable(nkeys, lf), mgv.HashTable(nkeys, lf)
+        assert b.nbuckets in (1 << 23, 1 << 24)
+        with pytest.raises(mgv.MgvError):   # the counted partitioning stops at 2048 slices
+            build_lds(b, keys, pay, fanout=F1, flags=COUNTED)
+        assert build_direct(a, keys, pay) == 0
+        b.clear()
+        assert build_lds(b, keys, pay, fanout=F1) == 0
+        assert np.array_equal(occupied(a), occupied(b))
+        assert_prefix_filled(b)
+        rk = orc.gen_range(1 << 22, orc.seed(7, 11), 9, 2 * nkeys)
+        rv = orc.gen_bits(1 << 22, orc.seed(7, 12), 9, 16)
+        assert probe(b, rk, rv) == probe(a, rk, rv)
+        del a, b
 
 
 def test_lds_build_chains_wrap_around_the_table_end(gpu, orc):

@@ -96,6 +96,33 @@ def test_lds_probe_fill_and_counted_partitionings_agree(gpu, orc, nb_keys, n, fa
     assert got == (want["count"], want["sum"]) == probe_lds(ht, rk, rv, fanout=fanout, flags=COUNTED)
 
 
+@pytest.mark.parametrize("nb_keys,lf,fanout", [(1 << 22, 0.0625, 32), (1 << 21, 0.015625, 8)])
+def test_lds_probe_more_slices_than_the_counted_partitioning_takes(gpu, orc, nb_keys, lf, fanout):
+    """2^23 buckets (4096 slices of 2048 buckets: level 2 cuts each of 32
+    regions into 128) and 2^24 buckets (8192 slices: 8 regions into 1024):
+    the count-free partitioning takes them, the counted one (2048 slices at
+    most) refuses them, and COUNT and SUM equal the oracle's and the direct
+    probe's, with and without a filter."""
+    sk = orc.gen_perm(nb_keys, nb_keys, orc.seed(2, 0), 15)
+    sw = orc.gen_bits(nb_keys, orc.seed(2, 1), 15, 16)
+    n = 1 << 22
+    rk = orc.gen_range(n, orc.seed(2, 2), 15, 2 * nb_keys)
+    rv = orc.gen_bits(n, orc.seed(2, 3), 15, 16)
+    c = orc.gen_bits(n, orc.seed(2, 4), 15, 31)
+    ht = build(sk, sw, lf)
+    assert ht.nbuckets in (1 << 23, 1 << 24)
+    t = orc.table(sk, sw)
+    want = orc.probe(t, rk, rv)
+    bound = (1 << 30) - 1
+    sel = c <= bound
+    want_f = orc.probe(t, rk[sel], rv[sel])
+    orc.free(t)
+    assert probe_lds(ht, rk, rv, fanout=fanout) == (want["count"], want["sum"]) == probe_direct(ht, rk, rv)
+    assert probe_lds(ht, rk, rv, pred=c, bound=bound, fanout=fanout) == (want_f["count"], want_f["sum"])
+    with pytest.raises(mgv.MgvError):
+        probe_lds(ht, rk, rv, fanout=fanout, flags=COUNTED)
+
+
 @pytest.mark.parametrize("kind", ["one_key", "zipf"])
 @pytest.mark.parametrize("nb_keys,n", [(1 << 16, 1 << 20), (1 << 20, 1 << 21)])
 def test_lds_probe_skewed_keys_overflow_their_regions(gpu, orc, kind, nb_keys, n):
@@ -214,8 +241,11 @@ def test_lds_probe_few_blocks_and_rejects_unsplittable_tables(gpu, orc):
     need = C.c_uint64()
     with pytest.raises(mgv.MgvError):
         mgv.call("mgv_probe_lds_workspace_size", 1000, 25, 2, C.byref(need))     # odd bucket count
-    with pytest.raises(mgv.MgvError):
-        mgv.call("mgv_probe_lds_workspace_size", 1000, 1 << 24, 64, C.byref(need))  # slices above 2048 buckets
+    with pytest.raises(mgv.MgvError):   # 8192 slices: 4096 per level-1 region, above 1024
+        mgv.call("mgv_probe_lds_workspace_size", 1000, 1 << 24, 2, C.byref(need))
+    with pytest.raises(mgv.MgvError):   # 65536 slices, above 32768
+        mgv.call("mgv_probe_lds_workspace_size", 1000, 1 << 27, 64, C.byref(need))
+    mgv.call("mgv_probe_lds_workspace_size", 1000, 1 << 24, 64, C.byref(need))   # 8192 slices, 128 per region
 
 
 @pytest.mark.parametrize("shape", [0, 1, 2, 3])
