@@ -97,13 +97,14 @@ constexpr uint64_t kLdsBuildRows = 1ull << 21;
 constexpr uint32_t kLdsBuildFanout = 32;
 
 // Work items of a request: a wave runs an item's rows one 256-row chunk
-// after another, so a request cut into few large items runs on few waves
-// (a 2^16-row build in one 65536-row item is 256 dependent chunks of
-// compare-and-swaps on one wave).  Each request is therefore cut into at
-// least kSpreadItems items while they hold at least kMinItemRows rows, and
-// into items of at most vector_rows rows.
-constexpr uint64_t kSpreadItems = 16384;
-constexpr uint64_t kMinItemRows = 1024;
+// after another (a few microseconds per chunk of random probes or
+// compare-and-swaps), and every item ends with a flush whose atomics on the
+// request's one result serialise (about 20 ns per item, measured).  A
+// request of n rows in k items then takes about (n / k) * c + k * f with
+// c / f ~ 1/2 per row, least at k = sqrt(n / 2): a 2^16-row build gets 128
+// items, a 2^24-row probe about 2900, and from 2^27 rows on the items are
+// capped at kMaxItems (every resident wave busy) or at vector_rows rows.
+constexpr uint64_t kMaxItems = 8192;
 
 // A batch's results are published together when its last launch ends, so a
 // small request batched with a large one waits for the large one.  A batch
@@ -184,11 +185,11 @@ struct Server {
     }
 
     // rows per work item of a request of `rows` rows: a multiple of 256 in
-    // [min(kMinItemRows, vector_rows), vector_rows]
+    // [256, vector_rows]
     uint32_t item_rows(uint64_t rows) const {
-        const uint64_t lo = std::min<uint64_t>(kMinItemRows, cfg.vector_rows);
-        const uint64_t want = align_up((rows + kSpreadItems - 1) / kSpreadItems, 256);
-        return (uint32_t)std::min<uint64_t>(cfg.vector_rows, std::max(lo, want));
+        const uint64_t k = std::max<uint64_t>(1, std::min<uint64_t>(kMaxItems, (uint64_t)std::sqrt(rows / 2.0)));
+        const uint64_t want = align_up((rows + k - 1) / k, 256);
+        return (uint32_t)std::min<uint64_t>(cfg.vector_rows, std::max<uint64_t>(256, want));
     }
 
     uint64_t items_of(uint64_t rows) const {
