@@ -394,8 +394,8 @@ constexpr uint32_t kQueue = 128;   // entries per wave (a power of two)
 
 // An entry's bkt is the bucket to probe, or (bit 31 set) the bucket whose
 // successor is to be probed: a row whose LDS home bucket was full is queued
-// without computing the successor (tables of the LDS path have at most 2^22
-// buckets, so bit 31 is free).
+// without computing the successor (tables of the LDS path have at most
+// 32768 slices of 2048 buckets, 2^26 buckets, so bit 31 is free).
 constexpr uint32_t kAfter = 1u << 31;
 
 struct WaveQueue {
@@ -1057,8 +1057,9 @@ __device__ __forceinline__ bool lds_insert(unsigned long long *tab, uint32_t sli
 // table slice into LDS, inserts the partition's rows (all of which have
 // their home bucket in the slice), and writes the slice back.  Rows whose
 // chain runs past the slice go to the overflow list.
+template <uint32_t NB>
 __global__ __launch_bounds__(kLdsBlock) void k_lds_build(LdsBuildArgs a) {
-    __shared__ __align__(16) unsigned long long tab[kSliceMax * 8];
+    __shared__ __align__(16) unsigned long long tab[NB * 8];
     __shared__ uint32_t s_part;
     const uint32_t t = threadIdx.x;
     const bool hashed = a.fill != nullptr;
@@ -1249,20 +1250,34 @@ static int build_plan(uint64_t n, uint32_t nb, uint32_t F1, void *ws, uint64_t w
     return MGV_OK;
 }
 
+// Slices of the fill-partitioned build: 1024 buckets (64 KiB of slots, so
+// two blocks share a CU and one inserts while the other copies), or 2048
+// when 1024-bucket slices would be too many.
+constexpr uint32_t kBuildSlice = 1024;
+static uint32_t build_slice(uint32_t nb, uint32_t F1) {
+    uint32_t F2 = 0;
+    if (F1 >= 2 && !(F1 & (F1 - 1)) && fanout_of(nb, F1, kBuildSlice, kMaxSlicesFill, &F2) &&
+        (F2 == F1 || F2 / F1 <= kMaxSlicesPerRegion))
+        return kBuildSlice;
+    return kSliceMax;
+}
+
 // Fill-partitioned build workspace: the fill probe's (counters, both
-// partitionings of n rows) followed by the chain-overflow list (n keys, n
-// payloads); the list's count is counter word 3.
+// partitionings of n rows, slices of build_slice buckets) followed by the
+// chain-overflow list (n keys, n payloads); the list's count is counter
+// word 3.
 static int build_fill_plan(uint64_t n, uint32_t nb, uint32_t F1, void *ws, uint64_t ws_bytes, FillLayout *L1,
                            FillLayout *L2, uint32_t *F2, int32_t **okey, int32_t **opay, uint64_t *need) {
     uint64_t base = 0;
-    int rc = fill_probe_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, F2, &base);
+    const uint32_t smax = build_slice(nb, F1);
+    int rc = fill_probe_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, F2, &base, smax);
     if (rc) return rc;
     base = (base + 255) / 256 * 256;
     const uint64_t ob = (n * 4 + 255) / 256 * 256;
     if (need) *need = base + 2 * ob;
     if (!L1) return MGV_OK;
     if (!ws || ws_bytes < base + 2 * ob) return fail(MGV_E_CAPACITY, "LDS build: workspace too small");
-    if ((rc = fill_probe_plan(n, nb, F1, ws, base, L1, L2, F2, nullptr))) return rc;
+    if ((rc = fill_probe_plan(n, nb, F1, ws, base, L1, L2, F2, nullptr, smax))) return rc;
     *okey = (int32_t *)((char *)ws + base);
     *opay = (int32_t *)((char *)ws + base + ob);
     return MGV_OK;
@@ -1577,9 +1592,17 @@ int mgv_hashjoin_build_lds(uint64_t *slots, uint32_t nbuckets, const int32_t *ke
         a.ovf_key = okey;
         a.ovf_pay = opay;
         a.res = res;
-        static thread_local int cached = 0;
-        if (!cached) cached = persistent_blocks(k_lds_build, kLdsBlock);
-        hipLaunchKernelGGL(k_lds_build, dim3(LC.grid ? LC.grid : (uint32_t)cached), dim3(kLdsBlock), 0, s, a);
+        if (a.slice <= kBuildSlice) {
+            static thread_local int cached = 0;
+            if (!cached) cached = persistent_blocks(k_lds_build<kBuildSlice>, kLdsBlock);
+            hipLaunchKernelGGL(k_lds_build<kBuildSlice>, dim3(LC.grid ? LC.grid : (uint32_t)cached),
+                               dim3(kLdsBlock), 0, s, a);
+        } else {
+            static thread_local int cached = 0;
+            if (!cached) cached = persistent_blocks(k_lds_build<kSliceMax>, kLdsBlock);
+            hipLaunchKernelGGL(k_lds_build<kSliceMax>, dim3(LC.grid ? LC.grid : (uint32_t)cached), dim3(kLdsBlock),
+                               0, s, a);
+        }
         if ((rc = check_launch("k_lds_build"))) return rc;
         // rows whose region was full at either level (duplicate or skewed
         // keys), then rows whose chain left their slice
@@ -1614,8 +1637,8 @@ int mgv_hashjoin_build_lds(uint64_t *slots, uint32_t nbuckets, const int32_t *ke
     a.ovf_pay = opay;
     a.res = res;
     static thread_local int cached = 0;
-    if (!cached) cached = persistent_blocks(k_lds_build, kLdsBlock);
-    hipLaunchKernelGGL(k_lds_build, dim3(LC.grid ? LC.grid : (uint32_t)cached), dim3(kLdsBlock), 0, s, a);
+    if (!cached) cached = persistent_blocks(k_lds_build<kSliceMax>, kLdsBlock);
+    hipLaunchKernelGGL(k_lds_build<kSliceMax>, dim3(LC.grid ? LC.grid : (uint32_t)cached), dim3(kLdsBlock), 0, s, a);
     if ((rc = check_launch("k_lds_build"))) return rc;
     const HashedRows none = {nullptr, nullptr, nullptr, 0};
     hipLaunchKernelGGL(k_lds_build_overflow, dim3(256), dim3(256), 0, s, a, none, none);
