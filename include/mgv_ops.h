@@ -157,7 +157,7 @@ typedef struct mgv_serve_config {
     uint64_t budget_bytes;     /* HBM budget for hash tables + group arrays */
     uint32_t max_inflight;     /* batches on the GPU at once (one stream each), 1..8 */
     uint32_t max_batch;        /* queries per batch, 1..1024 */
-    uint32_t vector_rows;      /* work-item size of the fused kernel */
+    uint32_t vector_rows;      /* largest work item of the fused kernel (a multiple of 256) */
     uint32_t reserved;
     double load_factor;        /* hash tables */
     mgv_launch launch;
@@ -177,8 +177,11 @@ typedef struct mgv_serve_stats {
 /* Continuous-batching request loop.  Requests (sorted by arrival_s) are
  * released at their arrival time; a waiting request is admitted when its
  * footprint (hash table at load_factor, or group arrays, + result) fits the
- * free HBM budget and a stream slot is free; the admitted requests form one
- * batch: one persistent launch for its builds and one for its probes and
+ * free HBM budget and a stream slot is free; the admitted requests (while
+ * their probe, group-by and build rows add up to at most 2^26; a larger
+ * request runs alone) form one batch, each request cut into about
+ * sqrt(rows / 2) work items (at most 8192, of at most vector_rows rows):
+ * one persistent launch for its builds and one for its probes and
  * group-bys (the fused kernel of mgv_batch_run; a group-by over more than
  * 4096 groups runs as mgv_groupby_partitioned, its workspace counted in its
  * footprint), after which the results are copied to pinned host memory
