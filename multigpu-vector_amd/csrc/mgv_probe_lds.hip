@@ -33,6 +33,7 @@ namespace mgv {
 
 constexpr uint32_t kSliceMax = 2048;     // buckets of a slice in LDS (128 KiB)
 constexpr uint32_t kSliceExtra = 8;      // buckets loaded past the slice
+constexpr uint32_t kProbeSlice = kSliceMax;   // slice of mgv_hashjoin_probe_lds (count-free)
 constexpr int kLdsBlock = 1024;
 // Quads (4 rows) each lane probes per step; the next step's quads are loaded
 // while these are probed.  Measured on the 2^28-row probe: 2 per step ran
@@ -1302,7 +1303,7 @@ static int build_ws_need(uint64_t n, uint32_t nb, uint32_t F1, uint64_t *need) {
 static int probe_ws_need(uint64_t n, uint32_t nb, uint32_t F1, uint64_t *need) {
     uint32_t F2 = 0;
     uint64_t a = 0, b = 0;
-    int rc = fill_probe_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, &F2, &b);
+    int rc = fill_probe_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, &F2, &b, kProbeSlice);
     if (rc) return rc;
     if (F1 >= 2 && F1 <= kMaxFanoutWide && !(F1 & (F1 - 1)) && fanout_of(nb, F1, kSliceMax, kMaxFanoutWide, &F2) &&
         (rc = lds_plan(n, nb, F1, nullptr, 0, nullptr, nullptr, &F2, &a)))
@@ -1435,7 +1436,7 @@ static int probe_lds(const uint64_t *slots, uint32_t nbuckets, const int32_t *pr
     if (n >= (1ull << 31)) return fail(MGV_E_INVALID, "probe_lds: n must be < 2^31");
     if (!(LC.reserved & MGV_LAUNCH_COUNTED_PART)) {
         LdsAccArgs aa = {res, nullptr, nullptr, 0};
-        return probe_lds_fill<kSliceMax, AccSum>(slots, nbuckets, pred, bound, keys, vals, n, fanout, ws, ws_bytes,
+        return probe_lds_fill<kProbeSlice, AccSum>(slots, nbuckets, pred, bound, keys, vals, n, fanout, ws, ws_bytes,
                                                  aa, LC, stream_of(st));
     }
     PartLayout L1, L2;
