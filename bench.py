@@ -832,6 +832,19 @@ def leg_c4(args, st):
     nbig = len(batch._routed(V, thr, nst, fan)["big"])
     log(f"C4 routed ({nbig} queries >= {thr} rows through LDS slices on {nst} streams): {min(rw) * 1e3:.2f} ms "
         f"wall, equal to fused: {routed_same}")
+    # XCD-sliced: the large probes' rows scattered into table-part regions,
+    # region r probed on XCD r mod 8 (mgv_batch_run_sliced), in one graph
+    smin, sreg = 1 << 16, 8
+    g = batch.capture_sliced(V, L, st.vp, min_rows=smin, nregions=sreg)
+    timed(run_graph, 2)
+    sw_, se = timed(run_graph, args.c4_reps)
+    sliced = batch.results()
+    sliced_same = bool(np.array_equal(sliced["count"], fused["count"]) and
+                       np.array_equal(sliced["sum"], fused["sum"]))
+    nsl = batch._sliced(V, smin, sreg)["queries"]
+    mgv.call("mgv_graph_destroy", g)
+    log(f"C4 sliced ({nsl} queries >= {smin} rows in {sreg} regions each): {min(sw_) * 1e3:.2f} ms wall, "
+        f"equal to fused: {sliced_same}")
     plain_out = {}
     plain_res = None
     for pv in (4096, 65536):
@@ -860,24 +873,28 @@ def leg_c4(args, st):
         ok &= (int(fused["count"][i]), int(fused["sum"][i])) == (want["count"], want["sum"])
         checked += 1
     mgv.call("mgv_graph_destroy", g)
-    fused_qps = len(shapes) / min(bw)
-    routed_qps = len(shapes) / min(rw)
-    best_w, best_e = (rw, re_) if routed_qps > fused_qps else (bw, be)
+    paths = {"fused": (bw, be), "routed": (rw, re_), "sliced": (sw_, se)}
+    qps = {k: len(shapes) / min(w) for k, (w, _) in paths.items()}
+    fused_qps, routed_qps = qps["fused"], qps["routed"]
+    best_path = max(qps, key=qps.get)
+    best_w, best_e = paths[best_path]
     algo = batch.algorithmic_bytes()
     out = {
         "queries": len(shapes), "probe_rows": batch.r_rows, "build_rows": batch.s_rows,
         "tuned": {"vector_rows": V, "block": blk, "compact": compact, "load_factor": lf, "from_file": bool(t)},
-        "batched": {"wall_ms": min(best_w) * 1e3, "gpu_ms": min(best_e), "queries_per_s": max(fused_qps, routed_qps),
-                    "path": "routed" if routed_qps > fused_qps else "fused",
+        "batched": {"wall_ms": min(best_w) * 1e3, "gpu_ms": min(best_e), "queries_per_s": qps[best_path],
+                    "path": best_path,
                     "algorithmic_gbps": algo / (min(best_e) / 1e3) / 1e9,
                     "frac": algo / (min(best_e) / 1e3) / 1e9 / HBM_PEAK_GBPS, "work_items": batch.items(V)},
         "fused_only": {"wall_ms": min(bw) * 1e3, "gpu_ms": min(be), "queries_per_s": fused_qps},
         "routed": {"wall_ms": min(rw) * 1e3, "gpu_ms": min(re_), "queries_per_s": routed_qps,
                    "lds_queries": nbig, "threshold_rows": thr, "side_streams": nst, "fanout": fan or "one level"},
+        "sliced": {"wall_ms": min(sw_) * 1e3, "gpu_ms": min(se), "queries_per_s": qps["sliced"],
+                   "sliced_queries": nsl, "min_rows": smin, "regions": sreg},
         "plain": plain_out,
-        "speedup_vs_best_plain": max(fused_qps, routed_qps) / best_plain["queries_per_s"],
+        "speedup_vs_best_plain": qps[best_path] / best_plain["queries_per_s"],
         "parity": {"fused_equals_plain": all(v["results_equal_fused"] for v in plain_out.values()),
-                   "routed_equals_fused": routed_same,
+                   "routed_equals_fused": routed_same, "sliced_equals_fused": sliced_same,
                    "oracle_checked_queries": checked, "oracle_equal": bool(ok)},
         "total_count": int(fused["count"].sum()), "total_sum": int(fused["sum"].sum()),
     }
@@ -1087,6 +1104,7 @@ def parity_failures(line):
     if c4:
         flags["c4 fused = plain"] = c4["parity"]["fused_equals_plain"]
         flags["c4 routed = fused"] = c4["parity"]["routed_equals_fused"]
+        flags["c4 sliced = fused"] = c4["parity"]["sliced_equals_fused"]
         flags["c4 oracle"] = c4["parity"]["oracle_equal"]
     for k, v in (line.get("f3") or {}).items():
         flags[f"f3 {k} kept rows"] = v["kept_rows_ok"]

@@ -401,6 +401,48 @@ int mgv_batch_plan(const mgv_query *queries, uint32_t nqueries, uint32_t vector_
 int mgv_batch_run(const mgv_query *queries, const mgv_work_item *items, uint64_t nitems, uint32_t *counter,
                   mgv_launch launch, mgv_stream_t s);
 
+/* XCD-sliced batch probe (same per-query results as mgv_batch_run).  A
+ * probe's bucket reads that miss the L2 fetch a line each from HBM, and every
+ * XCD's L2 holds the whole table when every XCD probes it; here the selected
+ * rows of each large PROBE_SUM query are first scattered by the top bits
+ * of fmix32(key) into nregions = 8, 16 or 32 regions (0: 32; the rows whose
+ * home buckets lie in one nregions-th of the table), then one persistent
+ * launch lets the waves of XCD x probe regions x, x + 8, ... of every query,
+ * one after another, so each L2 holds the part being probed and no line is
+ * cached twice.  Rows past a full region (skewed keys) are probed directly.
+ * mgv_batch_plan_sliced (host): validates the queries as mgv_batch_plan;
+ * queries of kind PROBE_SUM without MGV_PROBE_DIGEST and with n >= min_rows
+ * are sliced; fills items (capacity cap; *plan tells the count, also when
+ * items is NULL), regions (2 per query: first region row and rows per
+ * region, 0 for a query that is not sliced) and *plan.  Items: scatter tiles
+ * [0, nscatter), then probe sections x = 0..7 (regions x, x + 8, ... of every
+ * sliced query, vectors of vector_rows rows over each region's capacity;
+ * the item's query field holds the query index | region << 24) and section
+ * 8 (the other queries' vectors).  nqueries < 2^24.
+ * mgv_batch_run_sliced: device copies of queries, items and regions; the
+ * workspace (plan->workspace_bytes, 16-byte aligned) holds the claim
+ * counters, fill counts and region columns (8 bytes per row of nregions
+ * regions of about n / nregions rows each per sliced query); clears them
+ * with a fill kernel,
+ * so the call can be captured into a graph.  launch.block / compact as for
+ * mgv_batch_run.  No reference counterpart (the reference's per-vector
+ * probe, src/operators/hashjoin.cu:135-182, reads the whole table from every
+ * SM). */
+typedef struct mgv_sliced_plan {
+    uint64_t nitems;             /* all items */
+    uint64_t nscatter;           /* scatter tiles: items [0, nscatter) */
+    uint64_t section[10];        /* probe section s: items [section[s], section[s + 1]) */
+    uint64_t region_rows;        /* rows of all regions */
+    uint64_t workspace_bytes;
+    uint32_t nqueries;
+    uint32_t nregions;           /* regions per sliced query: 8, 16 or 32 */
+} mgv_sliced_plan;
+int mgv_batch_plan_sliced(const mgv_query *queries, uint32_t nqueries, uint32_t vector_rows, uint64_t min_rows,
+                          uint32_t nregions, mgv_work_item *items, uint64_t cap, uint64_t *regions,
+                          mgv_sliced_plan *plan);
+int mgv_batch_run_sliced(const mgv_query *queries, const mgv_work_item *items, const uint64_t *regions,
+                         const mgv_sliced_plan *plan, void *ws, uint64_t ws_bytes, mgv_launch launch, mgv_stream_t s);
+
 /* ------------------------------------------------------------ partitioned probe */
 
 /* Workspace bytes of a partitioned probe of n rows with the given fan-out

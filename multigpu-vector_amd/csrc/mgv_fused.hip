@@ -13,9 +13,11 @@
 // on one stream (normally captured together in one hipGraph).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "mgv_internal.hpp"
+#include "mgv_part.hpp"
 #include "mgv_pipes.hpp"
 
 namespace mgv {
@@ -178,6 +180,279 @@ static int fused_c(const mgv_query *Q, const mgv_work_item *items, uint64_t nite
     }
 }
 
+
+// ------------------------------------------------------------------ XCD-sliced probe
+//
+// A probe's random bucket reads that miss the L2 cost one 128-byte line each:
+// on MI355X about 55 G lines/s whatever the table size beyond the L2 (a
+// 64-byte read over 32 MiB to 1 GiB: 55-67 G/s), while an XCD's 4 MB L2
+// serves about 210 G reads/s.  When every XCD probes every table, each L2
+// must hold the whole table being probed; when the waves of XCD x probe only
+// the rows whose home bucket lies in the x-th eighth of the table, the eight
+// L2s hold one eighth each, and random reads over a 32 MiB table run at
+// 208 G/s instead of 67 (measured).  The sliced batch probe therefore
+// scatters the selected rows of each large PROBE_SUM query by the top bits
+// of fmix32(key) -- the bits that pick the bucket's 8th, 16th or 32nd of
+// the table -- into 8, 16 or 32 regions (k_slice_scatter); then one
+// persistent launch (k_slice_probe) lets the waves of XCD x take the regions
+// r = x mod 8 of every query first, region after region, then whatever is
+// left of the other XCDs' regions, and the other queries' vectors follow in
+// a k_fused launch.  The slice an XCD probes at a time stays in its L2
+// beside the streamed rows, and no table line is cached by two XCDs.
+
+constexpr int kSliceParts = 8;                  // XCDs: sections of the probe launch
+constexpr int kSliceRegions = 32;               // most regions per sliced query (region r in section r % 8)
+constexpr uint32_t kItemQueryBits = 24;         // work item .query: query | region << 24
+constexpr int kSliceScatterBlock = 256;
+constexpr int kSliceTile = 4096;   // rows per scatter item: 4 quads per thread
+
+struct SlicedRun {
+    const uint64_t *regions;     // per query {first region row, rows per region}; 0 rows: not sliced
+    unsigned long long *fill;    // per query kSliceRegions words, one per region: rows reserved (may pass the capacity)
+    int32_t *rkey, *rval;        // region columns
+    uint32_t *ctr;               // claim counters: [0] scatter items, [1 + s] probe section s
+    uint64_t nscatter;
+    uint64_t sec[kSliceParts + 2];   // probe items of section s: [sec[s], sec[s + 1])
+    uint32_t shift;                  // region of a row = fmix32(key) >> shift (8, 16 or 32 regions)
+};
+
+// Probes one row against a global table, following its chain (the overflow
+// rows of a full region): count and SUM(val + payload) of every hitting slot.
+__device__ __forceinline__ void probe_row(const uint64_t *slots, uint32_t nb, uint32_t key, int32_t val,
+                                          uint64_t &cnt, uint64_t &sum) {
+    uint32_t b = bucket_of((int32_t)key, nb);
+    for (uint32_t pass = 0; pass < nb; ++pass) {
+        const v4u *g = reinterpret_cast<const v4u *>(slots + (uint64_t)b * 8);
+        v4u q[4];
+#pragma unroll
+        for (int w = 0; w < 4; ++w) q[w] = load_global(g + w);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const uint32_t k = (t & 1) ? q[t >> 1].z : q[t >> 1].x;
+            const uint32_t p = (t & 1) ? q[t >> 1].w : q[t >> 1].y;
+            if (k == key && !(k == ~0u && p == ~0u)) {
+                cnt += 1;
+                sum += (uint64_t)((int64_t)(int32_t)p + (int64_t)val);
+            }
+        }
+        if (q[3].z == ~0u && q[3].w == ~0u) break;   // a bucket with an empty slot ends the chain
+        b = (b + 1 == nb) ? 0 : b + 1;
+    }
+}
+
+// Scatter items: up to kSliceTile rows of one sliced query.  The block
+// selects the rows (pred <= bound), sorts them in LDS by region (top 3 bits
+// of fmix32(key)), reserves each region's run with one atomic, and writes
+// {key, val} runs; rows past a full region's capacity (skewed keys) are
+// probed on the spot and added to the query's result.
+template <int BLOCK, int T>
+__global__ __launch_bounds__(BLOCK) void k_slice_scatter(const mgv_query *__restrict__ Q,
+                                                         const mgv_work_item *__restrict__ items, SlicedRun R) {
+    constexpr int NQ = T / (4 * BLOCK);
+    static_assert(NQ >= 1 && T % (4 * BLOCK) == 0, "slice scatter shape");
+    __shared__ int32_t sk[T];
+    __shared__ int32_t sv[T];
+    __shared__ uint32_t cnt[kSliceRegions], start[kSliceRegions + 1], lim[kSliceRegions];
+    __shared__ unsigned long long dst[kSliceRegions];
+    __shared__ uint32_t s_item;
+    const uint32_t t = threadIdx.x;
+    for (;;) {
+        if (t == 0) s_item = atomicAdd(&R.ctr[0], 1u);
+        if (t < kSliceRegions) cnt[t] = 0;   // all 32: a tile of 8 or 16 regions leaves the rest at 0
+        __syncthreads();
+        const uint32_t it = s_item;
+        if (it >= R.nscatter) break;
+        const mgv_work_item wi = items[it];
+        const mgv_query q = Q[wi.query];
+        const uint64_t hi = (wi.lo + wi.rows < q.n) ? wi.lo + wi.rows : q.n;
+        v4i k[NQ], v[NQ];
+        uint32_t m[NQ];
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+            const uint64_t i0 = wi.lo + 4ull * t + (uint64_t)j * 4 * BLOCK;
+            uint32_t mm = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) mm |= (uint32_t)(i0 + e < hi) << e;
+            k[j] = load_quad(q.key, i0, hi);
+            v[j] = q.val ? load_quad(q.val, i0, hi) : v4i{0, 0, 0, 0};
+            if (q.pred) {
+                const v4i c = load_quad(q.pred, i0, hi);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) mm &= ~((uint32_t)(q_get(c, e) > q.bound) << e);
+            }
+            m[j] = mm;
+        }
+        uint32_t rank[NQ][4];
+#pragma unroll
+        for (int j = 0; j < NQ; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                rank[j][e] = ((m[j] >> e) & 1u)
+                                 ? atomicAdd(&cnt[fmix32((uint32_t)q_get(k[j], e)) >> R.shift], 1u)
+                                 : 0u;
+        __syncthreads();
+        if (t == 0) {
+            uint32_t a = 0;
+            for (int p = 0; p < kSliceRegions; ++p) {
+                start[p] = a;
+                a += cnt[p];
+            }
+            start[kSliceRegions] = a;
+        }
+        if (t < kSliceRegions) {
+            const uint32_t c = cnt[t];
+            const uint64_t base = R.regions[2 * wi.query], cap = R.regions[2 * wi.query + 1];
+            uint32_t fit = 0;
+            if (c) {
+                const unsigned long long off =
+                    atomic_add_global(&R.fill[(uint64_t)wi.query * kSliceRegions + t], (unsigned long long)c);
+                const uint64_t room = off < cap ? cap - off : 0;
+                fit = room < c ? (uint32_t)room : c;
+                dst[t] = base + (uint64_t)t * cap + off;
+            }
+            lim[t] = fit;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < NQ; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if ((m[j] >> e) & 1u) {
+                    const uint32_t pos = start[fmix32((uint32_t)q_get(k[j], e)) >> R.shift] + rank[j][e];
+                    sk[pos] = q_get(k[j], e);
+                    sv[pos] = q_get(v[j], e);
+                }
+        __syncthreads();
+        const uint32_t total = start[kSliceRegions];
+        uint64_t ocnt = 0, osum = 0;
+        bool over = false;
+        for (uint32_t i = t; i < total; i += BLOCK) {
+            const int32_t key = sk[i], val = sv[i];
+            const uint32_t p = fmix32((uint32_t)key) >> R.shift;
+            const uint32_t j = i - start[p];
+            if (j < lim[p]) {
+                R.rkey[dst[p] + j] = key;
+                R.rval[dst[p] + j] = val;
+            } else {
+                probe_row(q.slots, q.nbuckets, (uint32_t)key, val, ocnt, osum);
+                over = true;
+            }
+        }
+        if (ballot(over)) {
+            const uint64_t a = wave_sum_u64(ocnt), b = wave_sum_u64(osum);
+            if (lane_id() == 0 && a) {
+                atomic_add_global((unsigned long long *)&q.res->count, (unsigned long long)a);
+                atomic_add_global((unsigned long long *)&q.res->sum, (unsigned long long)b);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The region probe of the sliced batch: a wave on XCD x claims items of
+// section x (the regions r = x mod 8 of every sliced query, query by query),
+// then of the other sections.  An item's .query holds the query index and,
+// above bit 24, its region.  (Section 8, the other queries' vectors, runs as
+// a k_fused launch of its own: its select-and-compact chain needs far more
+// registers, which would halve this kernel's waves.)
+template <int BLOCK, bool COOP>
+__global__ __launch_bounds__(BLOCK) void k_slice_probe(const mgv_query *__restrict__ Q,
+                                                       const mgv_work_item *__restrict__ items, SlicedRun R) {
+    const uint32_t x = xcc_id();
+    uint32_t step = 0;   // position in the section order x, x + 1, ..., x + 7 (wave-uniform)
+    for (;;) {
+        uint32_t s = 0, it = 0;
+        bool got = false;
+        while (step < kSliceParts) {
+            s = (x + step) & (kSliceParts - 1);
+            const uint64_t len = R.sec[s + 1] - R.sec[s];
+            uint32_t c = 0;
+            if (lane_id() == 0) c = atomicAdd(&R.ctr[1 + s], 1u);
+            c = __builtin_amdgcn_readlane(c, 0);
+            if (c < len) {
+                it = c;
+                got = true;
+                break;
+            }
+            ++step;
+        }
+        if (!got) break;
+        const mgv_work_item wi = items[R.sec[s] + it];
+        const uint32_t qi = wi.query & ((1u << kItemQueryBits) - 1u), region = wi.query >> kItemQueryBits;
+        const mgv_query q = Q[qi];
+        {
+            const uint64_t base = R.regions[2 * qi], cap = R.regions[2 * qi + 1];
+            const unsigned long long f = R.fill[(uint64_t)qi * kSliceRegions + region];
+            const uint64_t n = f < cap ? f : cap;
+            const uint64_t hi = (wi.lo + wi.rows < n) ? wi.lo + wi.rows : n;
+            if (wi.lo < hi) {
+                typename std::conditional<COOP, OpProbeCoop<OpAggWave<2, false>>, OpProbe<OpAggWave<2, false>>>::type
+                    op;
+                op.slots = q.slots;
+                op.nbuckets = q.nbuckets;
+                op.parent.res = q.res;
+                const uint64_t r0 = base + (uint64_t)region * cap;
+                Cols cols = {{R.rkey + r0, R.rval + r0, nullptr, nullptr}, -1};
+                run_rows<2>(op, cols, wi.lo, hi, 0);
+            }
+        }
+    }
+}
+
+template <int B, bool COOP>
+static int slice_probe_k(const mgv_query *Q, const mgv_work_item *items, const SlicedRun &R, const mgv_launch &L,
+                         hipStream_t s) {
+    auto kern = k_slice_probe<B, COOP>;
+    static thread_local int cached = 0;
+    if (!cached) cached = persistent_blocks(kern, B);
+    const uint64_t nitems = R.sec[kSliceParts] - R.sec[0];
+    uint64_t grid = L.grid ? L.grid : (uint64_t)cached;
+    const uint64_t need = (nitems + (B / kWave) - 1) / (B / kWave);
+    if (need < grid) grid = need ? need : 1;
+    hipLaunchKernelGGL(kern, dim3((uint32_t)grid), dim3(B), 0, s, Q, items, R);
+    return check_launch("k_slice_probe");
+}
+
+// MGV_LAUNCH_COOP_PROBE: the regions' rows through the 4-lanes-per-row
+// bucket scan (one 16-byte load per lane and row: a quarter of the
+// texture-addresser work of one row per lane, which bounds the probe once
+// the buckets come from the L2)
+template <int B>
+static int slice_probe_b(const mgv_query *Q, const mgv_work_item *items, const SlicedRun &R, const mgv_launch &L,
+                         hipStream_t s) {
+    if (L.reserved & MGV_LAUNCH_COOP_PROBE) return slice_probe_k<B, true>(Q, items, R, L, s);
+    return slice_probe_k<B, false>(Q, items, R, L, s);
+}
+
+static int slice_probe(const mgv_query *Q, const mgv_work_item *items, const SlicedRun &R, const mgv_launch &L,
+                       hipStream_t s) {
+    switch (L.block) {
+        case 64: return slice_probe_b<64>(Q, items, R, L, s);
+        case 128: return slice_probe_b<128>(Q, items, R, L, s);
+        case 0:
+        case 256: return slice_probe_b<256>(Q, items, R, L, s);
+        case 512: return slice_probe_b<512>(Q, items, R, L, s);
+        case 1024: return slice_probe_b<1024>(Q, items, R, L, s);
+        default: return fail(MGV_E_INVALID, "block %u not in {64,128,256,512,1024}", L.block);
+    }
+}
+
+// Workspace of a sliced batch: claim counters, fill counts, region columns.
+static uint64_t sliced_ws_bytes(uint32_t nq, uint64_t region_rows) {
+    const uint64_t fills = ((uint64_t)nq * kSliceRegions * 8 + 255) / 256 * 256;
+    const uint64_t col = (region_rows * 4 + 255) / 256 * 256;
+    return 256 + fills + 2 * col;
+}
+
+// Rows per region of a sliced query of n rows in nreg regions: n / nreg plus
+// eight standard deviations of a uniform hash partition's size, plus 256, in
+// multiples of 64.
+static uint64_t slice_cap(uint64_t n, uint32_t nreg) {
+    const double lam = (double)n / nreg;
+    const uint64_t c = (uint64_t)(lam + 8.0 * __builtin_sqrt(lam)) + 256;
+    return (c + 63) / 64 * 64;
+}
+
 }  // namespace mgv
 
 using namespace mgv;
@@ -232,6 +507,123 @@ int mgv_batch_run(const mgv_query *queries, const mgv_work_item *items, uint64_t
     if (rc) return rc;
     if (L.compact == 0) return fused_c<false>(queries, items, nitems, counter, L, s);
     return fused_c<true>(queries, items, nitems, counter, L, s);
+}
+
+int mgv_batch_plan_sliced(const mgv_query *queries, uint32_t nqueries, uint32_t vector_rows, uint64_t min_rows,
+                          uint32_t nregions, mgv_work_item *items, uint64_t cap, uint64_t *regions,
+                          mgv_sliced_plan *plan) {
+    if (!queries || !plan) return fail(MGV_E_INVALID, "mgv_batch_plan_sliced: null argument");
+    if (nregions == 0) nregions = kSliceRegions;
+    if (nregions != 8 && nregions != 16 && nregions != 32)
+        return fail(MGV_E_INVALID, "mgv_batch_plan_sliced: %u regions per query (8, 16 or 32)", nregions);
+    if (nqueries >= (1u << kItemQueryBits)) return fail(MGV_E_INVALID, "mgv_batch_plan_sliced: too many queries");
+    uint64_t n = 0;
+    int rc = mgv_batch_plan(queries, nqueries, vector_rows, nullptr, 0, &n);   // validates the queries
+    if (rc) return rc;
+    auto sliced = [&](const mgv_query &q) {
+        return q.kind == MGV_Q_PROBE_SUM && !(q.flags & MGV_PROBE_DIGEST) && q.n > 0 && q.n >= min_rows;
+    };
+    uint64_t k = 0, rows = 0;
+    auto put = [&](uint32_t qi, uint64_t lo, uint64_t r) {
+        if (items && k < cap) {
+            items[k].query = qi;
+            items[k].rows = (uint32_t)r;
+            items[k].lo = lo;
+        }
+        ++k;
+    };
+    // regions
+    for (uint32_t qi = 0; qi < nqueries; ++qi) {
+        const uint64_t c = sliced(queries[qi]) ? slice_cap(queries[qi].n, nregions) : 0;
+        if (regions) {
+            regions[2 * qi] = rows;
+            regions[2 * qi + 1] = c;
+        }
+        rows += (uint64_t)nregions * c;
+    }
+    // scatter tiles of the sliced queries
+    for (uint32_t qi = 0; qi < nqueries; ++qi)
+        if (sliced(queries[qi]))
+            for (uint64_t lo = 0; lo < queries[qi].n; lo += kSliceTile)
+                put(qi, lo, queries[qi].n - lo < (uint64_t)kSliceTile ? queries[qi].n - lo : kSliceTile);
+    plan->nscatter = k;
+    // probe sections x = 0..7: regions x, x + 8, ... of every sliced query,
+    // query by query (items over each region's capacity)
+    for (int x = 0; x < kSliceParts; ++x) {
+        plan->section[x] = k;
+        for (uint32_t qi = 0; qi < nqueries; ++qi)
+            if (sliced(queries[qi])) {
+                const uint64_t c = slice_cap(queries[qi].n, nregions);
+                for (uint32_t r = (uint32_t)x; r < nregions; r += kSliceParts)
+                    for (uint64_t lo = 0; lo < c; lo += vector_rows)
+                        put(qi | (r << kItemQueryBits), lo, c - lo < vector_rows ? c - lo : vector_rows);
+            }
+    }
+    // section 8: every other query, in vectors
+    plan->section[kSliceParts] = k;
+    for (uint32_t qi = 0; qi < nqueries; ++qi)
+        if (!sliced(queries[qi]))
+            for (uint64_t lo = 0; lo < queries[qi].n; lo += vector_rows)
+                put(qi, lo, queries[qi].n - lo < vector_rows ? queries[qi].n - lo : vector_rows);
+    plan->section[kSliceParts + 1] = k;
+    plan->nitems = k;
+    plan->nqueries = nqueries;
+    plan->nregions = nregions;
+    plan->region_rows = rows;
+    plan->workspace_bytes = sliced_ws_bytes(nqueries, rows);
+    if (items && k > cap) return fail(MGV_E_CAPACITY, "mgv_batch_plan_sliced: %llu items > capacity %llu",
+                                      (unsigned long long)k, (unsigned long long)cap);
+    return MGV_OK;
+}
+
+int mgv_batch_run_sliced(const mgv_query *queries, const mgv_work_item *items, const uint64_t *regions,
+                         const mgv_sliced_plan *plan, void *ws, uint64_t ws_bytes, mgv_launch L, mgv_stream_t st) {
+    if (!current_state()) return fail(MGV_E_NOT_INIT, "mgv_init was not called");
+    if (!queries || !plan || !regions || !ws || (plan->nitems && !items))
+        return fail(MGV_E_INVALID, "mgv_batch_run_sliced: null argument");
+    if (plan->nitems >= 0xffffffffull) return fail(MGV_E_INVALID, "mgv_batch_run_sliced: too many items");
+    if (plan->nscatter > plan->section[0] || plan->section[kSliceParts + 1] != plan->nitems ||
+        (plan->nregions != 8 && plan->nregions != 16 && plan->nregions != 32))
+        return fail(MGV_E_INVALID, "mgv_batch_run_sliced: inconsistent plan");
+    for (int p = 0; p <= kSliceParts; ++p)
+        if (plan->section[p] > plan->section[p + 1]) return fail(MGV_E_INVALID, "mgv_batch_run_sliced: inconsistent plan");
+    const uint64_t need = sliced_ws_bytes(plan->nqueries, plan->region_rows);
+    if (ws_bytes < need) return fail(MGV_E_CAPACITY, "mgv_batch_run_sliced: workspace %llu < %llu bytes",
+                                     (unsigned long long)ws_bytes, (unsigned long long)need);
+    if (!aligned16(ws)) return fail(MGV_E_ALIGN, "mgv_batch_run_sliced: workspace not 16-byte aligned");
+    if (plan->nitems == 0) return MGV_OK;
+    hipStream_t s = stream_of(st);
+    char *w = (char *)ws;
+    const uint64_t fills = ((uint64_t)plan->nqueries * kSliceRegions * 8 + 255) / 256 * 256;
+    const uint64_t col = (plan->region_rows * 4 + 255) / 256 * 256;
+    SlicedRun R = {};
+    R.regions = regions;
+    R.ctr = (uint32_t *)w;
+    R.fill = (unsigned long long *)(w + 256);
+    R.rkey = (int32_t *)(w + 256 + fills);
+    R.rval = (int32_t *)(w + 256 + fills + col);
+    R.nscatter = plan->nscatter;
+    R.shift = 32u - (uint32_t)__builtin_ctz(plan->nregions);
+    for (int p = 0; p <= kSliceParts + 1; ++p) R.sec[p] = plan->section[p];
+    // counters and fill counts: a fill kernel (the call can be captured into a graph)
+    int rc = mgv_fill_u64((uint64_t *)ws, (256 + fills) / 8, 0, (mgv_stream_t)s);
+    if (rc) return rc;
+    if (plan->nscatter) {
+        auto kern = k_slice_scatter<kSliceScatterBlock, kSliceTile>;
+        static thread_local int cached = 0;
+        if (!cached) cached = persistent_blocks(kern, kSliceScatterBlock);
+        const uint64_t grid = plan->nscatter < (uint64_t)cached ? plan->nscatter : (uint64_t)cached;
+        hipLaunchKernelGGL(kern, dim3((uint32_t)grid), dim3(kSliceScatterBlock), 0, s, queries, items, R);
+        if ((rc = check_launch("k_slice_scatter"))) return rc;
+    }
+    if (plan->section[kSliceParts] > plan->section[0] && (rc = slice_probe(queries, items, R, L, s))) return rc;
+    // section 8: the other queries' vectors, as mgv_batch_run
+    const uint64_t nd = plan->section[kSliceParts + 1] - plan->section[kSliceParts];
+    if (!nd) return MGV_OK;
+    const mgv_work_item *di = items + plan->section[kSliceParts];
+    uint32_t *dc = R.ctr + 1 + kSliceParts;
+    if (L.compact == 0) return fused_c<false>(queries, di, nd, dc, L, s);
+    return fused_c<true>(queries, di, nd, dc, L, s);
 }
 
 }  // extern "C"

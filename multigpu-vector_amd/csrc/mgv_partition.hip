@@ -22,12 +22,6 @@
 
 namespace mgv {
 
-// XCC (XCD) id of the executing wave: HW_REG_XCC_ID (hwreg 20), bits [3:0].
-// Used only to choose which partitions a block drains first.
-__device__ __forceinline__ uint32_t xcc_id() {
-    return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | 20) & 7u;
-}
-
 __device__ __forceinline__ uint32_t part_of(int32_t key, uint32_t F) {
     return (uint32_t)(((uint64_t)fmix32((uint32_t)key) * F) >> 32);
 }

@@ -24,6 +24,13 @@ __device__ __forceinline__ uint32_t lane_id() {
 
 __device__ __forceinline__ uint32_t wave_id() { return threadIdx.x / kWave; }
 
+// XCC (XCD) id of the executing wave: HW_REG_XCC_ID (hwreg 20), bits [2:0]
+// (8 XCDs).  Used only to choose which work a wave takes first: results never
+// depend on it.
+__device__ __forceinline__ uint32_t xcc_id() {
+    return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | 20) & 7u;
+}
+
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // Number of set bits of `mask` in lanes strictly below this lane.

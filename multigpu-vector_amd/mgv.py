@@ -99,6 +99,15 @@ class Query(C.Structure):
 
 class WorkItem(C.Structure):
     _fields_ = [("query", C.c_uint32), ("rows", C.c_uint32), ("lo", C.c_uint64)]
+
+
+class SlicedPlan(C.Structure):
+    """mgv_sliced_plan: item sections and workspace of an XCD-sliced batch."""
+    _fields_ = [
+        ("nitems", C.c_uint64), ("nscatter", C.c_uint64), ("section", C.c_uint64 * 10),
+        ("region_rows", C.c_uint64), ("workspace_bytes", C.c_uint64), ("nqueries", C.c_uint32),
+        ("nregions", C.c_uint32),
+    ]
 U32 = C.c_uint32
 I32 = C.c_int32
 I64 = C.c_int64
@@ -157,6 +166,8 @@ SIGNATURES = {
     "mgv_debug_xcc_histogram": [VP, U32, VP],
     "mgv_batch_plan": [VP, U32, U32, VP, U64, C.POINTER(U64)],
     "mgv_batch_run": [VP, VP, U64, VP, Launch, VP],
+    "mgv_batch_plan_sliced": [VP, U32, U32, U64, U32, VP, U64, VP, VP],
+    "mgv_batch_run_sliced": [VP, VP, VP, VP, VP, U64, Launch, VP],
     "mgv_pool_create": [U32, U32, C.POINTER(VP)],
     "mgv_pool_destroy": [VP],
     "mgv_pool_acquire": [VP, C.POINTER(I32), VP],

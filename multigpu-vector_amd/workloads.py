@@ -120,6 +120,10 @@ class JoinBatch:
 
     def _drop_plans(self):
         for plan in self._plans.values():
+            if isinstance(plan, dict) and "sliced" in plan:
+                for b in (plan["dq"], plan["di"], plan["dr"], plan["ws"]):
+                    b.close()
+                continue
             if isinstance(plan, dict):   # routed plan
                 dq, di, _ = plan["small"]
                 for b in [dq, di] + plan["ws"] + plan["streams"] + plan["events"]:
@@ -259,6 +263,53 @@ class JoinBatch:
         self._routed(vector_rows, threshold, nstreams, fanout)
         mgv.call("mgv_graph_begin_capture", stream)
         self.issue_routed(vector_rows, launch, stream, threshold, nstreams, fanout)
+        g = C.c_void_p()
+        mgv.call("mgv_graph_end_capture", stream, C.byref(g))
+        return g
+
+    # -------------------------------------------------------------- XCD-sliced path
+    def _sliced(self, vector_rows: int, min_rows: int, nregions: int = 0):
+        """Plan of the sliced probe launch (mgv_batch_plan_sliced over the
+        probe queries): device queries, items, regions and workspace; cached."""
+        key = ("sliced", vector_rows, min_rows, nregions)
+        if key in self._plans:
+            return self._plans[key]
+        plan = mgv.SlicedPlan()
+        regions = (C.c_uint64 * (2 * max(1, self.nq)))()
+        mgv.call("mgv_batch_plan_sliced", self.probe_q, self.nq, vector_rows, min_rows, nregions, None, 0, regions,
+                 C.byref(plan))
+        items = (mgv.WorkItem * max(1, plan.nitems))()
+        mgv.call("mgv_batch_plan_sliced", self.probe_q, self.nq, vector_rows, min_rows, nregions, items, plan.nitems,
+                 regions, C.byref(plan))
+        dq = mgv.DeviceBuffer(C.sizeof(self.probe_q)).upload(np.frombuffer(bytes(self.probe_q), dtype=np.uint8))
+        di = mgv.DeviceBuffer(max(16, C.sizeof(items))).upload(np.frombuffer(bytes(items), dtype=np.uint8))
+        dr = mgv.DeviceBuffer(C.sizeof(regions)).upload(np.frombuffer(bytes(regions), dtype=np.uint8))
+        ws = mgv.DeviceBuffer(max(256, plan.workspace_bytes))
+        out = {"sliced": True, "plan": plan, "dq": dq, "di": di, "dr": dr, "ws": ws,
+               "queries": sum(1 for i in range(self.nq) if regions[2 * i + 1])}
+        self._plans[key] = out
+        return out
+
+    def issue_sliced(self, vector_rows: int, launch: mgv.Launch, stream=None, min_rows: int = 1 << 20,
+                     nregions: int = 0):
+        """clear + the fused build launch + the XCD-sliced probe
+        (mgv_batch_run_sliced: probes of at least min_rows rows scattered
+        into nregions regions by table part, regions r = x mod 8 probed on
+        XCD x)."""
+        s = stream if stream is not None else self.st
+        (bq, bi, bn), _ = self._plan(vector_rows)
+        sp = self._sliced(vector_rows, min_rows, nregions)
+        self.clear(s)
+        mgv.call("mgv_batch_run", bq.vp, bi.vp, bn, self.counter.vp, launch, s)
+        mgv.call("mgv_batch_run_sliced", sp["dq"].vp, sp["di"].vp, sp["dr"].vp, C.byref(sp["plan"]), sp["ws"].vp,
+                 sp["ws"].nbytes, launch, s)
+
+    def capture_sliced(self, vector_rows: int, launch: mgv.Launch, stream, min_rows: int = 1 << 20,
+                       nregions: int = 0) -> C.c_void_p:
+        self._plan(vector_rows)
+        self._sliced(vector_rows, min_rows, nregions)
+        mgv.call("mgv_graph_begin_capture", stream)
+        self.issue_sliced(vector_rows, launch, stream, min_rows, nregions)
         g = C.c_void_p()
         mgv.call("mgv_graph_end_capture", stream, C.byref(g))
         return g
