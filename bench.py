@@ -851,7 +851,7 @@ def leg_c4(args, st):
         f"equal to fused: {sliced_same}")
     plain_out = {}
     plain_res = None
-    for pv in (4096, 65536):
+    for pv in ((4096, 65536) if args.c4_plain else ()):
         Lp = mgv.launch(0, 256, 1)
         timed(lambda: batch.issue_plain(pv, Lp, st.vp), 1)
         pw, pe = timed(lambda: batch.issue_plain(pv, Lp, st.vp), max(1, args.c4_reps // 2))
@@ -861,7 +861,7 @@ def leg_c4(args, st):
                               "launches": 2 * (batch.items(pv)), "results_equal_fused": same}
         log(f"C4 plain V={pv}: {min(pw) * 1e3:.2f} ms wall, equal to fused: {same}")
         plain_res = r
-    best_plain = max(plain_out.values(), key=lambda x: x["queries_per_s"])
+    best_plain = max(plain_out.values(), key=lambda x: x["queries_per_s"]) if plain_out else None
     # oracle check of the smaller queries (host copies of their columns)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py
@@ -896,14 +896,16 @@ def leg_c4(args, st):
         "sliced": {"wall_ms": min(sw_) * 1e3, "gpu_ms": min(se), "queries_per_s": qps["sliced"],
                    "sliced_queries": nsl, "min_rows": smin, "regions": sreg},
         "plain": plain_out,
-        "speedup_vs_best_plain": qps[best_path] / best_plain["queries_per_s"],
-        "parity": {"fused_equals_plain": all(v["results_equal_fused"] for v in plain_out.values()),
+        "speedup_vs_best_plain": qps[best_path] / best_plain["queries_per_s"] if best_plain else None,
+        "parity": {"fused_equals_plain": (all(v["results_equal_fused"] for v in plain_out.values())
+                                          if plain_out else None),
                    "routed_equals_fused": routed_same, "sliced_equals_fused": sliced_same,
                    "oracle_checked_queries": checked, "oracle_equal": bool(ok)},
         "total_count": int(fused["count"].sum()), "total_sum": int(fused["sum"].sum()),
     }
-    log(f"C4 batched {min(bw) * 1e3:.2f} ms wall ({fused_qps:.0f} q/s), x{out['speedup_vs_best_plain']:.2f} vs best "
-        f"plain; oracle {checked} queries ok={ok}")
+    sp = out["speedup_vs_best_plain"]
+    log(f"C4 batched {min(bw) * 1e3:.2f} ms wall ({fused_qps:.0f} q/s), "
+        f"{'x%.2f vs best plain' % sp if sp else 'plain path not run'}; oracle {checked} queries ok={ok}")
     del plain_res
     batch.close()
     return out
@@ -999,6 +1001,7 @@ def main():
     ap.add_argument("--legs", default="c1,c3,c4,c5,copy,f3,f1")
     ap.add_argument("--c5-requests", type=int, default=300)
     ap.add_argument("--c4-reps", type=int, default=5)
+    ap.add_argument("--c4-plain", type=int, default=1, help="time the plain per-vector path in the C4 leg (0 = skip)")
     ap.add_argument("--cpu", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=1 << 25, help="R rows of the one-thread C2 figure")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -1106,7 +1109,8 @@ def parity_failures(line):
         flags[f"c3 {sel} oracle"] = v["oracle_equal_first_2^22"]
     c4 = line.get("c4")
     if c4:
-        flags["c4 fused = plain"] = c4["parity"]["fused_equals_plain"]
+        if c4["parity"]["fused_equals_plain"] is not None:
+            flags["c4 fused = plain"] = c4["parity"]["fused_equals_plain"]
         flags["c4 routed = fused"] = c4["parity"]["routed_equals_fused"]
         flags["c4 sliced = fused"] = c4["parity"]["sliced_equals_fused"]
         flags["c4 oracle"] = c4["parity"]["oracle_equal"]
