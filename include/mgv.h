@@ -413,8 +413,8 @@ int mgv_batch_run(const mgv_query *queries, const mgv_work_item *items, uint64_t
  * mgv_batch_plan_sliced (host): validates the queries as mgv_batch_plan;
  * queries of kind PROBE_SUM without MGV_PROBE_DIGEST and with n >= min_rows
  * are sliced; fills items (capacity cap; *plan tells the count, also when
- * items is NULL), regions (2 per query: first region row and rows per
- * region, 0 for a query that is not sliced) and *plan.  Items: scatter tiles
+ * items is NULL), regions (4 words per query: first region row, rows per
+ * region, regions (0 for a query that is not sliced), 0) and *plan.  Items: scatter tiles
  * [0, nscatter), then probe sections x = 0..7 (regions x, x + 8, ... of every
  * sliced query, vectors of vector_rows rows over each region's capacity;
  * the item's query field holds the query index | region << 24) and section
@@ -442,6 +442,38 @@ int mgv_batch_plan_sliced(const mgv_query *queries, uint32_t nqueries, uint32_t 
                           mgv_sliced_plan *plan);
 int mgv_batch_run_sliced(const mgv_query *queries, const mgv_work_item *items, const uint64_t *regions,
                          const mgv_sliced_plan *plan, void *ws, uint64_t ws_bytes, mgv_launch launch, mgv_stream_t s);
+
+/* LDS batch probe (same per-query results as mgv_batch_run): the PROBE_SUM
+ * queries (no MGV_PROBE_DIGEST, n >= min_rows) whose table splits into at
+ * most 256 power-of-two slices of at most 2048 buckets have their selected
+ * rows scattered (hashed keys) into one region per slice; one persistent
+ * launch then copies each (query, slice) into LDS and probes the slice's
+ * region from it (the probe of mgv_hashjoin_probe_lds), so the probes fetch
+ * no random lines; rows past a full region are probed directly; the other
+ * queries run as mgv_batch_run.
+ * mgv_batch_plan_lds (host): validates as mgv_batch_plan; fills items
+ * (capacity cap; *plan tells the count, also when items is NULL): scatter
+ * tiles [0, nscatter), then one item per (query, slice) (.lo = slice), then
+ * the other queries' vectors; regions (4 words per query: first region row,
+ * rows per region, slices F (0: not LDS-probed), buckets per slice).
+ * mgv_batch_run_lds: device copies of queries, items and regions; workspace
+ * of plan->workspace_bytes (16-byte aligned), cleared by a fill kernel, so
+ * the call can be captured into a graph.  launch.block / compact apply to
+ * the other queries' launch.  No reference counterpart (the reference probes
+ * each vector against the whole table, src/operators/hashjoin.cu:135-182). */
+typedef struct mgv_lds_batch_plan {
+    uint64_t nitems;             /* all items */
+    uint64_t nscatter;           /* scatter tiles: items [0, nscatter) */
+    uint64_t nparts;             /* (query, slice) items: [nscatter, nscatter + nparts) */
+    uint64_t region_rows;        /* rows of all regions */
+    uint64_t workspace_bytes;
+    uint32_t nqueries;
+    uint32_t reserved;
+} mgv_lds_batch_plan;
+int mgv_batch_plan_lds(const mgv_query *queries, uint32_t nqueries, uint32_t vector_rows, uint64_t min_rows,
+                       mgv_work_item *items, uint64_t cap, uint64_t *regions, mgv_lds_batch_plan *plan);
+int mgv_batch_run_lds(const mgv_query *queries, const mgv_work_item *items, const uint64_t *regions,
+                      const mgv_lds_batch_plan *plan, void *ws, uint64_t ws_bytes, mgv_launch launch, mgv_stream_t s);
 
 /* ------------------------------------------------------------ partitioned probe */
 

@@ -16,6 +16,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "mgv_batch.hpp"
 #include "mgv_internal.hpp"
 #include "mgv_part.hpp"
 #include "mgv_pipes.hpp"
@@ -201,69 +202,45 @@ static int fused_c(const mgv_query *Q, const mgv_work_item *items, uint64_t nite
 // beside the streamed rows, and no table line is cached by two XCDs.
 
 constexpr int kSliceParts = 8;                  // XCDs: sections of the probe launch
-constexpr int kSliceRegions = 32;               // most regions per sliced query (region r in section r % 8)
 constexpr uint32_t kItemQueryBits = 24;         // work item .query: query | region << 24
-constexpr int kSliceScatterBlock = 256;
-constexpr int kSliceTile = 4096;   // rows per scatter item: 4 quads per thread
 
 struct SlicedRun {
-    const uint64_t *regions;     // per query {first region row, rows per region}; 0 rows: not sliced
-    unsigned long long *fill;    // per query kSliceRegions words, one per region: rows reserved (may pass the capacity)
+    const uint64_t *regions;     // per query kRegionWords words (mgv_batch.hpp); F = 0: not sliced
+    unsigned long long *fill;    // per query kBatchMaxParts words, one per region: rows reserved (may pass the capacity)
     int32_t *rkey, *rval;        // region columns
     uint32_t *ctr;               // claim counters: [0] scatter items, [1 + s] probe section s
-    uint64_t nscatter;
     uint64_t sec[kSliceParts + 2];   // probe items of section s: [sec[s], sec[s + 1])
-    uint32_t shift;                  // region of a row = fmix32(key) >> shift (8, 16 or 32 regions)
 };
 
-// Probes one row against a global table, following its chain (the overflow
-// rows of a full region): count and SUM(val + payload) of every hitting slot.
-__device__ __forceinline__ void probe_row(const uint64_t *slots, uint32_t nb, uint32_t key, int32_t val,
-                                          uint64_t &cnt, uint64_t &sum) {
-    uint32_t b = bucket_of((int32_t)key, nb);
-    for (uint32_t pass = 0; pass < nb; ++pass) {
-        const v4u *g = reinterpret_cast<const v4u *>(slots + (uint64_t)b * 8);
-        v4u q[4];
-#pragma unroll
-        for (int w = 0; w < 4; ++w) q[w] = load_global(g + w);
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const uint32_t k = (t & 1) ? q[t >> 1].z : q[t >> 1].x;
-            const uint32_t p = (t & 1) ? q[t >> 1].w : q[t >> 1].y;
-            if (k == key && !(k == ~0u && p == ~0u)) {
-                cnt += 1;
-                sum += (uint64_t)((int64_t)(int32_t)p + (int64_t)val);
-            }
-        }
-        if (q[3].z == ~0u && q[3].w == ~0u) break;   // a bucket with an empty slot ends the chain
-        b = (b + 1 == nb) ? 0 : b + 1;
-    }
-}
-
-// Scatter items: up to kSliceTile rows of one sliced query.  The block
-// selects the rows (pred <= bound), sorts them in LDS by region (top 3 bits
-// of fmix32(key)), reserves each region's run with one atomic, and writes
-// {key, val} runs; rows past a full region's capacity (skewed keys) are
-// probed on the spot and added to the query's result.
+// Scatter items: up to kBatchTile rows of one partitioned query.  The block
+// selects the rows (pred <= bound), sorts them in LDS by partition (top bits
+// of fmix32(key)), reserves each partition's run with one atomic, and
+// writes {key or fmix32(key), val} runs; rows past a full region's capacity
+// (skewed keys) are probed on the spot and added to the query's result.
 template <int BLOCK, int T>
-__global__ __launch_bounds__(BLOCK) void k_slice_scatter(const mgv_query *__restrict__ Q,
-                                                         const mgv_work_item *__restrict__ items, SlicedRun R) {
+__global__ __launch_bounds__(BLOCK) void k_batch_scatter(const mgv_query *__restrict__ Q,
+                                                         const mgv_work_item *__restrict__ items, BatchScatter R) {
     constexpr int NQ = T / (4 * BLOCK);
-    static_assert(NQ >= 1 && T % (4 * BLOCK) == 0, "slice scatter shape");
+    constexpr int PER = kBatchMaxParts / BLOCK > 0 ? kBatchMaxParts / BLOCK : 1;
+    static_assert(NQ >= 1 && T % (4 * BLOCK) == 0 && (BLOCK >= kBatchMaxParts || kBatchMaxParts % BLOCK == 0),
+                  "batch scatter shape");
     __shared__ int32_t sk[T];
     __shared__ int32_t sv[T];
-    __shared__ uint32_t cnt[kSliceRegions], start[kSliceRegions + 1], lim[kSliceRegions];
-    __shared__ unsigned long long dst[kSliceRegions];
+    __shared__ uint32_t cnt[kBatchMaxParts], start[kBatchMaxParts + 1], lim[kBatchMaxParts];
+    __shared__ unsigned long long dst[kBatchMaxParts];
+    __shared__ uint32_t wtot[BLOCK / kWave];
     __shared__ uint32_t s_item;
     const uint32_t t = threadIdx.x;
     for (;;) {
         if (t == 0) s_item = atomicAdd(&R.ctr[0], 1u);
-        if (t < kSliceRegions) cnt[t] = 0;   // all 32: a tile of 8 or 16 regions leaves the rest at 0
+        for (uint32_t p = t; p < (uint32_t)kBatchMaxParts; p += BLOCK) cnt[p] = 0;
         __syncthreads();
         const uint32_t it = s_item;
         if (it >= R.nscatter) break;
         const mgv_work_item wi = items[it];
         const mgv_query q = Q[wi.query];
+        const uint64_t base = R.regions[kRegionWords * wi.query], cap = R.regions[kRegionWords * wi.query + 1];
+        const uint32_t F = (uint32_t)R.regions[kRegionWords * wi.query + 2];
         const uint64_t hi = (wi.lo + wi.rows < q.n) ? wi.lo + wi.rows : q.n;
         v4i k[NQ], v[NQ];
         uint32_t m[NQ];
@@ -288,29 +265,35 @@ __global__ __launch_bounds__(BLOCK) void k_slice_scatter(const mgv_query *__rest
 #pragma unroll
             for (int e = 0; e < 4; ++e)
                 rank[j][e] = ((m[j] >> e) & 1u)
-                                 ? atomicAdd(&cnt[fmix32((uint32_t)q_get(k[j], e)) >> R.shift], 1u)
+                                 ? atomicAdd(&cnt[part_of_hash((int32_t)fmix32((uint32_t)q_get(k[j], e)), F)], 1u)
                                  : 0u;
         __syncthreads();
-        if (t == 0) {
-            uint32_t a = 0;
-            for (int p = 0; p < kSliceRegions; ++p) {
-                start[p] = a;
-                a += cnt[p];
-            }
-            start[kSliceRegions] = a;
+        // partition starts (block scan over F counts) and run reservations
+        uint32_t c[PER], mine = 0;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const uint32_t p = t * PER + e;
+            c[e] = p < F ? cnt[p] : 0u;
+            mine += c[e];
         }
-        if (t < kSliceRegions) {
-            const uint32_t c = cnt[t];
-            const uint64_t base = R.regions[2 * wi.query], cap = R.regions[2 * wi.query + 1];
-            uint32_t fit = 0;
-            if (c) {
-                const unsigned long long off =
-                    atomic_add_global(&R.fill[(uint64_t)wi.query * kSliceRegions + t], (unsigned long long)c);
-                const uint64_t room = off < cap ? cap - off : 0;
-                fit = room < c ? (uint32_t)room : c;
-                dst[t] = base + (uint64_t)t * cap + off;
+        uint32_t st = scan_block_excl<BLOCK>(mine, wtot);
+        if (t == BLOCK - 1) start[kBatchMaxParts] = st + mine;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const uint32_t p = t * PER + e;
+            if (p < F) {
+                start[p] = st;
+                uint32_t fit = 0;
+                if (c[e]) {
+                    const unsigned long long off = atomic_add_global(
+                        &R.fill[(uint64_t)wi.query * kBatchMaxParts + p], (unsigned long long)c[e]);
+                    const uint64_t room = off < cap ? cap - off : 0;
+                    fit = room < c[e] ? (uint32_t)room : c[e];
+                    dst[p] = base + (uint64_t)p * cap + off;
+                }
+                lim[p] = fit;
+                st += c[e];
             }
-            lim[t] = fit;
         }
         __syncthreads();
 #pragma unroll
@@ -318,35 +301,58 @@ __global__ __launch_bounds__(BLOCK) void k_slice_scatter(const mgv_query *__rest
 #pragma unroll
             for (int e = 0; e < 4; ++e)
                 if ((m[j] >> e) & 1u) {
-                    const uint32_t pos = start[fmix32((uint32_t)q_get(k[j], e)) >> R.shift] + rank[j][e];
-                    sk[pos] = q_get(k[j], e);
+                    const uint32_t h = fmix32((uint32_t)q_get(k[j], e));
+                    const uint32_t pos = start[part_of_hash((int32_t)h, F)] + rank[j][e];
+                    sk[pos] = R.hashed ? (int32_t)h : q_get(k[j], e);
                     sv[pos] = q_get(v[j], e);
                 }
         __syncthreads();
-        const uint32_t total = start[kSliceRegions];
+        const uint32_t total = start[kBatchMaxParts];
         uint64_t ocnt = 0, osum = 0;
         bool over = false;
         for (uint32_t i = t; i < total; i += BLOCK) {
-            const int32_t key = sk[i], val = sv[i];
-            const uint32_t p = fmix32((uint32_t)key) >> R.shift;
+            const int32_t x = sk[i], val = sv[i];
+            const uint32_t h = R.hashed ? (uint32_t)x : fmix32((uint32_t)x);
+            const uint32_t p = part_of_hash((int32_t)h, F);
             const uint32_t j = i - start[p];
             if (j < lim[p]) {
-                R.rkey[dst[p] + j] = key;
+                R.rkey[dst[p] + j] = x;
                 R.rval[dst[p] + j] = val;
             } else {
-                probe_row(q.slots, q.nbuckets, (uint32_t)key, val, ocnt, osum);
+                probe_row(q.slots, q.nbuckets, R.hashed ? fmix32_inv(h) : (uint32_t)x, val, ocnt, osum);
                 over = true;
             }
         }
         if (ballot(over)) {
-            const uint64_t a = wave_sum_u64(ocnt), b = wave_sum_u64(osum);
+            const uint64_t a = wave_sum_u64(ocnt), b2 = wave_sum_u64(osum);
             if (lane_id() == 0 && a) {
                 atomic_add_global((unsigned long long *)&q.res->count, (unsigned long long)a);
-                atomic_add_global((unsigned long long *)&q.res->sum, (unsigned long long)b);
+                atomic_add_global((unsigned long long *)&q.res->sum, (unsigned long long)b2);
             }
         }
         __syncthreads();
     }
+}
+
+template <int BLOCK, int T>
+static int batch_scatter_k(const mgv_query *queries, const mgv_work_item *items, const BatchScatter &S,
+                           hipStream_t s) {
+    auto kern = k_batch_scatter<BLOCK, T>;
+    static thread_local int cached = 0;
+    if (!cached) cached = persistent_blocks(kern, BLOCK);
+    const uint64_t grid = S.nscatter < (uint64_t)cached ? S.nscatter : (uint64_t)cached;
+    hipLaunchKernelGGL(kern, dim3((uint32_t)grid), dim3(BLOCK), 0, s, queries, items, S);
+    return check_launch("k_batch_scatter");
+}
+
+// Items of kBatchTile rows: 256-thread blocks; of kBatchTileWide rows (up
+// to 256 partitions a tile: runs four times longer): 1024-thread blocks.
+int launch_batch_scatter(const mgv_query *queries, const mgv_work_item *items, const BatchScatter &S,
+                         hipStream_t s) {
+    if (!S.nscatter) return MGV_OK;
+    if (S.tile == (uint32_t)kBatchTileWide) return batch_scatter_k<1024, kBatchTileWide>(queries, items, S, s);
+    if (S.tile == (uint32_t)kBatchTile) return batch_scatter_k<256, kBatchTile>(queries, items, S, s);
+    return fail(MGV_E_INVALID, "batch scatter: tile of %u rows", S.tile);
 }
 
 // The region probe of the sliced batch: a wave on XCD x claims items of
@@ -381,8 +387,8 @@ __global__ __launch_bounds__(BLOCK) void k_slice_probe(const mgv_query *__restri
         const uint32_t qi = wi.query & ((1u << kItemQueryBits) - 1u), region = wi.query >> kItemQueryBits;
         const mgv_query q = Q[qi];
         {
-            const uint64_t base = R.regions[2 * qi], cap = R.regions[2 * qi + 1];
-            const unsigned long long f = R.fill[(uint64_t)qi * kSliceRegions + region];
+            const uint64_t base = R.regions[kRegionWords * qi], cap = R.regions[kRegionWords * qi + 1];
+            const unsigned long long f = R.fill[(uint64_t)qi * kBatchMaxParts + region];
             const uint64_t n = f < cap ? f : cap;
             const uint64_t hi = (wi.lo + wi.rows < n) ? wi.lo + wi.rows : n;
             if (wi.lo < hi) {
@@ -435,22 +441,6 @@ static int slice_probe(const mgv_query *Q, const mgv_work_item *items, const Sli
         case 1024: return slice_probe_b<1024>(Q, items, R, L, s);
         default: return fail(MGV_E_INVALID, "block %u not in {64,128,256,512,1024}", L.block);
     }
-}
-
-// Workspace of a sliced batch: claim counters, fill counts, region columns.
-static uint64_t sliced_ws_bytes(uint32_t nq, uint64_t region_rows) {
-    const uint64_t fills = ((uint64_t)nq * kSliceRegions * 8 + 255) / 256 * 256;
-    const uint64_t col = (region_rows * 4 + 255) / 256 * 256;
-    return 256 + fills + 2 * col;
-}
-
-// Rows per region of a sliced query of n rows in nreg regions: n / nreg plus
-// eight standard deviations of a uniform hash partition's size, plus 256, in
-// multiples of 64.
-static uint64_t slice_cap(uint64_t n, uint32_t nreg) {
-    const double lam = (double)n / nreg;
-    const uint64_t c = (uint64_t)(lam + 8.0 * __builtin_sqrt(lam)) + 256;
-    return (c + 63) / 64 * 64;
 }
 
 }  // namespace mgv
@@ -513,7 +503,7 @@ int mgv_batch_plan_sliced(const mgv_query *queries, uint32_t nqueries, uint32_t 
                           uint32_t nregions, mgv_work_item *items, uint64_t cap, uint64_t *regions,
                           mgv_sliced_plan *plan) {
     if (!queries || !plan) return fail(MGV_E_INVALID, "mgv_batch_plan_sliced: null argument");
-    if (nregions == 0) nregions = kSliceRegions;
+    if (nregions == 0) nregions = 32;
     if (nregions != 8 && nregions != 16 && nregions != 32)
         return fail(MGV_E_INVALID, "mgv_batch_plan_sliced: %u regions per query (8, 16 or 32)", nregions);
     if (nqueries >= (1u << kItemQueryBits)) return fail(MGV_E_INVALID, "mgv_batch_plan_sliced: too many queries");
@@ -534,18 +524,21 @@ int mgv_batch_plan_sliced(const mgv_query *queries, uint32_t nqueries, uint32_t 
     };
     // regions
     for (uint32_t qi = 0; qi < nqueries; ++qi) {
-        const uint64_t c = sliced(queries[qi]) ? slice_cap(queries[qi].n, nregions) : 0;
+        const bool sl = sliced(queries[qi]);
+        const uint64_t c = sl ? batch_region_cap(queries[qi].n, nregions) : 0;
         if (regions) {
-            regions[2 * qi] = rows;
-            regions[2 * qi + 1] = c;
+            regions[kRegionWords * qi] = rows;
+            regions[kRegionWords * qi + 1] = c;
+            regions[kRegionWords * qi + 2] = sl ? nregions : 0;
+            regions[kRegionWords * qi + 3] = 0;
         }
         rows += (uint64_t)nregions * c;
     }
     // scatter tiles of the sliced queries
     for (uint32_t qi = 0; qi < nqueries; ++qi)
         if (sliced(queries[qi]))
-            for (uint64_t lo = 0; lo < queries[qi].n; lo += kSliceTile)
-                put(qi, lo, queries[qi].n - lo < (uint64_t)kSliceTile ? queries[qi].n - lo : kSliceTile);
+            for (uint64_t lo = 0; lo < queries[qi].n; lo += kBatchTile)
+                put(qi, lo, queries[qi].n - lo < (uint64_t)kBatchTile ? queries[qi].n - lo : kBatchTile);
     plan->nscatter = k;
     // probe sections x = 0..7: regions x, x + 8, ... of every sliced query,
     // query by query (items over each region's capacity)
@@ -553,7 +546,7 @@ int mgv_batch_plan_sliced(const mgv_query *queries, uint32_t nqueries, uint32_t 
         plan->section[x] = k;
         for (uint32_t qi = 0; qi < nqueries; ++qi)
             if (sliced(queries[qi])) {
-                const uint64_t c = slice_cap(queries[qi].n, nregions);
+                const uint64_t c = batch_region_cap(queries[qi].n, nregions);
                 for (uint32_t r = (uint32_t)x; r < nregions; r += kSliceParts)
                     for (uint64_t lo = 0; lo < c; lo += vector_rows)
                         put(qi | (r << kItemQueryBits), lo, c - lo < vector_rows ? c - lo : vector_rows);
@@ -570,7 +563,7 @@ int mgv_batch_plan_sliced(const mgv_query *queries, uint32_t nqueries, uint32_t 
     plan->nqueries = nqueries;
     plan->nregions = nregions;
     plan->region_rows = rows;
-    plan->workspace_bytes = sliced_ws_bytes(nqueries, rows);
+    plan->workspace_bytes = batch_ws_bytes(nqueries, rows);
     if (items && k > cap) return fail(MGV_E_CAPACITY, "mgv_batch_plan_sliced: %llu items > capacity %llu",
                                       (unsigned long long)k, (unsigned long long)cap);
     return MGV_OK;
@@ -587,14 +580,14 @@ int mgv_batch_run_sliced(const mgv_query *queries, const mgv_work_item *items, c
         return fail(MGV_E_INVALID, "mgv_batch_run_sliced: inconsistent plan");
     for (int p = 0; p <= kSliceParts; ++p)
         if (plan->section[p] > plan->section[p + 1]) return fail(MGV_E_INVALID, "mgv_batch_run_sliced: inconsistent plan");
-    const uint64_t need = sliced_ws_bytes(plan->nqueries, plan->region_rows);
+    const uint64_t need = batch_ws_bytes(plan->nqueries, plan->region_rows);
     if (ws_bytes < need) return fail(MGV_E_CAPACITY, "mgv_batch_run_sliced: workspace %llu < %llu bytes",
                                      (unsigned long long)ws_bytes, (unsigned long long)need);
     if (!aligned16(ws)) return fail(MGV_E_ALIGN, "mgv_batch_run_sliced: workspace not 16-byte aligned");
     if (plan->nitems == 0) return MGV_OK;
     hipStream_t s = stream_of(st);
     char *w = (char *)ws;
-    const uint64_t fills = ((uint64_t)plan->nqueries * kSliceRegions * 8 + 255) / 256 * 256;
+    const uint64_t fills = ((uint64_t)plan->nqueries * kBatchMaxParts * 8 + 255) / 256 * 256;
     const uint64_t col = (plan->region_rows * 4 + 255) / 256 * 256;
     SlicedRun R = {};
     R.regions = regions;
@@ -602,20 +595,12 @@ int mgv_batch_run_sliced(const mgv_query *queries, const mgv_work_item *items, c
     R.fill = (unsigned long long *)(w + 256);
     R.rkey = (int32_t *)(w + 256 + fills);
     R.rval = (int32_t *)(w + 256 + fills + col);
-    R.nscatter = plan->nscatter;
-    R.shift = 32u - (uint32_t)__builtin_ctz(plan->nregions);
     for (int p = 0; p <= kSliceParts + 1; ++p) R.sec[p] = plan->section[p];
     // counters and fill counts: a fill kernel (the call can be captured into a graph)
     int rc = mgv_fill_u64((uint64_t *)ws, (256 + fills) / 8, 0, (mgv_stream_t)s);
     if (rc) return rc;
-    if (plan->nscatter) {
-        auto kern = k_slice_scatter<kSliceScatterBlock, kSliceTile>;
-        static thread_local int cached = 0;
-        if (!cached) cached = persistent_blocks(kern, kSliceScatterBlock);
-        const uint64_t grid = plan->nscatter < (uint64_t)cached ? plan->nscatter : (uint64_t)cached;
-        hipLaunchKernelGGL(kern, dim3((uint32_t)grid), dim3(kSliceScatterBlock), 0, s, queries, items, R);
-        if ((rc = check_launch("k_slice_scatter"))) return rc;
-    }
+    const BatchScatter S = {regions, R.fill, R.rkey, R.rval, R.ctr, plan->nscatter, 0u, (uint32_t)kBatchTile};
+    if ((rc = launch_batch_scatter(queries, items, S, s))) return rc;
     if (plan->section[kSliceParts] > plan->section[0] && (rc = slice_probe(queries, items, R, L, s))) return rc;
     // section 8: the other queries' vectors, as mgv_batch_run
     const uint64_t nd = plan->section[kSliceParts + 1] - plan->section[kSliceParts];

@@ -25,6 +25,7 @@
 // (src/operators/aggregation.cu:27-66).
 #include <hip/hip_runtime.h>
 
+#include "mgv_batch.hpp"
 #include "mgv_internal.hpp"
 #include "mgv_ops.hpp"
 #include "mgv_part.hpp"
@@ -1423,6 +1424,105 @@ static int probe_lds_fill(const uint64_t *slots, uint32_t nbuckets, const int32_
     return check_launch("k_probe_hashed");
 }
 
+// ------------------------------------------------------------------ LDS batch probe
+//
+// The probes of a batch whose tables split into slices of at most 2048
+// buckets: every such query's selected rows are scattered (k_batch_scatter,
+// hashed keys) into F regions, F = nbuckets / slice, region p holding the
+// rows whose home bucket lies in slice p; one persistent launch then takes
+// one (query, region) item per block iteration, copies the query's slice
+// into LDS as key slices, probes the region's rows from it exactly as
+// k_lds_probe does, and adds the counts to the query's result.  The probes
+// never fetch a random line (a line miss costs about 1/55 G s on MI355X,
+// the rate that bounds the fused batch's direct probes).
+
+struct LdsBatchArgs {
+    const mgv_query *Q;
+    const mgv_work_item *items;   // partition items: .query, .lo = region index
+    uint64_t nparts;
+    const uint64_t *regions;      // kRegionWords per query: first row, rows per region, F, slice buckets
+    const unsigned long long *fill;
+    const int32_t *rhash, *rval;  // region columns: fmix32(key), val
+    uint32_t *next;               // claim counter (zeroed before the launch)
+};
+
+__global__ __launch_bounds__(kLdsBlock) void k_batch_lds_probe(LdsBatchArgs a) {
+    typedef SliceT<kSliceMax> Slice;
+    __shared__ __align__(16) Slice S;
+    __shared__ WaveQueue s_queue[kLdsBlock / kWave];
+    __shared__ uint32_t s_part;
+    const uint32_t t = threadIdx.x;
+    WaveQueue &Q = s_queue[t / kWave];
+    const TagGeom geom = {0u, 0u, 0u};   // key slices: unused
+    if (t == 0) S.pay[Slice::kZero] = 0u;   // visible after the first barrier below
+    for (;;) {
+        if (t == 0) s_part = atomicAdd(a.next, 1u);
+        __syncthreads();
+        const uint32_t it = s_part;
+        __syncthreads();   // s_part is rewritten next round; the slice is free again
+        if (it >= a.nparts) break;
+        const mgv_work_item wi = a.items[it];
+        const uint32_t qi = wi.query;
+        const mgv_query q = a.Q[qi];
+        const uint64_t *rg = a.regions + (uint64_t)kRegionWords * qi;
+        const uint64_t cap = rg[1];
+        const uint32_t slice = (uint32_t)rg[3];
+        const uint32_t p = (uint32_t)wi.lo;
+        const uint64_t r0 = rg[0] + (uint64_t)p * cap;
+        const unsigned long long f = a.fill[(uint64_t)qi * kBatchMaxParts + p];
+        const uint64_t r1 = r0 + (f < cap ? f : cap);
+        if (r0 == r1) continue;
+        const uint32_t nb = q.nbuckets;
+        const uint32_t lo = p * slice;
+        const uint32_t nload = min(slice + kSliceExtra, nb - lo);   // slice <= kSliceMax
+        AccSum acc;
+        v4i k[kAhead], v[kAhead];
+        uint32_t m[kAhead];
+        uint64_t i0 = r0 + 4ull * t;
+#pragma unroll
+        for (int x = 0; x < kAhead; ++x) {
+            const uint64_t ix = i0 + (uint64_t)x * kQuadStride;
+            m[x] = load_rows4(a.rhash, a.rval, ix < r1 ? ix : r0, ix < r1 ? r1 : r0, k[x], v[x]);
+        }
+        load_slice(S, q.slots, lo, nload, nb, geom);
+        __syncthreads();
+        // r0 is a multiple of 64 rows, so quads are 16-byte aligned
+        uint32_t head = 0, tail = 0;   // the wave's queue (wave-uniform counters)
+        while (ballot(i0 < r1)) {
+            const uint64_t i1 = i0 + (uint64_t)kAhead * kQuadStride;
+            v4i kn[kAhead], vn[kAhead];
+            uint32_t mn[kAhead];
+#pragma unroll
+            for (int x = 0; x < kAhead; ++x) {
+                const uint64_t ix = i1 + (uint64_t)x * kQuadStride;
+                mn[x] = load_rows4(a.rhash, a.rval, ix < r1 ? ix : r0, ix < r1 ? r1 : r0, kn[x], vn[x]);
+            }
+#pragma unroll
+            for (int x = 0; x < kAhead; ++x)
+                lds_probe_quad(k[x], v[x], m[x], S, q.slots, nb, lo, nload, geom, acc, Q, head, tail);
+#pragma unroll
+            for (int x = 0; x < kAhead; ++x) {
+                k[x] = kn[x];
+                v[x] = vn[x];
+                m[x] = mn[x];
+            }
+            i0 = i1;
+        }
+        while (tail != head) head = queue_round(Q, head, tail, S, q.slots, nb, lo, nload, geom, acc);
+        acc.close(LdsAccArgs{q.res, nullptr, nullptr, 0});
+        __syncthreads();
+    }
+}
+
+// Partitions of a query's table for the LDS batch probe: the smallest power
+// of two F <= kBatchMaxParts that divides nbuckets into slices of at most
+// kSliceMax buckets (0: none).
+static uint32_t lds_batch_parts(uint32_t nb) {
+    for (uint32_t F = 1; F <= (uint32_t)kBatchMaxParts; F <<= 1)
+        if (nb % F == 0 && nb / F <= kSliceMax && nb / F > 0) return F;
+    return 0;
+}
+
 // Slices of the group-by probe: its LDS group table needs half the slice.
 constexpr uint32_t kGroupSlice = 1024;
 constexpr uint32_t kLdsMaxGroups = 4096;
@@ -1644,6 +1744,97 @@ int mgv_hashjoin_build_lds(uint64_t *slots, uint32_t nbuckets, const int32_t *ke
     const HashedRows none = {nullptr, nullptr, nullptr, 0};
     hipLaunchKernelGGL(k_lds_build_overflow, dim3(256), dim3(256), 0, s, a, none, none);
     return check_launch("k_lds_build_overflow");
+}
+
+int mgv_batch_plan_lds(const mgv_query *queries, uint32_t nqueries, uint32_t vector_rows, uint64_t min_rows,
+                       mgv_work_item *items, uint64_t cap, uint64_t *regions, mgv_lds_batch_plan *plan) {
+    if (!queries || !plan) return fail(MGV_E_INVALID, "mgv_batch_plan_lds: null argument");
+    uint64_t n = 0;
+    int rc = mgv_batch_plan(queries, nqueries, vector_rows, nullptr, 0, &n);   // validates the queries
+    if (rc) return rc;
+    auto parts = [&](const mgv_query &q) -> uint32_t {
+        if (q.kind != MGV_Q_PROBE_SUM || (q.flags & MGV_PROBE_DIGEST) || q.n == 0 || q.n < min_rows) return 0;
+        return lds_batch_parts(q.nbuckets);
+    };
+    uint64_t k = 0, rows = 0;
+    auto put = [&](uint32_t qi, uint64_t lo, uint64_t r) {
+        if (items && k < cap) {
+            items[k].query = qi;
+            items[k].rows = (uint32_t)r;
+            items[k].lo = lo;
+        }
+        ++k;
+    };
+    for (uint32_t qi = 0; qi < nqueries; ++qi) {
+        const uint32_t F = parts(queries[qi]);
+        const uint64_t c = F ? batch_region_cap(queries[qi].n, F) : 0;
+        if (regions) {
+            regions[kRegionWords * qi] = rows;
+            regions[kRegionWords * qi + 1] = c;
+            regions[kRegionWords * qi + 2] = F;
+            regions[kRegionWords * qi + 3] = F ? queries[qi].nbuckets / F : 0;
+        }
+        rows += (uint64_t)F * c;
+    }
+    for (uint32_t qi = 0; qi < nqueries; ++qi)
+        if (parts(queries[qi]))
+            for (uint64_t lo = 0; lo < queries[qi].n; lo += kBatchTileWide)
+                put(qi, lo, queries[qi].n - lo < (uint64_t)kBatchTileWide ? queries[qi].n - lo : kBatchTileWide);
+    plan->nscatter = k;
+    for (uint32_t qi = 0; qi < nqueries; ++qi) {
+        const uint32_t F = parts(queries[qi]);
+        for (uint32_t p = 0; p < F; ++p) put(qi, p, 0);
+    }
+    plan->nparts = k - plan->nscatter;
+    for (uint32_t qi = 0; qi < nqueries; ++qi)
+        if (!parts(queries[qi]))
+            for (uint64_t lo = 0; lo < queries[qi].n; lo += vector_rows)
+                put(qi, lo, queries[qi].n - lo < vector_rows ? queries[qi].n - lo : vector_rows);
+    plan->nitems = k;
+    plan->nqueries = nqueries;
+    plan->reserved = 0;
+    plan->region_rows = rows;
+    plan->workspace_bytes = batch_ws_bytes(nqueries, rows);
+    if (items && k > cap) return fail(MGV_E_CAPACITY, "mgv_batch_plan_lds: %llu items > capacity %llu",
+                                      (unsigned long long)k, (unsigned long long)cap);
+    return MGV_OK;
+}
+
+int mgv_batch_run_lds(const mgv_query *queries, const mgv_work_item *items, const uint64_t *regions,
+                      const mgv_lds_batch_plan *plan, void *ws, uint64_t ws_bytes, mgv_launch L, mgv_stream_t st) {
+    if (!current_state()) return fail(MGV_E_NOT_INIT, "mgv_init was not called");
+    if (!queries || !plan || !regions || !ws || (plan->nitems && !items))
+        return fail(MGV_E_INVALID, "mgv_batch_run_lds: null argument");
+    if (plan->nitems >= 0xffffffffull || plan->nscatter + plan->nparts > plan->nitems)
+        return fail(MGV_E_INVALID, "mgv_batch_run_lds: inconsistent plan");
+    const uint64_t need = batch_ws_bytes(plan->nqueries, plan->region_rows);
+    if (ws_bytes < need) return fail(MGV_E_CAPACITY, "mgv_batch_run_lds: workspace %llu < %llu bytes",
+                                     (unsigned long long)ws_bytes, (unsigned long long)need);
+    if (!aligned16(ws)) return fail(MGV_E_ALIGN, "mgv_batch_run_lds: workspace not 16-byte aligned");
+    if (plan->nitems == 0) return MGV_OK;
+    hipStream_t s = stream_of(st);
+    char *w = (char *)ws;
+    const uint64_t fills = batch_fill_bytes(plan->nqueries);
+    const uint64_t col = (plan->region_rows * 4 + 255) / 256 * 256;
+    uint32_t *ctr = (uint32_t *)w;   // [0] scatter claims, [1] partition claims, [8] the direct launch
+    unsigned long long *fill = (unsigned long long *)(w + 256);
+    int32_t *rhash = (int32_t *)(w + 256 + fills), *rval = (int32_t *)(w + 256 + fills + col);
+    // counters and fill counts: a fill kernel (the call can be captured into a graph)
+    int rc = mgv_fill_u64((uint64_t *)ws, (256 + fills) / 8, 0, (mgv_stream_t)s);
+    if (rc) return rc;
+    const BatchScatter S = {regions, fill, rhash, rval, ctr, plan->nscatter, 1u, (uint32_t)kBatchTileWide};
+    if ((rc = launch_batch_scatter(queries, items, S, s))) return rc;
+    if (plan->nparts) {
+        LdsBatchArgs a = {queries, items + plan->nscatter, plan->nparts, regions, fill, rhash, rval, ctr + 1};
+        static thread_local int cached = 0;
+        if (!cached) cached = persistent_blocks(k_batch_lds_probe, kLdsBlock);
+        const uint64_t grid = plan->nparts < (uint64_t)cached ? plan->nparts : (uint64_t)cached;
+        hipLaunchKernelGGL(k_batch_lds_probe, dim3((uint32_t)grid), dim3(kLdsBlock), 0, s, a);
+        if ((rc = check_launch("k_batch_lds_probe"))) return rc;
+    }
+    const uint64_t nd = plan->nitems - plan->nscatter - plan->nparts;
+    if (!nd) return MGV_OK;
+    return mgv_batch_run(queries, items + plan->nscatter + plan->nparts, nd, ctr + 8, L, st);
 }
 
 }  // extern "C"

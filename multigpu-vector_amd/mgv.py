@@ -101,6 +101,14 @@ class WorkItem(C.Structure):
     _fields_ = [("query", C.c_uint32), ("rows", C.c_uint32), ("lo", C.c_uint64)]
 
 
+class LdsBatchPlan(C.Structure):
+    """mgv_lds_batch_plan: item ranges and workspace of an LDS batch probe."""
+    _fields_ = [
+        ("nitems", C.c_uint64), ("nscatter", C.c_uint64), ("nparts", C.c_uint64), ("region_rows", C.c_uint64),
+        ("workspace_bytes", C.c_uint64), ("nqueries", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+
 class SlicedPlan(C.Structure):
     """mgv_sliced_plan: item sections and workspace of an XCD-sliced batch."""
     _fields_ = [
@@ -168,6 +176,8 @@ SIGNATURES = {
     "mgv_batch_run": [VP, VP, U64, VP, Launch, VP],
     "mgv_batch_plan_sliced": [VP, U32, U32, U64, U32, VP, U64, VP, VP],
     "mgv_batch_run_sliced": [VP, VP, VP, VP, VP, U64, Launch, VP],
+    "mgv_batch_plan_lds": [VP, U32, U32, U64, VP, U64, VP, VP],
+    "mgv_batch_run_lds": [VP, VP, VP, VP, VP, U64, Launch, VP],
     "mgv_pool_create": [U32, U32, C.POINTER(VP)],
     "mgv_pool_destroy": [VP],
     "mgv_pool_acquire": [VP, C.POINTER(I32), VP],

@@ -120,7 +120,7 @@ class JoinBatch:
 
     def _drop_plans(self):
         for plan in self._plans.values():
-            if isinstance(plan, dict) and "sliced" in plan:
+            if isinstance(plan, dict) and ("sliced" in plan or "ldsb" in plan):
                 for b in (plan["dq"], plan["di"], plan["dr"], plan["ws"]):
                     b.close()
                 continue
@@ -275,7 +275,7 @@ class JoinBatch:
         if key in self._plans:
             return self._plans[key]
         plan = mgv.SlicedPlan()
-        regions = (C.c_uint64 * (2 * max(1, self.nq)))()
+        regions = (C.c_uint64 * (4 * max(1, self.nq)))()
         mgv.call("mgv_batch_plan_sliced", self.probe_q, self.nq, vector_rows, min_rows, nregions, None, 0, regions,
                  C.byref(plan))
         items = (mgv.WorkItem * max(1, plan.nitems))()
@@ -286,7 +286,7 @@ class JoinBatch:
         dr = mgv.DeviceBuffer(C.sizeof(regions)).upload(np.frombuffer(bytes(regions), dtype=np.uint8))
         ws = mgv.DeviceBuffer(max(256, plan.workspace_bytes))
         out = {"sliced": True, "plan": plan, "dq": dq, "di": di, "dr": dr, "ws": ws,
-               "queries": sum(1 for i in range(self.nq) if regions[2 * i + 1])}
+               "queries": sum(1 for i in range(self.nq) if regions[4 * i + 2])}
         self._plans[key] = out
         return out
 
@@ -310,6 +310,49 @@ class JoinBatch:
         self._sliced(vector_rows, min_rows, nregions)
         mgv.call("mgv_graph_begin_capture", stream)
         self.issue_sliced(vector_rows, launch, stream, min_rows, nregions)
+        g = C.c_void_p()
+        mgv.call("mgv_graph_end_capture", stream, C.byref(g))
+        return g
+
+    # -------------------------------------------------------------- LDS batch path
+    def _ldsb(self, vector_rows: int, min_rows: int):
+        """Plan of the LDS batch probe (mgv_batch_plan_lds over the probe
+        queries): device queries, items, regions and workspace; cached."""
+        key = ("ldsb", vector_rows, min_rows)
+        if key in self._plans:
+            return self._plans[key]
+        plan = mgv.LdsBatchPlan()
+        regions = (C.c_uint64 * (4 * max(1, self.nq)))()
+        mgv.call("mgv_batch_plan_lds", self.probe_q, self.nq, vector_rows, min_rows, None, 0, regions, C.byref(plan))
+        items = (mgv.WorkItem * max(1, plan.nitems))()
+        mgv.call("mgv_batch_plan_lds", self.probe_q, self.nq, vector_rows, min_rows, items, plan.nitems, regions,
+                 C.byref(plan))
+        dq = mgv.DeviceBuffer(C.sizeof(self.probe_q)).upload(np.frombuffer(bytes(self.probe_q), dtype=np.uint8))
+        di = mgv.DeviceBuffer(max(16, C.sizeof(items))).upload(np.frombuffer(bytes(items), dtype=np.uint8))
+        dr = mgv.DeviceBuffer(C.sizeof(regions)).upload(np.frombuffer(bytes(regions), dtype=np.uint8))
+        ws = mgv.DeviceBuffer(max(256, plan.workspace_bytes))
+        out = {"ldsb": True, "plan": plan, "dq": dq, "di": di, "dr": dr, "ws": ws,
+               "queries": sum(1 for i in range(self.nq) if regions[4 * i + 2])}
+        self._plans[key] = out
+        return out
+
+    def issue_ldsb(self, vector_rows: int, launch: mgv.Launch, stream=None, min_rows: int = 1 << 16):
+        """clear + the fused build launch + the LDS batch probe
+        (mgv_batch_run_lds: probes of at least min_rows rows scattered by
+        table slice, each (query, slice) probed from LDS)."""
+        s = stream if stream is not None else self.st
+        (bq, bi, bn), _ = self._plan(vector_rows)
+        lp = self._ldsb(vector_rows, min_rows)
+        self.clear(s)
+        mgv.call("mgv_batch_run", bq.vp, bi.vp, bn, self.counter.vp, launch, s)
+        mgv.call("mgv_batch_run_lds", lp["dq"].vp, lp["di"].vp, lp["dr"].vp, C.byref(lp["plan"]), lp["ws"].vp,
+                 lp["ws"].nbytes, launch, s)
+
+    def capture_ldsb(self, vector_rows: int, launch: mgv.Launch, stream, min_rows: int = 1 << 16) -> C.c_void_p:
+        self._plan(vector_rows)
+        self._ldsb(vector_rows, min_rows)
+        mgv.call("mgv_graph_begin_capture", stream)
+        self.issue_ldsb(vector_rows, launch, stream, min_rows)
         g = C.c_void_p()
         mgv.call("mgv_graph_end_capture", stream, C.byref(g))
         return g

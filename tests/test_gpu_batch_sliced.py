@@ -76,7 +76,7 @@ def build_tables(gpu, qs, lf=0.5):
 
 def plan_sliced(gpu, probe_q, nq, vector_rows, min_rows, nregions=0):
     plan = mgv.SlicedPlan()
-    regions = (C.c_uint64 * (2 * nq))()
+    regions = (C.c_uint64 * (4 * nq))()
     gpu.call("mgv_batch_plan_sliced", probe_q, nq, vector_rows, min_rows, nregions, None, 0, regions, C.byref(plan))
     items = (mgv.WorkItem * max(1, plan.nitems))()
     gpu.call("mgv_batch_plan_sliced", probe_q, nq, vector_rows, min_rows, nregions, items, plan.nitems, regions,
@@ -96,7 +96,7 @@ def run_sliced(gpu, qs, vector_rows, min_rows, block=256, compact=1, flags=0, nr
     gpu.call("mgv_batch_run_sliced", dq.vp, di.vp, dr.vp, C.byref(plan), ws.vp, ws.nbytes,
              mgv.Launch(0, block, compact, flags), None)
     mgv.sync()
-    sliced = sum(1 for i in range(nq) if regions[2 * i + 1])
+    sliced = sum(1 for i in range(nq) if regions[4 * i + 2])
     return [res.read(nq + i) for i in range(nq)], sliced, plan
 
 
@@ -194,3 +194,79 @@ def test_sliced_batch_rejects_bad_arguments(gpu, orc):
     mgv.sync()
     # nothing ran: every probe result is still zero
     assert all((res.read(nq + i).count, res.read(nq + i).sum) == (0, 0) for i in range(nq))
+
+
+# ---------------------------------------------------------------- LDS batch probe
+
+def run_ldsb(gpu, qs, vector_rows, min_rows, block=256, compact=1, lf=0.5):
+    nq = len(qs)
+    tables, res, probe_q = build_tables(gpu, qs, lf)
+    plan = mgv.LdsBatchPlan()
+    regions = (C.c_uint64 * (4 * nq))()
+    gpu.call("mgv_batch_plan_lds", probe_q, nq, vector_rows, min_rows, None, 0, regions, C.byref(plan))
+    items = (mgv.WorkItem * max(1, plan.nitems))()
+    gpu.call("mgv_batch_plan_lds", probe_q, nq, vector_rows, min_rows, items, plan.nitems, regions, C.byref(plan))
+    dq, di, dr = upload(probe_q), upload(items), upload(regions)
+    ws = mgv.DeviceBuffer(max(256, plan.workspace_bytes))
+    gpu.call("mgv_batch_run_lds", dq.vp, di.vp, dr.vp, C.byref(plan), ws.vp, ws.nbytes,
+             mgv.launch(0, block, compact), None)
+    mgv.sync()
+    lds = [i for i in range(nq) if regions[4 * i + 2]]
+    for i in lds:   # the slices cover the table
+        assert regions[4 * i + 2] * regions[4 * i + 3] == tables[i].nbuckets and regions[4 * i + 3] <= 2048
+    return [res.read(nq + i) for i in range(nq)], lds, plan
+
+
+@pytest.mark.parametrize("vector_rows,min_rows,block,compact,lf", [(1024, 0, 256, 1, 0.5), (4096, 1 << 12, 64, 0, 0.25),
+                                                                   (2048, 1 << 30, 512, 1, 0.5),
+                                                                   (8192, 1 << 14, 1024, 0, 0.75)])
+def test_lds_batch_matches_oracle(gpu, orc, vector_rows, min_rows, block, compact, lf):
+    qs = make_queries(orc, 24, seed_cfg=13, hi_log=19)
+    # the oracle's tables do not depend on the load factor; the GPU tables do
+    got, lds, plan = run_ldsb(gpu, qs, vector_rows, min_rows, block, compact, lf)
+    assert plan.nparts == 0 or len(lds) > 0
+    if min_rows <= 1 << 12:
+        assert len(lds) >= 8
+    for q, r in zip(qs, got):
+        assert (r.count, r.sum) == (q["want"]["count"], q["want"]["sum"]), (q["P"], q["bound"])
+
+
+def test_lds_batch_skewed_keys_and_duplicate_build_keys(gpu, orc):
+    """Probes of 2^18.1-2^19.5 rows: their tables (more than 4096 buckets, a
+    multiple of 256) all split into slices, so query 3's 90 % of rows on one
+    key overfill their region and are probed by the scatter itself; queries
+    5 and 9 have duplicate build keys."""
+    qs = make_queries(orc, 12, seed_cfg=5, lo_log=18.1, hi_log=19.5, hot=3, dup=(5, 9))
+    got, lds, _ = run_ldsb(gpu, qs, 2048, 0)
+    assert len(lds) == len(qs)
+    for q, r in zip(qs, got):
+        assert (r.count, r.sum) == (q["want"]["count"], q["want"]["sum"]), (q["P"], q["bound"])
+
+
+def test_lds_batch_equals_fused_on_graph_replays(gpu, orc):
+    """The C4-shaped batch of the sliced test through the LDS batch probe,
+    captured with the builds into one graph: every query equals the fused
+    batch on every replay."""
+    import workloads
+    shapes = workloads.c4_shapes(40, seed=0x5C, lo=12.0, hi=20.0)
+    st = mgv.Stream()
+    batch = workloads.JoinBatch(shapes, 0.25, st.vp)
+    st.sync()
+    L = mgv.launch(0, 256, 0)
+    g = batch.capture_fused(4096, L, st.vp)
+    gpu.call("mgv_graph_launch", g, st.vp)
+    st.sync()
+    fused = batch.results()
+    gpu.call("mgv_graph_destroy", g)
+    g = batch.capture_ldsb(4096, L, st.vp, min_rows=1 << 14)
+    assert batch._ldsb(4096, 1 << 14)["queries"] >= 4
+    for rep in range(3):
+        gpu.call("mgv_graph_launch", g, st.vp)
+        st.sync()
+        r = batch.results()
+        assert np.array_equal(r["count"], fused["count"]) and np.array_equal(r["sum"], fused["sum"]), rep
+        batch.res.clear(st.vp)
+        st.sync()
+    gpu.call("mgv_graph_destroy", g)
+    batch.close()
+    st.close()
