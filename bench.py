@@ -830,6 +830,7 @@ def leg_c4(args, st):
     routed_same = bool(np.array_equal(routed["count"], fused["count"]) and
                        np.array_equal(routed["sum"], fused["sum"]))
     nbig = len(batch._routed(V, thr, nst, fan)["big"])
+    mgv.call("mgv_graph_destroy", g)   # before any load-factor change frees what the graph uses
     log(f"C4 routed ({nbig} queries >= {thr} rows through LDS slices on {nst} streams): {min(rw) * 1e3:.2f} ms "
         f"wall, equal to fused: {routed_same}")
     # XCD-sliced: the large probes' rows scattered into table-part regions,
@@ -849,6 +850,33 @@ def leg_c4(args, st):
     mgv.call("mgv_graph_destroy", gs)
     log(f"C4 sliced ({nsl} queries >= {smin} rows in {sreg} regions each): {min(sw_) * 1e3:.2f} ms wall, "
         f"equal to fused: {sliced_same}")
+    # LDS batch: the large probes' rows scattered by table slice, each
+    # (query, slice) probed from LDS (mgv_batch_run_lds), in one graph; the
+    # slices hold half as many buckets at load factor 0.5, so it is timed at
+    # the tuned load factor and at 0.5 (the tables are rebuilt in the graph)
+    lmin = 1 << 14
+    ldsb = {}
+    for lfl in sorted({lf, 0.5}):
+        if batch.lf != lfl:
+            batch.set_load_factor(lfl)
+        gl = batch.capture_ldsb(V, L, st.vp, min_rows=lmin)
+
+        def run_ldsb_graph():
+            mgv.call("mgv_graph_launch", gl, st.vp)
+
+        timed(run_ldsb_graph, 2)
+        lw, le = timed(run_ldsb_graph, args.c4_reps)
+        r = batch.results()
+        same = bool(np.array_equal(r["count"], fused["count"]) and np.array_equal(r["sum"], fused["sum"]))
+        mgv.call("mgv_graph_destroy", gl)
+        ldsb[lfl] = (lw, le, same, batch._ldsb(V, lmin)["queries"])
+        log(f"C4 LDS batch at load factor {lfl} ({ldsb[lfl][3]} queries >= {lmin} rows): {min(lw) * 1e3:.2f} ms "
+            f"wall, equal to fused: {same}")
+    if batch.lf != lf:
+        batch.set_load_factor(lf)
+    ldsb_lf = min(ldsb, key=lambda k: min(ldsb[k][0]))
+    lw, le, ldsb_same, nlq = ldsb[ldsb_lf]
+    ldsb_same = all(v[2] for v in ldsb.values())
     plain_out = {}
     plain_res = None
     for pv in ((4096, 65536) if args.c4_plain else ()):
@@ -876,16 +904,16 @@ def leg_c4(args, st):
         orc.free(tb)
         ok &= (int(fused["count"][i]), int(fused["sum"][i])) == (want["count"], want["sum"])
         checked += 1
-    mgv.call("mgv_graph_destroy", g)
-    paths = {"fused": (bw, be), "routed": (rw, re_), "sliced": (sw_, se)}
+    paths = {"fused": (bw, be), "routed": (rw, re_), "sliced": (sw_, se), "lds_batch": (lw, le)}
     qps = {k: len(shapes) / min(w) for k, (w, _) in paths.items()}
     fused_qps, routed_qps = qps["fused"], qps["routed"]
     best_path = max(qps, key=qps.get)
     best_w, best_e = paths[best_path]
-    algo = batch.algorithmic_bytes()
+    algo = batch.algorithmic_bytes(ldsb_lf if best_path == "lds_batch" else None)
     out = {
         "queries": len(shapes), "probe_rows": batch.r_rows, "build_rows": batch.s_rows,
         "tuned": {"vector_rows": V, "block": blk, "compact": compact, "load_factor": lf, "from_file": bool(t)},
+        "load_factor_of_best_path": ldsb_lf if best_path == "lds_batch" else lf,
         "batched": {"wall_ms": min(best_w) * 1e3, "gpu_ms": min(best_e), "queries_per_s": qps[best_path],
                     "path": best_path,
                     "algorithmic_gbps": algo / (min(best_e) / 1e3) / 1e9,
@@ -895,16 +923,20 @@ def leg_c4(args, st):
                    "lds_queries": nbig, "threshold_rows": thr, "side_streams": nst, "fanout": fan or "one level"},
         "sliced": {"wall_ms": min(sw_) * 1e3, "gpu_ms": min(se), "queries_per_s": qps["sliced"],
                    "sliced_queries": nsl, "min_rows": smin, "regions": sreg},
+        "lds_batch": {"wall_ms": min(lw) * 1e3, "gpu_ms": min(le), "queries_per_s": qps["lds_batch"],
+                      "lds_queries": nlq, "min_rows": lmin, "load_factor": ldsb_lf,
+                      "wall_ms_by_load_factor": {str(k): min(v[0]) * 1e3 for k, v in ldsb.items()}},
         "plain": plain_out,
         "speedup_vs_best_plain": qps[best_path] / best_plain["queries_per_s"] if best_plain else None,
         "parity": {"fused_equals_plain": (all(v["results_equal_fused"] for v in plain_out.values())
                                           if plain_out else None),
                    "routed_equals_fused": routed_same, "sliced_equals_fused": sliced_same,
+                   "lds_batch_equals_fused": ldsb_same,
                    "oracle_checked_queries": checked, "oracle_equal": bool(ok)},
         "total_count": int(fused["count"].sum()), "total_sum": int(fused["sum"].sum()),
     }
     sp = out["speedup_vs_best_plain"]
-    log(f"C4 batched {min(bw) * 1e3:.2f} ms wall ({fused_qps:.0f} q/s), "
+    log(f"C4 batched ({best_path}) {min(best_w) * 1e3:.2f} ms wall ({qps[best_path]:.0f} q/s), "
         f"{'x%.2f vs best plain' % sp if sp else 'plain path not run'}; oracle {checked} queries ok={ok}")
     del plain_res
     batch.close()
@@ -1113,6 +1145,7 @@ def parity_failures(line):
             flags["c4 fused = plain"] = c4["parity"]["fused_equals_plain"]
         flags["c4 routed = fused"] = c4["parity"]["routed_equals_fused"]
         flags["c4 sliced = fused"] = c4["parity"]["sliced_equals_fused"]
+        flags["c4 LDS batch = fused"] = c4["parity"]["lds_batch_equals_fused"]
         flags["c4 oracle"] = c4["parity"]["oracle_equal"]
     for k, v in (line.get("f3") or {}).items():
         flags[f"f3 {k} kept rows"] = v["kept_rows_ok"]

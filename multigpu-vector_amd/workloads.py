@@ -376,15 +376,17 @@ class JoinBatch:
         return {"count": probes[:, 0].copy(), "sum": probes[:, 1].view(np.int64).copy(),
                 "build_errors": builds[:, 4].copy()}
 
-    def algorithmic_bytes(self) -> int:
+    def algorithmic_bytes(self, load_factor: float = None) -> int:
         """Probe: 4 B of c per row + 8 B (k, v) per 32-byte granule with a
         survivor (g = 1 - (1 - s)^8) + an 8-byte slot per survivor; build:
-        8 B per row read + the table written."""
+        8 B per row read + the table written (tables at load_factor, default
+        the batch's)."""
         tot = 0.0
         for x in self.q:
             g = 1.0 - (1.0 - x.sel) ** 8
             tot += x.P * (4 + 8 * g + 8 * x.sel)
-            tot += x.B * 8 + x.nbuckets * 64
+            nb = x.nbuckets if load_factor is None else mgv.ht_plan(x.B, load_factor)[0]
+            tot += x.B * 8 + nb * 64
         return int(tot)
 
     def host_columns(self, i: int):
