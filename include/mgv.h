@@ -443,19 +443,26 @@ int mgv_batch_plan_sliced(const mgv_query *queries, uint32_t nqueries, uint32_t 
 int mgv_batch_run_sliced(const mgv_query *queries, const mgv_work_item *items, const uint64_t *regions,
                          const mgv_sliced_plan *plan, void *ws, uint64_t ws_bytes, mgv_launch launch, mgv_stream_t s);
 
-/* LDS batch probe (same per-query results as mgv_batch_run): the PROBE_SUM
- * queries (no MGV_PROBE_DIGEST, n >= min_rows) whose table splits into at
- * most 256 power-of-two slices of at most 2048 buckets have their selected
- * rows scattered (hashed keys) into one region per slice; one persistent
- * launch then copies each (query, slice) into LDS and probes the slice's
- * region from it (the probe of mgv_hashjoin_probe_lds), so the probes fetch
- * no random lines; rows past a full region are probed directly; the other
- * queries run as mgv_batch_run.
+/* LDS batch probe and build (same per-query results and table contents as
+ * mgv_batch_run): the PROBE_SUM queries (no MGV_PROBE_DIGEST, n >= min_rows)
+ * whose table splits into at most 256 power-of-two slices of at most 2048
+ * buckets, and the BUILD queries (not MGV_PAYLOAD_ROWID, n >= min_rows) whose
+ * table splits into at most 256 slices of at most 1024 buckets, have their
+ * selected rows scattered (hashed keys) into one region per slice; one
+ * persistent launch then copies each probed (query, slice) into LDS and
+ * probes the slice's region from it (the probe of mgv_hashjoin_probe_lds),
+ * another inserts each built slice's region in LDS (the build of
+ * mgv_hashjoin_build_lds), so neither fetches random lines; probe rows past
+ * a full region are probed directly, build rows past one (and rows whose
+ * chain leaves their slice) are inserted into the table afterwards; the
+ * other queries run as mgv_batch_run.  A batch's builds and the probes that
+ * read their tables go to two calls, builds first.
  * mgv_batch_plan_lds (host): validates as mgv_batch_plan; fills items
  * (capacity cap; *plan tells the count, also when items is NULL): scatter
- * tiles [0, nscatter), then one item per (query, slice) (.lo = slice), then
- * the other queries' vectors; regions (4 words per query: first region row,
- * rows per region, slices F (0: not LDS-probed), buckets per slice).
+ * tiles [0, nscatter), then one item per probed (query, slice) (.lo =
+ * slice), then one per built (query, slice), then the other queries'
+ * vectors; regions (4 words per query: first region row, rows per region,
+ * slices F (0: not through LDS), buckets per slice).
  * mgv_batch_run_lds: device copies of queries, items and regions; workspace
  * of plan->workspace_bytes (16-byte aligned), cleared by a fill kernel, so
  * the call can be captured into a graph.  launch.block / compact apply to
@@ -464,7 +471,9 @@ int mgv_batch_run_sliced(const mgv_query *queries, const mgv_work_item *items, c
 typedef struct mgv_lds_batch_plan {
     uint64_t nitems;             /* all items */
     uint64_t nscatter;           /* scatter tiles: items [0, nscatter) */
-    uint64_t nparts;             /* (query, slice) items: [nscatter, nscatter + nparts) */
+    uint64_t nparts;             /* probe (query, slice) items: [nscatter, nscatter + nparts) */
+    uint64_t nbuild_parts;       /* build (query, slice) items: the next nbuild_parts */
+    uint64_t build_rows;         /* rows of the LDS-built queries (their overflow list) */
     uint64_t region_rows;        /* rows of all regions */
     uint64_t workspace_bytes;
     uint32_t nqueries;

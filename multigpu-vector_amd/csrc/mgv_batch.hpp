@@ -29,6 +29,11 @@ struct BatchScatter {
     uint64_t nscatter;          // scatter items: up to `tile` rows of one partitioned query each
     uint32_t hashed;            // 1: the key column receives fmix32(key)
     uint32_t tile;              // kBatchTile or kBatchTileWide
+    // rows of BUILD queries past a full region: appended here and inserted
+    // into their tables after the slices are written back (null: no builds)
+    unsigned long long *ocount;
+    uint32_t *oquery;
+    int32_t *okey, *opay;
 };
 
 // Rows per region of a query of n rows in F regions: n / F plus eight
@@ -47,6 +52,9 @@ inline uint64_t batch_ws_bytes(uint32_t nq, uint64_t region_rows) {
     const uint64_t col = (region_rows * 4 + 255) / 256 * 256;
     return 256 + batch_fill_bytes(nq) + 2 * col;
 }
+
+// The LDS batch build's overflow list: query index, key and payload per row.
+inline uint64_t lds_batch_ovf_bytes(uint64_t build_rows) { return 3 * ((build_rows * 4 + 255) / 256 * 256); }
 
 // Launches the scatter (k_batch_scatter) of every scatter item.
 int launch_batch_scatter(const mgv_query *queries, const mgv_work_item *items, const BatchScatter &S, hipStream_t s);

@@ -251,7 +251,9 @@ __global__ __launch_bounds__(BLOCK) void k_batch_scatter(const mgv_query *__rest
 #pragma unroll
             for (int e = 0; e < 4; ++e) mm |= (uint32_t)(i0 + e < hi) << e;
             k[j] = load_quad(q.key, i0, hi);
-            v[j] = q.val ? load_quad(q.val, i0, hi) : v4i{0, 0, 0, 0};
+            // a BUILD's payload column only with MGV_PAYLOAD_COLUMN (NONE: zeros)
+            const bool payload = q.val && (q.kind != MGV_Q_BUILD || q.payload_mode == MGV_PAYLOAD_COLUMN);
+            v[j] = payload ? load_quad(q.val, i0, hi) : v4i{0, 0, 0, 0};
             if (q.pred) {
                 const v4i c = load_quad(q.pred, i0, hi);
 #pragma unroll
@@ -318,6 +320,11 @@ __global__ __launch_bounds__(BLOCK) void k_batch_scatter(const mgv_query *__rest
             if (j < lim[p]) {
                 R.rkey[dst[p] + j] = x;
                 R.rval[dst[p] + j] = val;
+            } else if (q.kind == MGV_Q_BUILD) {
+                const unsigned long long o = atomic_add_global(R.ocount, 1ull);
+                R.oquery[o] = wi.query;
+                R.okey[o] = (int32_t)(R.hashed ? fmix32_inv(h) : (uint32_t)x);
+                R.opay[o] = val;
             } else {
                 probe_row(q.slots, q.nbuckets, R.hashed ? fmix32_inv(h) : (uint32_t)x, val, ocnt, osum);
                 over = true;
@@ -599,7 +606,8 @@ int mgv_batch_run_sliced(const mgv_query *queries, const mgv_work_item *items, c
     // counters and fill counts: a fill kernel (the call can be captured into a graph)
     int rc = mgv_fill_u64((uint64_t *)ws, (256 + fills) / 8, 0, (mgv_stream_t)s);
     if (rc) return rc;
-    const BatchScatter S = {regions, R.fill, R.rkey, R.rval, R.ctr, plan->nscatter, 0u, (uint32_t)kBatchTile};
+    const BatchScatter S = {regions, R.fill, R.rkey, R.rval, R.ctr, plan->nscatter, 0u, (uint32_t)kBatchTile,
+                            nullptr, nullptr, nullptr, nullptr};
     if ((rc = launch_batch_scatter(queries, items, S, s))) return rc;
     if (plan->section[kSliceParts] > plan->section[0] && (rc = slice_probe(queries, items, R, L, s))) return rc;
     // section 8: the other queries' vectors, as mgv_batch_run

@@ -1514,12 +1514,109 @@ __global__ __launch_bounds__(kLdsBlock) void k_batch_lds_probe(LdsBatchArgs a) {
     }
 }
 
-// Partitions of a query's table for the LDS batch probe: the smallest power
-// of two F <= kBatchMaxParts that divides nbuckets into slices of at most
-// kSliceMax buckets (0: none).
-static uint32_t lds_batch_parts(uint32_t nb) {
+// The builds of an LDS batch: one (query, slice) item per block iteration
+// copies the slice (up to kBuildSlice buckets) into LDS, inserts the
+// region's rows with LDS compare-and-swap in slot order (as k_lds_build), and
+// writes the slice back; rows whose chain leaves the slice are appended to
+// the overflow list, which k_batch_build_overflow inserts afterwards.
+struct LdsBatchBuildArgs {
+    const mgv_query *Q;
+    const mgv_work_item *items;   // (query, slice) items of BUILD queries: .lo = slice
+    uint64_t nparts;
+    const uint64_t *regions;
+    const unsigned long long *fill;
+    const int32_t *rhash, *rval;
+    uint32_t *next;
+    unsigned long long *ocount;   // overflow list: {query, key, payload}
+    uint32_t *oquery;
+    int32_t *okey, *opay;
+};
+
+__global__ __launch_bounds__(kLdsBlock) void k_batch_lds_build(LdsBatchBuildArgs a) {
+    __shared__ __align__(16) unsigned long long tab[kBuildSlice * 8];
+    __shared__ uint32_t s_part;
+    const uint32_t t = threadIdx.x;
+    for (;;) {
+        if (t == 0) s_part = atomicAdd(a.next, 1u);
+        __syncthreads();
+        const uint32_t it = s_part;
+        __syncthreads();
+        if (it >= a.nparts) break;
+        const mgv_work_item wi = a.items[it];
+        const uint32_t qi = wi.query;
+        const mgv_query q = a.Q[qi];
+        const uint64_t *rg = a.regions + (uint64_t)kRegionWords * qi;
+        const uint64_t cap = rg[1];
+        const uint32_t slice = (uint32_t)rg[3];   // <= kBuildSlice
+        const uint32_t p = (uint32_t)wi.lo;
+        const uint64_t r0 = rg[0] + (uint64_t)p * cap;
+        const unsigned long long f = a.fill[(uint64_t)qi * kBatchMaxParts + p];
+        const uint64_t r1 = r0 + (f < cap ? f : cap);
+        if (r0 == r1) continue;   // an untouched slice is not copied
+        const uint32_t nb = q.nbuckets;
+        const uint32_t lo = p * slice;
+        v4u *g = reinterpret_cast<v4u *>(q.slots + (uint64_t)lo * 8);
+        v4u *l = reinterpret_cast<v4u *>(tab);
+        for (uint32_t i = t; i < slice * 4; i += kLdsBlock) l[i] = load_global(g + i);
+        __syncthreads();
+        uint64_t errs = 0;
+        // r0 is a multiple of 64 rows; the region runs on to its capacity
+        for (uint64_t i0 = r0 + 4ull * t; i0 < r1; i0 += 4ull * kLdsBlock) {
+            const v4i h4 = load_stream4(a.rhash + i0);
+            const v4i p4 = load_stream4(a.rval + i0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (i0 + j >= r1) break;
+                const uint32_t h = (uint32_t)q_get(h4, j), pay = (uint32_t)q_get(p4, j);
+                const uint32_t key = fmix32_inv(h);
+                const uint64_t val = ((uint64_t)pay << 32) | key;
+                if (val == kEmptySlot) {
+                    ++errs;
+                    continue;
+                }
+                const uint32_t idx = bucket_of_hash(h, nb) - lo;   // < slice: the region's rows
+                if (idx >= slice || !lds_insert(tab, slice, idx, val)) {
+                    const unsigned long long o = atomic_add_global(a.ocount, 1ull);
+                    a.oquery[o] = qi;
+                    a.okey[o] = (int32_t)key;
+                    a.opay[o] = (int32_t)pay;
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = t; i < slice * 4; i += kLdsBlock) g[i] = l[i];
+        errs = wave_sum_u64(errs);
+        if (lane_id() == 0 && errs && q.res)
+            atomic_add_global((unsigned long long *)&q.res->errors, (unsigned long long)errs);
+        __syncthreads();
+    }
+}
+
+// The overflow list of an LDS batch build, inserted into each row's table
+// after every slice is written back (chains continue into the next slices,
+// or wrap to bucket 0).
+__global__ __launch_bounds__(256) void k_batch_build_overflow(const mgv_query *Q, const unsigned long long *ocount,
+                                                              const uint32_t *oquery, const int32_t *okey,
+                                                              const int32_t *opay) {
+    const uint64_t n = *ocount;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const mgv_query q = Q[oquery[i]];
+        OpBuild<1> op;
+        op.slots = q.slots;
+        op.nbuckets = q.nbuckets;
+        op.res = nullptr;
+        op.errs = 0;
+        op.insert((uint32_t)okey[i], (uint32_t)opay[i]);
+        if (op.errs && q.res) atomic_add_global((unsigned long long *)&q.res->errors, (unsigned long long)op.errs);
+    }
+}
+
+// Partitions of a query's table for the LDS batch: the smallest power of two
+// F <= kBatchMaxParts that divides nbuckets into slices of at most smax
+// buckets (kSliceMax for a probe, kBuildSlice for a build; 0: none).
+static uint32_t lds_batch_parts(uint32_t nb, uint32_t smax) {
     for (uint32_t F = 1; F <= (uint32_t)kBatchMaxParts; F <<= 1)
-        if (nb % F == 0 && nb / F <= kSliceMax && nb / F > 0) return F;
+        if (nb % F == 0 && nb / F <= smax && nb / F > 0) return F;
     return 0;
 }
 
@@ -1753,8 +1850,11 @@ int mgv_batch_plan_lds(const mgv_query *queries, uint32_t nqueries, uint32_t vec
     int rc = mgv_batch_plan(queries, nqueries, vector_rows, nullptr, 0, &n);   // validates the queries
     if (rc) return rc;
     auto parts = [&](const mgv_query &q) -> uint32_t {
-        if (q.kind != MGV_Q_PROBE_SUM || (q.flags & MGV_PROBE_DIGEST) || q.n == 0 || q.n < min_rows) return 0;
-        return lds_batch_parts(q.nbuckets);
+        if (q.n == 0 || q.n < min_rows) return 0;
+        if (q.kind == MGV_Q_PROBE_SUM && !(q.flags & MGV_PROBE_DIGEST)) return lds_batch_parts(q.nbuckets, kSliceMax);
+        if (q.kind == MGV_Q_BUILD && q.payload_mode != MGV_PAYLOAD_ROWID)
+            return lds_batch_parts(q.nbuckets, kBuildSlice);
+        return 0;
     };
     uint64_t k = 0, rows = 0;
     auto put = [&](uint32_t qi, uint64_t lo, uint64_t r) {
@@ -1781,11 +1881,22 @@ int mgv_batch_plan_lds(const mgv_query *queries, uint32_t nqueries, uint32_t vec
             for (uint64_t lo = 0; lo < queries[qi].n; lo += kBatchTileWide)
                 put(qi, lo, queries[qi].n - lo < (uint64_t)kBatchTileWide ? queries[qi].n - lo : kBatchTileWide);
     plan->nscatter = k;
+    uint64_t build_rows = 0;
     for (uint32_t qi = 0; qi < nqueries; ++qi) {
         const uint32_t F = parts(queries[qi]);
-        for (uint32_t p = 0; p < F; ++p) put(qi, p, 0);
+        if (queries[qi].kind != MGV_Q_BUILD)
+            for (uint32_t p = 0; p < F; ++p) put(qi, p, 0);
     }
     plan->nparts = k - plan->nscatter;
+    for (uint32_t qi = 0; qi < nqueries; ++qi) {
+        const uint32_t F = parts(queries[qi]);
+        if (queries[qi].kind == MGV_Q_BUILD && F) {
+            for (uint32_t p = 0; p < F; ++p) put(qi, p, 0);
+            build_rows += queries[qi].n;
+        }
+    }
+    plan->nbuild_parts = k - plan->nscatter - plan->nparts;
+    plan->build_rows = build_rows;
     for (uint32_t qi = 0; qi < nqueries; ++qi)
         if (!parts(queries[qi]))
             for (uint64_t lo = 0; lo < queries[qi].n; lo += vector_rows)
@@ -1794,7 +1905,7 @@ int mgv_batch_plan_lds(const mgv_query *queries, uint32_t nqueries, uint32_t vec
     plan->nqueries = nqueries;
     plan->reserved = 0;
     plan->region_rows = rows;
-    plan->workspace_bytes = batch_ws_bytes(nqueries, rows);
+    plan->workspace_bytes = batch_ws_bytes(nqueries, rows) + lds_batch_ovf_bytes(build_rows);
     if (items && k > cap) return fail(MGV_E_CAPACITY, "mgv_batch_plan_lds: %llu items > capacity %llu",
                                       (unsigned long long)k, (unsigned long long)cap);
     return MGV_OK;
@@ -1805,9 +1916,9 @@ int mgv_batch_run_lds(const mgv_query *queries, const mgv_work_item *items, cons
     if (!current_state()) return fail(MGV_E_NOT_INIT, "mgv_init was not called");
     if (!queries || !plan || !regions || !ws || (plan->nitems && !items))
         return fail(MGV_E_INVALID, "mgv_batch_run_lds: null argument");
-    if (plan->nitems >= 0xffffffffull || plan->nscatter + plan->nparts > plan->nitems)
+    if (plan->nitems >= 0xffffffffull || plan->nscatter + plan->nparts + plan->nbuild_parts > plan->nitems)
         return fail(MGV_E_INVALID, "mgv_batch_run_lds: inconsistent plan");
-    const uint64_t need = batch_ws_bytes(plan->nqueries, plan->region_rows);
+    const uint64_t need = batch_ws_bytes(plan->nqueries, plan->region_rows) + lds_batch_ovf_bytes(plan->build_rows);
     if (ws_bytes < need) return fail(MGV_E_CAPACITY, "mgv_batch_run_lds: workspace %llu < %llu bytes",
                                      (unsigned long long)ws_bytes, (unsigned long long)need);
     if (!aligned16(ws)) return fail(MGV_E_ALIGN, "mgv_batch_run_lds: workspace not 16-byte aligned");
@@ -1816,13 +1927,22 @@ int mgv_batch_run_lds(const mgv_query *queries, const mgv_work_item *items, cons
     char *w = (char *)ws;
     const uint64_t fills = batch_fill_bytes(plan->nqueries);
     const uint64_t col = (plan->region_rows * 4 + 255) / 256 * 256;
-    uint32_t *ctr = (uint32_t *)w;   // [0] scatter claims, [1] partition claims, [8] the direct launch
+    // counters: [0] scatter claims, [1] probe partition claims, [2] build
+    // partition claims, 64-bit word 2 the overflow count, [8] the direct launch
+    uint32_t *ctr = (uint32_t *)w;
+    unsigned long long *ocount = (unsigned long long *)w + 2;
     unsigned long long *fill = (unsigned long long *)(w + 256);
     int32_t *rhash = (int32_t *)(w + 256 + fills), *rval = (int32_t *)(w + 256 + fills + col);
+    // the overflow list of the builds: query indices, keys, payloads
+    const uint64_t ob = (plan->build_rows * 4 + 255) / 256 * 256;
+    char *ov = w + batch_ws_bytes(plan->nqueries, plan->region_rows);
+    uint32_t *oquery = (uint32_t *)ov;
+    int32_t *okey = (int32_t *)(ov + ob), *opay = (int32_t *)(ov + 2 * ob);
     // counters and fill counts: a fill kernel (the call can be captured into a graph)
     int rc = mgv_fill_u64((uint64_t *)ws, (256 + fills) / 8, 0, (mgv_stream_t)s);
     if (rc) return rc;
-    const BatchScatter S = {regions, fill, rhash, rval, ctr, plan->nscatter, 1u, (uint32_t)kBatchTileWide};
+    const BatchScatter S = {regions, fill, rhash, rval, ctr, plan->nscatter, 1u, (uint32_t)kBatchTileWide,
+                            ocount, oquery, okey, opay};
     if ((rc = launch_batch_scatter(queries, items, S, s))) return rc;
     if (plan->nparts) {
         LdsBatchArgs a = {queries, items + plan->nscatter, plan->nparts, regions, fill, rhash, rval, ctr + 1};
@@ -1832,9 +1952,22 @@ int mgv_batch_run_lds(const mgv_query *queries, const mgv_work_item *items, cons
         hipLaunchKernelGGL(k_batch_lds_probe, dim3((uint32_t)grid), dim3(kLdsBlock), 0, s, a);
         if ((rc = check_launch("k_batch_lds_probe"))) return rc;
     }
-    const uint64_t nd = plan->nitems - plan->nscatter - plan->nparts;
+    if (plan->nbuild_parts) {
+        LdsBatchBuildArgs b = {queries, items + plan->nscatter + plan->nparts, plan->nbuild_parts, regions, fill,
+                               rhash, rval, ctr + 2, ocount, oquery, okey, opay};
+        static thread_local int cached = 0;
+        if (!cached) cached = persistent_blocks(k_batch_lds_build, kLdsBlock);
+        const uint64_t grid = plan->nbuild_parts < (uint64_t)cached ? plan->nbuild_parts : (uint64_t)cached;
+        hipLaunchKernelGGL(k_batch_lds_build, dim3((uint32_t)grid), dim3(kLdsBlock), 0, s, b);
+        if ((rc = check_launch("k_batch_lds_build"))) return rc;
+        hipLaunchKernelGGL(k_batch_build_overflow, dim3(4 * device_cus()), dim3(256), 0, s, queries, ocount, oquery,
+                           okey, opay);
+        if ((rc = check_launch("k_batch_build_overflow"))) return rc;
+    }
+    const uint64_t done = plan->nscatter + plan->nparts + plan->nbuild_parts;
+    const uint64_t nd = plan->nitems - done;
     if (!nd) return MGV_OK;
-    return mgv_batch_run(queries, items + plan->nscatter + plan->nparts, nd, ctr + 8, L, st);
+    return mgv_batch_run(queries, items + done, nd, ctr + 8, L, st);
 }
 
 }  // extern "C"

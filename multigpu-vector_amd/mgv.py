@@ -104,8 +104,9 @@ class WorkItem(C.Structure):
 class LdsBatchPlan(C.Structure):
     """mgv_lds_batch_plan: item ranges and workspace of an LDS batch probe."""
     _fields_ = [
-        ("nitems", C.c_uint64), ("nscatter", C.c_uint64), ("nparts", C.c_uint64), ("region_rows", C.c_uint64),
-        ("workspace_bytes", C.c_uint64), ("nqueries", C.c_uint32), ("reserved", C.c_uint32),
+        ("nitems", C.c_uint64), ("nscatter", C.c_uint64), ("nparts", C.c_uint64), ("nbuild_parts", C.c_uint64),
+        ("build_rows", C.c_uint64), ("region_rows", C.c_uint64), ("workspace_bytes", C.c_uint64),
+        ("nqueries", C.c_uint32), ("reserved", C.c_uint32),
     ]
 
 

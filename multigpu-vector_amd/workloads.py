@@ -315,19 +315,21 @@ class JoinBatch:
         return g
 
     # -------------------------------------------------------------- LDS batch path
-    def _ldsb(self, vector_rows: int, min_rows: int):
-        """Plan of the LDS batch probe (mgv_batch_plan_lds over the probe
-        queries): device queries, items, regions and workspace; cached."""
-        key = ("ldsb", vector_rows, min_rows)
+    def _ldsb(self, vector_rows: int, min_rows: int, builds: bool = False):
+        """Plan of the LDS batch (mgv_batch_plan_lds over the probe queries,
+        or over the build queries): device queries, items, regions and
+        workspace; cached."""
+        key = ("ldsb", vector_rows, min_rows, builds)
         if key in self._plans:
             return self._plans[key]
+        qarr = self.build_q if builds else self.probe_q
         plan = mgv.LdsBatchPlan()
         regions = (C.c_uint64 * (4 * max(1, self.nq)))()
-        mgv.call("mgv_batch_plan_lds", self.probe_q, self.nq, vector_rows, min_rows, None, 0, regions, C.byref(plan))
+        mgv.call("mgv_batch_plan_lds", qarr, self.nq, vector_rows, min_rows, None, 0, regions, C.byref(plan))
         items = (mgv.WorkItem * max(1, plan.nitems))()
-        mgv.call("mgv_batch_plan_lds", self.probe_q, self.nq, vector_rows, min_rows, items, plan.nitems, regions,
+        mgv.call("mgv_batch_plan_lds", qarr, self.nq, vector_rows, min_rows, items, plan.nitems, regions,
                  C.byref(plan))
-        dq = mgv.DeviceBuffer(C.sizeof(self.probe_q)).upload(np.frombuffer(bytes(self.probe_q), dtype=np.uint8))
+        dq = mgv.DeviceBuffer(C.sizeof(qarr)).upload(np.frombuffer(bytes(qarr), dtype=np.uint8))
         di = mgv.DeviceBuffer(max(16, C.sizeof(items))).upload(np.frombuffer(bytes(items), dtype=np.uint8))
         dr = mgv.DeviceBuffer(C.sizeof(regions)).upload(np.frombuffer(bytes(regions), dtype=np.uint8))
         ws = mgv.DeviceBuffer(max(256, plan.workspace_bytes))
@@ -336,23 +338,33 @@ class JoinBatch:
         self._plans[key] = out
         return out
 
-    def issue_ldsb(self, vector_rows: int, launch: mgv.Launch, stream=None, min_rows: int = 1 << 16):
-        """clear + the fused build launch + the LDS batch probe
-        (mgv_batch_run_lds: probes of at least min_rows rows scattered by
-        table slice, each (query, slice) probed from LDS)."""
+    def issue_ldsb(self, vector_rows: int, launch: mgv.Launch, stream=None, min_rows: int = 1 << 16,
+                   lds_builds: bool = True):
+        """clear + the builds (through the LDS batch when lds_builds, else
+        the fused build launch) + the LDS batch probe (mgv_batch_run_lds:
+        probes of at least min_rows rows scattered by table slice, each
+        (query, slice) probed from LDS)."""
         s = stream if stream is not None else self.st
         (bq, bi, bn), _ = self._plan(vector_rows)
         lp = self._ldsb(vector_rows, min_rows)
         self.clear(s)
-        mgv.call("mgv_batch_run", bq.vp, bi.vp, bn, self.counter.vp, launch, s)
+        if lds_builds:
+            lb = self._ldsb(vector_rows, min_rows, builds=True)
+            mgv.call("mgv_batch_run_lds", lb["dq"].vp, lb["di"].vp, lb["dr"].vp, C.byref(lb["plan"]), lb["ws"].vp,
+                     lb["ws"].nbytes, launch, s)
+        else:
+            mgv.call("mgv_batch_run", bq.vp, bi.vp, bn, self.counter.vp, launch, s)
         mgv.call("mgv_batch_run_lds", lp["dq"].vp, lp["di"].vp, lp["dr"].vp, C.byref(lp["plan"]), lp["ws"].vp,
                  lp["ws"].nbytes, launch, s)
 
-    def capture_ldsb(self, vector_rows: int, launch: mgv.Launch, stream, min_rows: int = 1 << 16) -> C.c_void_p:
+    def capture_ldsb(self, vector_rows: int, launch: mgv.Launch, stream, min_rows: int = 1 << 16,
+                     lds_builds: bool = True) -> C.c_void_p:
         self._plan(vector_rows)
         self._ldsb(vector_rows, min_rows)
+        if lds_builds:
+            self._ldsb(vector_rows, min_rows, builds=True)
         mgv.call("mgv_graph_begin_capture", stream)
-        self.issue_ldsb(vector_rows, launch, stream, min_rows)
+        self.issue_ldsb(vector_rows, launch, stream, min_rows, lds_builds)
         g = C.c_void_p()
         mgv.call("mgv_graph_end_capture", stream, C.byref(g))
         return g

@@ -50,7 +50,10 @@ def upload(arr):
     return mgv.DeviceBuffer(max(16, C.sizeof(arr))).upload(np.frombuffer(bytes(arr), dtype=np.uint8))
 
 
-def build_tables(gpu, qs, lf=0.5):
+def build_tables(gpu, qs, lf=0.5, lds_min_rows=None):
+    """The queries' tables, built by one fused launch, or (lds_min_rows set)
+    by the LDS batch build (builds of at least lds_min_rows rows whose table
+    splits into 1024-bucket slices; the others through the fused launch)."""
     nq = len(qs)
     tables = [mgv.HashTable(q["B"], lf) for q in qs]
     res = mgv.ResultBuffer(2 * nq)   # [0, nq): builds, [nq, 2nq): probes
@@ -62,13 +65,25 @@ def build_tables(gpu, qs, lf=0.5):
                                None, None, res.ptr + 64 * i, mgv.MGV_PAYLOAD_COLUMN, 0, 0)
         probe_q[i] = mgv.Query(mgv.MGV_Q_PROBE_SUM, q["bound"], d["c"].ptr, d["rk"].ptr, d["rv"].ptr, q["P"],
                                t.buf.ptr, t.nbuckets, 0, None, None, res.ptr + 64 * (nq + i), 0, 0, 0)
-    n = C.c_uint64()
-    gpu.call("mgv_batch_plan", build_q, nq, 1024, None, 0, C.byref(n))
-    items = (mgv.WorkItem * n.value)()
-    gpu.call("mgv_batch_plan", build_q, nq, 1024, items, n.value, C.byref(n))
-    dq, di = upload(build_q), upload(items)
-    counter = mgv.DeviceBuffer(64)
-    gpu.call("mgv_batch_run", dq.vp, di.vp, n.value, counter.vp, mgv.launch(0, 256, 1), None)
+    if lds_min_rows is None:
+        n = C.c_uint64()
+        gpu.call("mgv_batch_plan", build_q, nq, 1024, None, 0, C.byref(n))
+        items = (mgv.WorkItem * n.value)()
+        gpu.call("mgv_batch_plan", build_q, nq, 1024, items, n.value, C.byref(n))
+        dq, di = upload(build_q), upload(items)
+        counter = mgv.DeviceBuffer(64)
+        gpu.call("mgv_batch_run", dq.vp, di.vp, n.value, counter.vp, mgv.launch(0, 256, 1), None)
+    else:
+        plan = mgv.LdsBatchPlan()
+        regions = (C.c_uint64 * (4 * nq))()
+        gpu.call("mgv_batch_plan_lds", build_q, nq, 1024, lds_min_rows, None, 0, regions, C.byref(plan))
+        items = (mgv.WorkItem * max(1, plan.nitems))()
+        gpu.call("mgv_batch_plan_lds", build_q, nq, 1024, lds_min_rows, items, plan.nitems, regions, C.byref(plan))
+        assert plan.nparts == 0
+        dq, di, dr = upload(build_q), upload(items), upload(regions)
+        ws = mgv.DeviceBuffer(max(256, plan.workspace_bytes))
+        gpu.call("mgv_batch_run_lds", dq.vp, di.vp, dr.vp, C.byref(plan), ws.vp, ws.nbytes, mgv.launch(0, 256, 1),
+                 None)
     mgv.sync()
     assert all(res.read(i).errors == 0 for i in range(nq))
     return tables, res, probe_q
@@ -198,9 +213,9 @@ def test_sliced_batch_rejects_bad_arguments(gpu, orc):
 
 # ---------------------------------------------------------------- LDS batch probe
 
-def run_ldsb(gpu, qs, vector_rows, min_rows, block=256, compact=1, lf=0.5):
+def run_ldsb(gpu, qs, vector_rows, min_rows, block=256, compact=1, lf=0.5, lds_build=True):
     nq = len(qs)
-    tables, res, probe_q = build_tables(gpu, qs, lf)
+    tables, res, probe_q = build_tables(gpu, qs, lf, min_rows if lds_build else None)
     plan = mgv.LdsBatchPlan()
     regions = (C.c_uint64 * (4 * nq))()
     gpu.call("mgv_batch_plan_lds", probe_q, nq, vector_rows, min_rows, None, 0, regions, C.byref(plan))
@@ -241,6 +256,22 @@ def test_lds_batch_skewed_keys_and_duplicate_build_keys(gpu, orc):
     assert len(lds) == len(qs)
     for q, r in zip(qs, got):
         assert (r.count, r.sum) == (q["want"]["count"], q["want"]["sum"]), (q["P"], q["bound"])
+
+
+@pytest.mark.parametrize("lf", [0.5, 0.75])
+def test_lds_batch_build_equals_fused_build(gpu, orc, lf):
+    """Tables built by the LDS batch build hold exactly the slot multiset of
+    the fused build's tables and fill every bucket's slots in order (unique
+    and duplicate build keys)."""
+    qs = make_queries(orc, 10, seed_cfg=17, lo_log=18.1, hi_log=20, dup=(2, 7))
+    a, _, _ = build_tables(gpu, qs, lf)
+    b, _, _ = build_tables(gpu, qs, lf, lds_min_rows=0)
+    empty = np.uint64(0xFFFFFFFFFFFFFFFF)
+    for ta, tb in zip(a, b):
+        sa, sb = ta.slots(), tb.slots()
+        assert np.array_equal(np.sort(sa[sa != empty]), np.sort(sb[sb != empty]))
+        occ = (sb != empty).reshape(tb.nbuckets, 8)
+        assert np.array_equal(occ, np.arange(8)[None, :] < occ.sum(axis=1)[:, None])
 
 
 def test_lds_batch_equals_fused_on_graph_replays(gpu, orc):
