@@ -854,9 +854,10 @@ def leg_c4(args, st):
     # (query, slice) probed from LDS (mgv_batch_run_lds), in one graph; the
     # slices hold half as many buckets at load factor 0.5, so it is timed at
     # the tuned load factor and at 0.5 (the tables are rebuilt in the graph)
-    lmin = 1 << 14
+    tl = t.get("lds_batch", {})
+    lmin = int(tl.get("min_rows", 1 << 14))
     ldsb = {}
-    for lfl in sorted({lf, 0.5}):
+    for lfl in sorted({lf, float(tl.get("load_factor", 0.5))}):
         if batch.lf != lfl:
             batch.set_load_factor(lfl)
         gl = batch.capture_ldsb(V, L, st.vp, min_rows=lmin)

@@ -125,12 +125,29 @@ def tune_c4(st, quick=False, nq=256):
                      {"route_fanout": fan, "route_threshold": thr, "route_streams": nst, "routed_queries": nbig,
                       "ms": ms, "digest": d})
         log(f"c4 routed fan-out {fan} ({nbig} queries): {ms:.3f} ms")
+    # the LDS batch (builds and probes of at least min_rows rows through LDS
+    # slices) at the best launch, per load factor and threshold
+    lds_table = []
+    for lf in lfs:
+        batch.set_load_factor(lf)
+        for mr in ((1 << 14,) if quick else (1 << 12, 1 << 14, 1 << 16)):
+            g = batch.capture_ldsb(best["vector_rows"], L, st.vp, min_rows=mr)
+            ms = time_ms(lambda: mgv.call("mgv_graph_launch", g, st.vp), st)
+            mgv.call("mgv_graph_destroy", g)
+            r = batch.results()
+            d = digest(r["count"], r["sum"])
+            ref = check_same(ref, d, f"c4 LDS batch lf={lf} min_rows={mr}")
+            lds_table.append({"load_factor": lf, "min_rows": mr, "ms": ms, "digest": d})
+            log(f"c4 LDS batch lf={lf} min_rows={mr}: {ms:.3f} ms")
     batch.close()
     best = min(table, key=lambda x: x["ms"])
     out = {k: best[k] for k in ("vector_rows", "block", "compact", "load_factor", "route_fanout")}
     if best["route_fanout"] > 1:
         out |= {"route_threshold": best["route_threshold"], "route_streams": best["route_streams"]}
-    return out | {"ms": best["ms"], "queries": nq, "candidates": table}
+    lb = min(lds_table, key=lambda x: x["ms"])
+    out["lds_batch"] = {"load_factor": lb["load_factor"], "min_rows": lb["min_rows"], "ms": lb["ms"]}
+    out["path"] = "lds_batch" if lb["ms"] < best["ms"] else ("routed" if best["route_fanout"] > 1 else "fused")
+    return out | {"ms": min(best["ms"], lb["ms"]), "queries": nq, "candidates": table + lds_table}
 
 
 def tune_c2(st, quick=False, ns=1 << 24, nr=1 << 28):

@@ -18,12 +18,17 @@ def test_autotune_quick_spaces_keep_results_identical(gpu):
     st = mgv.Stream()
     c4 = autotune.tune_c4(st, quick=True, nq=24)
     cands = c4["candidates"]
-    assert len({c["digest"] for c in cands}) == 1
-    assert {c["block"] for c in cands} == {64, 128, 256, 512, 1024}
-    routed = [c for c in cands if c["route_fanout"] > 1]
+    assert len({c["digest"] for c in cands}) == 1   # fused, routed and LDS-batch candidates alike
+    launches = [c for c in cands if "block" in c]
+    lds = [c for c in cands if "min_rows" in c]
+    assert lds and len(launches) + len(lds) == len(cands)
+    assert {c["block"] for c in launches} == {64, 128, 256, 512, 1024}
+    routed = [c for c in launches if c["route_fanout"] > 1]
     assert routed and all(c["routed_queries"] > 0 for c in routed)
-    assert c4["route_fanout"] in {c["route_fanout"] for c in cands}
+    assert c4["route_fanout"] in {c["route_fanout"] for c in launches}
     assert c4["ms"] == min(c["ms"] for c in cands)
+    assert c4["path"] in ("fused", "routed", "lds_batch")
+    assert c4["lds_batch"]["ms"] == min(c["ms"] for c in lds)
 
     c2 = autotune.tune_c2(st, quick=True, ns=1 << 20, nr=1 << 22)
     assert len({c["digest"] for c in c2["candidates"]}) == 1
