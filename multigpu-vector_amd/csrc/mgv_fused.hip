@@ -194,7 +194,7 @@ static int fused_c(const mgv_query *Q, const mgv_work_item *items, uint64_t nite
 // 208 G/s instead of 67 (measured).  The sliced batch probe therefore
 // scatters the selected rows of each large PROBE_SUM query by the top bits
 // of fmix32(key) -- the bits that pick the bucket's 8th, 16th or 32nd of
-// the table -- into 8, 16 or 32 regions (k_slice_scatter); then one
+// the table -- into 8, 16 or 32 regions (k_batch_scatter); then one
 // persistent launch (k_slice_probe) lets the waves of XCD x take the regions
 // r = x mod 8 of every query first, region after region, then whatever is
 // left of the other XCDs' regions, and the other queries' vectors follow in
