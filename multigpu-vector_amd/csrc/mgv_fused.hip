@@ -231,19 +231,14 @@ __global__ __launch_bounds__(BLOCK) void k_batch_scatter(const mgv_query *__rest
     __shared__ uint32_t wtot[BLOCK / kWave];
     __shared__ uint32_t s_item;
     const uint32_t t = threadIdx.x;
-    for (;;) {
-        if (t == 0) s_item = atomicAdd(&R.ctr[0], 1u);
-        for (uint32_t p = t; p < (uint32_t)kBatchMaxParts; p += BLOCK) cnt[p] = 0;
-        __syncthreads();
-        const uint32_t it = s_item;
-        if (it >= R.nscatter) break;
-        const mgv_work_item wi = items[it];
-        const mgv_query q = Q[wi.query];
-        const uint64_t base = R.regions[kRegionWords * wi.query], cap = R.regions[kRegionWords * wi.query + 1];
-        const uint32_t F = (uint32_t)R.regions[kRegionWords * wi.query + 2];
+    // a tile's rows: key and value quads, and the mask of the rows in range
+    // that pass the filter
+    v4i k[NQ], v[NQ];
+    uint32_t m[NQ];
+    auto load_tile = [&](const mgv_work_item &wi, const mgv_query &q) {
         const uint64_t hi = (wi.lo + wi.rows < q.n) ? wi.lo + wi.rows : q.n;
-        v4i k[NQ], v[NQ];
-        uint32_t m[NQ];
+        // a BUILD's payload column only with MGV_PAYLOAD_COLUMN (NONE: zeros)
+        const bool payload = q.val && (q.kind != MGV_Q_BUILD || q.payload_mode == MGV_PAYLOAD_COLUMN);
 #pragma unroll
         for (int j = 0; j < NQ; ++j) {
             const uint64_t i0 = wi.lo + 4ull * t + (uint64_t)j * 4 * BLOCK;
@@ -251,8 +246,6 @@ __global__ __launch_bounds__(BLOCK) void k_batch_scatter(const mgv_query *__rest
 #pragma unroll
             for (int e = 0; e < 4; ++e) mm |= (uint32_t)(i0 + e < hi) << e;
             k[j] = load_quad(q.key, i0, hi);
-            // a BUILD's payload column only with MGV_PAYLOAD_COLUMN (NONE: zeros)
-            const bool payload = q.val && (q.kind != MGV_Q_BUILD || q.payload_mode == MGV_PAYLOAD_COLUMN);
             v[j] = payload ? load_quad(q.val, i0, hi) : v4i{0, 0, 0, 0};
             if (q.pred) {
                 const v4i c = load_quad(q.pred, i0, hi);
@@ -261,6 +254,25 @@ __global__ __launch_bounds__(BLOCK) void k_batch_scatter(const mgv_query *__rest
             }
             m[j] = mm;
         }
+    };
+    if (t == 0) s_item = atomicAdd(&R.ctr[0], 1u);
+    for (uint32_t p = t; p < (uint32_t)kBatchMaxParts; p += BLOCK) cnt[p] = 0;
+    __syncthreads();
+    uint32_t it = s_item;
+    mgv_work_item wi = {};
+    mgv_query q = {};
+    if (it < R.nscatter) {
+        wi = items[it];
+        q = Q[wi.query];
+        load_tile(wi, q);
+    }
+    // the next tile's rows load while this tile's runs are written
+    while (it < R.nscatter) {
+        const uint64_t base = R.regions[kRegionWords * wi.query], cap = R.regions[kRegionWords * wi.query + 1];
+        const uint32_t F = (uint32_t)R.regions[kRegionWords * wi.query + 2];
+        const uint32_t qi = wi.query;
+        __syncthreads();   // every thread has read s_item
+        if (t == 0) s_item = atomicAdd(&R.ctr[0], 1u);   // the next tile
         uint32_t rank[NQ][4];
 #pragma unroll
         for (int j = 0; j < NQ; ++j)
@@ -288,7 +300,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_scatter(const mgv_query *__rest
                 uint32_t fit = 0;
                 if (c[e]) {
                     const unsigned long long off = atomic_add_global(
-                        &R.fill[(uint64_t)wi.query * kBatchMaxParts + p], (unsigned long long)c[e]);
+                        &R.fill[(uint64_t)qi * kBatchMaxParts + p], (unsigned long long)c[e]);
                     const uint64_t room = off < cap ? cap - off : 0;
                     fit = room < c[e] ? (uint32_t)room : c[e];
                     dst[p] = base + (uint64_t)p * cap + off;
@@ -298,6 +310,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch_scatter(const mgv_query *__rest
             }
         }
         __syncthreads();
+        const uint32_t nit = s_item;
+        for (uint32_t p = t; p < (uint32_t)kBatchMaxParts; p += BLOCK) cnt[p] = 0;   // read before the barrier above
 #pragma unroll
         for (int j = 0; j < NQ; ++j)
 #pragma unroll
@@ -308,6 +322,12 @@ __global__ __launch_bounds__(BLOCK) void k_batch_scatter(const mgv_query *__rest
                     sk[pos] = R.hashed ? (int32_t)h : q_get(k[j], e);
                     sv[pos] = q_get(v[j], e);
                 }
+        const mgv_query qc = q;
+        if (nit < R.nscatter) {
+            wi = items[nit];
+            q = Q[wi.query];
+            load_tile(wi, q);
+        }
         __syncthreads();
         const uint32_t total = start[kBatchMaxParts];
         uint64_t ocnt = 0, osum = 0;
@@ -320,24 +340,24 @@ __global__ __launch_bounds__(BLOCK) void k_batch_scatter(const mgv_query *__rest
             if (j < lim[p]) {
                 R.rkey[dst[p] + j] = x;
                 R.rval[dst[p] + j] = val;
-            } else if (q.kind == MGV_Q_BUILD) {
+            } else if (qc.kind == MGV_Q_BUILD) {
                 const unsigned long long o = atomic_add_global(R.ocount, 1ull);
-                R.oquery[o] = wi.query;
+                R.oquery[o] = qi;
                 R.okey[o] = (int32_t)(R.hashed ? fmix32_inv(h) : (uint32_t)x);
                 R.opay[o] = val;
             } else {
-                probe_row(q.slots, q.nbuckets, R.hashed ? fmix32_inv(h) : (uint32_t)x, val, ocnt, osum);
+                probe_row(qc.slots, qc.nbuckets, R.hashed ? fmix32_inv(h) : (uint32_t)x, val, ocnt, osum);
                 over = true;
             }
         }
         if (ballot(over)) {
             const uint64_t a = wave_sum_u64(ocnt), b2 = wave_sum_u64(osum);
             if (lane_id() == 0 && a) {
-                atomic_add_global((unsigned long long *)&q.res->count, (unsigned long long)a);
-                atomic_add_global((unsigned long long *)&q.res->sum, (unsigned long long)b2);
+                atomic_add_global((unsigned long long *)&qc.res->count, (unsigned long long)a);
+                atomic_add_global((unsigned long long *)&qc.res->sum, (unsigned long long)b2);
             }
         }
-        __syncthreads();
+        it = nit;
     }
 }
 
