@@ -856,11 +856,12 @@ def leg_c4(args, st):
     # the tuned load factor and at 0.5 (the tables are rebuilt in the graph)
     tl = t.get("lds_batch", {})
     lmin = int(tl.get("min_rows", 1 << 14))
+    lV = int(tl.get("vector_rows", 1024))   # vector rows of the direct (small) queries
     ldsb = {}
     for lfl in sorted({lf, float(tl.get("load_factor", 0.5))}):
         if batch.lf != lfl:
             batch.set_load_factor(lfl)
-        gl = batch.capture_ldsb(V, L, st.vp, min_rows=lmin)
+        gl = batch.capture_ldsb(lV, L, st.vp, min_rows=lmin)
 
         def run_ldsb_graph():
             mgv.call("mgv_graph_launch", gl, st.vp)
@@ -870,8 +871,8 @@ def leg_c4(args, st):
         r = batch.results()
         same = bool(np.array_equal(r["count"], fused["count"]) and np.array_equal(r["sum"], fused["sum"]))
         mgv.call("mgv_graph_destroy", gl)
-        ldsb[lfl] = (lw, le, same, batch._ldsb(V, lmin)["queries"])
-        log(f"C4 LDS batch at load factor {lfl} ({ldsb[lfl][3]} queries >= {lmin} rows): {min(lw) * 1e3:.2f} ms "
+        ldsb[lfl] = (lw, le, same, batch._ldsb(lV, lmin)["queries"])
+        log(f"C4 LDS batch at load factor {lfl} ({ldsb[lfl][3]} queries >= {lmin} rows, V={lV}): {min(lw) * 1e3:.2f} ms "
             f"wall, equal to fused: {same}")
     if batch.lf != lf:
         batch.set_load_factor(lf)
@@ -925,7 +926,7 @@ def leg_c4(args, st):
         "sliced": {"wall_ms": min(sw_) * 1e3, "gpu_ms": min(se), "queries_per_s": qps["sliced"],
                    "sliced_queries": nsl, "min_rows": smin, "regions": sreg},
         "lds_batch": {"wall_ms": min(lw) * 1e3, "gpu_ms": min(le), "queries_per_s": qps["lds_batch"],
-                      "lds_queries": nlq, "min_rows": lmin, "load_factor": ldsb_lf,
+                      "lds_queries": nlq, "min_rows": lmin, "vector_rows": lV, "load_factor": ldsb_lf,
                       "wall_ms_by_load_factor": {str(k): min(v[0]) * 1e3 for k, v in ldsb.items()}},
         "plain": plain_out,
         "speedup_vs_best_plain": qps[best_path] / best_plain["queries_per_s"] if best_plain else None,

@@ -8,7 +8,9 @@ Workloads and search spaces:
       those, the partition fan-out of the queries' probes: 1 (every probe in
       the fused kernel) or the large probes (>= 2^20 rows) routed through
       the LDS probe with level-1 fan-out {16, 64, 256} on side streams of
-      the same graph
+      the same graph; and the LDS batch (builds and probes of at least
+      {2^12, 2^14, 2^16} rows through LDS slices, the other queries fused at
+      vector size {1K, the best}) at each load factor
   c2  2^28-row probe of a 2^24-key table (load factor 0.5): the direct probe
       (workgroup {64, 256, 1024} x {per-lane, 4-lane cooperative}), the
       partitioned probe (fan-out {32..256} x scatter shape) and the LDS probe
@@ -126,26 +128,29 @@ def tune_c4(st, quick=False, nq=256):
                       "ms": ms, "digest": d})
         log(f"c4 routed fan-out {fan} ({nbig} queries): {ms:.3f} ms")
     # the LDS batch (builds and probes of at least min_rows rows through LDS
-    # slices) at the best launch, per load factor and threshold
+    # slices) at the best launch, per load factor, threshold and vector size
+    # of the other (direct) queries: their few rows per query leave a fused
+    # launch of large vectors latency-bound
     lds_table = []
+    lds_Vs = sorted({1024, best["vector_rows"]})
     for lf in lfs:
         batch.set_load_factor(lf)
-        for mr in ((1 << 14,) if quick else (1 << 12, 1 << 14, 1 << 16)):
-            g = batch.capture_ldsb(best["vector_rows"], L, st.vp, min_rows=mr)
+        for mr, lv in itertools.product((1 << 14,) if quick else (1 << 12, 1 << 14, 1 << 16), lds_Vs):
+            g = batch.capture_ldsb(lv, L, st.vp, min_rows=mr)
             ms = time_ms(lambda: mgv.call("mgv_graph_launch", g, st.vp), st)
             mgv.call("mgv_graph_destroy", g)
             r = batch.results()
             d = digest(r["count"], r["sum"])
-            ref = check_same(ref, d, f"c4 LDS batch lf={lf} min_rows={mr}")
-            lds_table.append({"load_factor": lf, "min_rows": mr, "ms": ms, "digest": d})
-            log(f"c4 LDS batch lf={lf} min_rows={mr}: {ms:.3f} ms")
+            ref = check_same(ref, d, f"c4 LDS batch lf={lf} min_rows={mr} V={lv}")
+            lds_table.append({"load_factor": lf, "min_rows": mr, "vector_rows": lv, "ms": ms, "digest": d})
+            log(f"c4 LDS batch lf={lf} min_rows={mr} V={lv}: {ms:.3f} ms")
     batch.close()
     best = min(table, key=lambda x: x["ms"])
     out = {k: best[k] for k in ("vector_rows", "block", "compact", "load_factor", "route_fanout")}
     if best["route_fanout"] > 1:
         out |= {"route_threshold": best["route_threshold"], "route_streams": best["route_streams"]}
     lb = min(lds_table, key=lambda x: x["ms"])
-    out["lds_batch"] = {"load_factor": lb["load_factor"], "min_rows": lb["min_rows"], "ms": lb["ms"]}
+    out["lds_batch"] = {k: lb[k] for k in ("load_factor", "min_rows", "vector_rows", "ms")}
     out["path"] = "lds_batch" if lb["ms"] < best["ms"] else ("routed" if best["route_fanout"] > 1 else "fused")
     return out | {"ms": min(best["ms"], lb["ms"]), "queries": nq, "candidates": table + lds_table}
 

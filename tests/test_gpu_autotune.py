@@ -29,6 +29,7 @@ def test_autotune_quick_spaces_keep_results_identical(gpu):
     assert c4["ms"] == min(c["ms"] for c in cands)
     assert c4["path"] in ("fused", "routed", "lds_batch")
     assert c4["lds_batch"]["ms"] == min(c["ms"] for c in lds)
+    assert 1024 in {c["vector_rows"] for c in lds}   # the direct queries' own vector size
 
     c2 = autotune.tune_c2(st, quick=True, ns=1 << 20, nr=1 << 22)
     assert len({c["digest"] for c in c2["candidates"]}) == 1
