@@ -461,8 +461,10 @@ int mgv_batch_run_sliced(const mgv_query *queries, const mgv_work_item *items, c
  * (capacity cap; *plan tells the count, also when items is NULL): scatter
  * tiles [0, nscatter), then one item per probed (query, slice) (.lo =
  * slice), then one per built (query, slice), then the other queries'
- * vectors; regions (4 words per query: first region row, rows per region,
- * slices F (0: not through LDS), buckets per slice).
+ * vectors of vector_rows rows (those queries are the small ones: short
+ * vectors, such as 1024 rows, spread them over more waves); regions (4 words
+ * per query: first region row, rows per region, slices F (0: not through
+ * LDS), buckets per slice).
  * mgv_batch_run_lds: device copies of queries, items and regions; workspace
  * of plan->workspace_bytes (16-byte aligned), cleared by a fill kernel, so
  * the call can be captured into a graph.  launch.block / compact apply to
