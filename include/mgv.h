@@ -142,6 +142,12 @@ typedef struct mgv_launch {
  * by default for power-of-two tables of at least 2^18 buckets (same
  * results; kept for comparison). */
 #define MGV_LAUNCH_KEY_SLICES (1u << 13)
+/* Launch flag of mgv_batch_run_lds: the tables of the queries it builds
+ * through LDS slices (regions word F != 0) are written whole, every slice
+ * from an empty start, whatever they held: they need no clear before the
+ * call.  The tables of its other build queries are inserted into as they
+ * stand (clear them first, as for mgv_batch_run). */
+#define MGV_LAUNCH_FRESH_TABLES (1u << 14)
 
 /* ------------------------------------------------------------ datagen */
 

@@ -1518,7 +1518,9 @@ __global__ __launch_bounds__(kLdsBlock) void k_batch_lds_probe(LdsBatchArgs a) {
 // copies the slice (up to kBuildSlice buckets) into LDS, inserts the
 // region's rows with LDS compare-and-swap in slot order (as k_lds_build), and
 // writes the slice back; rows whose chain leaves the slice are appended to
-// the overflow list, which k_batch_build_overflow inserts afterwards.
+// the overflow list, which k_batch_build_overflow inserts afterwards.  With
+// fresh set every slice starts empty instead of being read, and every slice
+// is written, rows or not: the tables need no clear before the call.
 struct LdsBatchBuildArgs {
     const mgv_query *Q;
     const mgv_work_item *items;   // (query, slice) items of BUILD queries: .lo = slice
@@ -1530,6 +1532,7 @@ struct LdsBatchBuildArgs {
     unsigned long long *ocount;   // overflow list: {query, key, payload}
     uint32_t *oquery;
     int32_t *okey, *opay;
+    uint32_t fresh;
 };
 
 __global__ __launch_bounds__(kLdsBlock) void k_batch_lds_build(LdsBatchBuildArgs a) {
@@ -1552,12 +1555,17 @@ __global__ __launch_bounds__(kLdsBlock) void k_batch_lds_build(LdsBatchBuildArgs
         const uint64_t r0 = rg[0] + (uint64_t)p * cap;
         const unsigned long long f = a.fill[(uint64_t)qi * kBatchMaxParts + p];
         const uint64_t r1 = r0 + (f < cap ? f : cap);
-        if (r0 == r1) continue;   // an untouched slice is not copied
+        if (r0 == r1 && !a.fresh) continue;   // an untouched slice is not copied
         const uint32_t nb = q.nbuckets;
         const uint32_t lo = p * slice;
         v4u *g = reinterpret_cast<v4u *>(q.slots + (uint64_t)lo * 8);
         v4u *l = reinterpret_cast<v4u *>(tab);
-        for (uint32_t i = t; i < slice * 4; i += kLdsBlock) l[i] = load_global(g + i);
+        if (a.fresh) {
+            const v4u e = {~0u, ~0u, ~0u, ~0u};   // kEmptySlot in every slot
+            for (uint32_t i = t; i < slice * 4; i += kLdsBlock) l[i] = e;
+        } else {
+            for (uint32_t i = t; i < slice * 4; i += kLdsBlock) l[i] = load_global(g + i);
+        }
         __syncthreads();
         uint64_t errs = 0;
         // r0 is a multiple of 64 rows; the region runs on to its capacity
@@ -1954,7 +1962,8 @@ int mgv_batch_run_lds(const mgv_query *queries, const mgv_work_item *items, cons
     }
     if (plan->nbuild_parts) {
         LdsBatchBuildArgs b = {queries, items + plan->nscatter + plan->nparts, plan->nbuild_parts, regions, fill,
-                               rhash, rval, ctr + 2, ocount, oquery, okey, opay};
+                               rhash, rval, ctr + 2, ocount, oquery, okey, opay,
+                               (L.reserved & MGV_LAUNCH_FRESH_TABLES) ? 1u : 0u};
         static thread_local int cached = 0;
         if (!cached) cached = persistent_blocks(k_batch_lds_build, kLdsBlock);
         const uint64_t grid = plan->nbuild_parts < (uint64_t)cached ? plan->nbuild_parts : (uint64_t)cached;

@@ -19,6 +19,7 @@ MGV_PAYLOAD_NONE = 0
 MGV_PAYLOAD_COLUMN = 1
 MGV_PAYLOAD_ROWID = 2
 MGV_PROBE_DIGEST = 1
+MGV_LAUNCH_FRESH_TABLES = 1 << 14   # mgv_batch_run_lds: LDS-built tables written whole, no clear needed
 SLOTS_PER_BUCKET = 8
 
 E_INVALID = 10001

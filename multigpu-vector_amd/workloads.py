@@ -96,9 +96,11 @@ class JoinBatch:
 
     def set_load_factor(self, load_factor: float):
         """(Re)allocates the hash tables for a load factor and rewrites the
-        query descriptors; the columns are kept."""
+        query descriptors; the columns are kept.  Tables lie in order of
+        decreasing build size, so the tables an LDS batch builds (the builds
+        between a threshold and the slice limit) are one contiguous span."""
         t_bytes = 0
-        for x in self.q:
+        for x in sorted(self.q, key=lambda y: -y.B):
             nb, nbytes = mgv.ht_plan(x.B, load_factor)
             x.nbuckets = nb
             x.t_off, t_bytes = t_bytes, t_bytes + _align(nbytes, 256)
@@ -334,37 +336,61 @@ class JoinBatch:
         dr = mgv.DeviceBuffer(C.sizeof(regions)).upload(np.frombuffer(bytes(regions), dtype=np.uint8))
         ws = mgv.DeviceBuffer(max(256, plan.workspace_bytes))
         out = {"ldsb": True, "plan": plan, "dq": dq, "di": di, "dr": dr, "ws": ws,
-               "queries": sum(1 for i in range(self.nq) if regions[4 * i + 2])}
+               "queries": sum(1 for i in range(self.nq) if regions[4 * i + 2]),
+               "span": self._lds_span([i for i in range(self.nq) if regions[4 * i + 2]]) if builds else None}
         self._plans[key] = out
         return out
 
+    def _lds_span(self, built):
+        """Byte span [lo, hi) of the tables of the queries in built when they
+        lie contiguously (no other table between them), else None."""
+        if not built:
+            return None
+        offs = sorted((self.q[i].t_off for i in built))
+        lo = offs[0]
+        hi = max(self.q[i].t_off for i in built)
+        hi = min((x.t_off for x in self.q if x.t_off > hi), default=self.t_bytes)
+        inside = [x for x in self.q if lo <= x.t_off < hi]
+        return (lo, hi) if len(inside) == len(built) else None
+
     def issue_ldsb(self, vector_rows: int, launch: mgv.Launch, stream=None, min_rows: int = 1 << 16,
-                   lds_builds: bool = True):
+                   lds_builds: bool = True, fresh: bool = True):
         """clear + the builds (through the LDS batch when lds_builds, else
         the fused build launch) + the LDS batch probe (mgv_batch_run_lds:
         probes of at least min_rows rows scattered by table slice, each
-        (query, slice) probed from LDS)."""
+        (query, slice) probed from LDS).  With fresh, the LDS-built tables
+        (one contiguous span) are not cleared: MGV_LAUNCH_FRESH_TABLES writes
+        them whole."""
         s = stream if stream is not None else self.st
         (bq, bi, bn), _ = self._plan(vector_rows)
         lp = self._ldsb(vector_rows, min_rows)
-        self.clear(s)
+        lb = self._ldsb(vector_rows, min_rows, builds=True) if lds_builds else None
+        span = lb["span"] if (lb is not None and fresh) else None
+        if span is None:
+            self.clear(s)
+        else:
+            for a, b in ((0, span[0]), (span[1], self.t_bytes)):
+                if b > a:
+                    mgv.call("mgv_fill_u64", self.tables.ptr + a, (b - a) // 8, 0xFFFFFFFFFFFFFFFF, s)
+            self.res.clear(s)
         if lds_builds:
-            lb = self._ldsb(vector_rows, min_rows, builds=True)
+            Lb = mgv.Launch(launch.grid, launch.block, launch.compact,
+                            launch.reserved | (mgv.MGV_LAUNCH_FRESH_TABLES if span is not None else 0))
             mgv.call("mgv_batch_run_lds", lb["dq"].vp, lb["di"].vp, lb["dr"].vp, C.byref(lb["plan"]), lb["ws"].vp,
-                     lb["ws"].nbytes, launch, s)
+                     lb["ws"].nbytes, Lb, s)
         else:
             mgv.call("mgv_batch_run", bq.vp, bi.vp, bn, self.counter.vp, launch, s)
         mgv.call("mgv_batch_run_lds", lp["dq"].vp, lp["di"].vp, lp["dr"].vp, C.byref(lp["plan"]), lp["ws"].vp,
                  lp["ws"].nbytes, launch, s)
 
     def capture_ldsb(self, vector_rows: int, launch: mgv.Launch, stream, min_rows: int = 1 << 16,
-                     lds_builds: bool = True) -> C.c_void_p:
+                     lds_builds: bool = True, fresh: bool = True) -> C.c_void_p:
         self._plan(vector_rows)
         self._ldsb(vector_rows, min_rows)
         if lds_builds:
             self._ldsb(vector_rows, min_rows, builds=True)
         mgv.call("mgv_graph_begin_capture", stream)
-        self.issue_ldsb(vector_rows, launch, stream, min_rows, lds_builds)
+        self.issue_ldsb(vector_rows, launch, stream, min_rows, lds_builds, fresh)
         g = C.c_void_p()
         mgv.call("mgv_graph_end_capture", stream, C.byref(g))
         return g

@@ -50,10 +50,12 @@ def upload(arr):
     return mgv.DeviceBuffer(max(16, C.sizeof(arr))).upload(np.frombuffer(bytes(arr), dtype=np.uint8))
 
 
-def build_tables(gpu, qs, lf=0.5, lds_min_rows=None):
+def build_tables(gpu, qs, lf=0.5, lds_min_rows=None, fresh=False):
     """The queries' tables, built by one fused launch, or (lds_min_rows set)
     by the LDS batch build (builds of at least lds_min_rows rows whose table
-    splits into 1024-bucket slices; the others through the fused launch)."""
+    splits into 1024-bucket slices; the others through the fused launch).
+    fresh: the LDS-built tables are first filled with a non-empty pattern
+    and built with MGV_LAUNCH_FRESH_TABLES, which must overwrite it."""
     nq = len(qs)
     tables = [mgv.HashTable(q["B"], lf) for q in qs]
     res = mgv.ResultBuffer(2 * nq)   # [0, nq): builds, [nq, 2nq): probes
@@ -82,8 +84,14 @@ def build_tables(gpu, qs, lf=0.5, lds_min_rows=None):
         assert plan.nparts == 0
         dq, di, dr = upload(build_q), upload(items), upload(regions)
         ws = mgv.DeviceBuffer(max(256, plan.workspace_bytes))
-        gpu.call("mgv_batch_run_lds", dq.vp, di.vp, dr.vp, C.byref(plan), ws.vp, ws.nbytes, mgv.launch(0, 256, 1),
-                 None)
+        L = mgv.launch(0, 256, 1)
+        if fresh:
+            built = [i for i in range(nq) if regions[4 * i + 2]]
+            assert built
+            for i in built:
+                gpu.call("mgv_fill_u64", tables[i].buf.vp, tables[i].nbuckets * 8, 0x0123456789ABCDEF, None)
+            L = mgv.Launch(0, 256, 1, mgv.MGV_LAUNCH_FRESH_TABLES)
+        gpu.call("mgv_batch_run_lds", dq.vp, di.vp, dr.vp, C.byref(plan), ws.vp, ws.nbytes, L, None)
     mgv.sync()
     assert all(res.read(i).errors == 0 for i in range(nq))
     return tables, res, probe_q
@@ -258,14 +266,16 @@ def test_lds_batch_skewed_keys_and_duplicate_build_keys(gpu, orc):
         assert (r.count, r.sum) == (q["want"]["count"], q["want"]["sum"]), (q["P"], q["bound"])
 
 
+@pytest.mark.parametrize("fresh", [False, True])
 @pytest.mark.parametrize("lf", [0.5, 0.75])
-def test_lds_batch_build_equals_fused_build(gpu, orc, lf):
+def test_lds_batch_build_equals_fused_build(gpu, orc, lf, fresh):
     """Tables built by the LDS batch build hold exactly the slot multiset of
     the fused build's tables and fill every bucket's slots in order (unique
-    and duplicate build keys)."""
+    and duplicate build keys); with MGV_LAUNCH_FRESH_TABLES also over tables
+    that held other slots before the call."""
     qs = make_queries(orc, 10, seed_cfg=17, lo_log=18.1, hi_log=20, dup=(2, 7))
     a, _, _ = build_tables(gpu, qs, lf)
-    b, _, _ = build_tables(gpu, qs, lf, lds_min_rows=0)
+    b, _, _ = build_tables(gpu, qs, lf, lds_min_rows=0, fresh=fresh)
     empty = np.uint64(0xFFFFFFFFFFFFFFFF)
     for ta, tb in zip(a, b):
         sa, sb = ta.slots(), tb.slots()
@@ -291,6 +301,8 @@ def test_lds_batch_equals_fused_on_graph_replays(gpu, orc):
     gpu.call("mgv_graph_destroy", g)
     g = batch.capture_ldsb(4096, L, st.vp, min_rows=1 << 14)
     assert batch._ldsb(4096, 1 << 14)["queries"] >= 4
+    # the LDS-built tables are one span, left uncleared: every replay writes them whole
+    assert batch._ldsb(4096, 1 << 14, True)["span"] is not None
     for rep in range(3):
         gpu.call("mgv_graph_launch", g, st.vp)
         st.sync()
