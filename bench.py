@@ -871,13 +871,14 @@ def leg_c4(args, st):
         r = batch.results()
         same = bool(np.array_equal(r["count"], fused["count"]) and np.array_equal(r["sum"], fused["sum"]))
         mgv.call("mgv_graph_destroy", gl)
-        ldsb[lfl] = (lw, le, same, batch._ldsb(lV, lmin)["queries"])
+        # fresh_tables: the LDS-built tables lie in one span, left uncleared
+        ldsb[lfl] = (lw, le, same, batch._ldsb(lV, lmin)["queries"], batch._ldsb(lV, lmin, True)["span"] is not None)
         log(f"C4 LDS batch at load factor {lfl} ({ldsb[lfl][3]} queries >= {lmin} rows, V={lV}): {min(lw) * 1e3:.2f} ms "
             f"wall, equal to fused: {same}")
     if batch.lf != lf:
         batch.set_load_factor(lf)
     ldsb_lf = min(ldsb, key=lambda k: min(ldsb[k][0]))
-    lw, le, ldsb_same, nlq = ldsb[ldsb_lf]
+    lw, le, ldsb_same, nlq, lfresh = ldsb[ldsb_lf]
     ldsb_same = all(v[2] for v in ldsb.values())
     plain_out = {}
     plain_res = None
@@ -927,6 +928,7 @@ def leg_c4(args, st):
                    "sliced_queries": nsl, "min_rows": smin, "regions": sreg},
         "lds_batch": {"wall_ms": min(lw) * 1e3, "gpu_ms": min(le), "queries_per_s": qps["lds_batch"],
                       "lds_queries": nlq, "min_rows": lmin, "vector_rows": lV, "load_factor": ldsb_lf,
+                      "fresh_tables": lfresh,
                       "wall_ms_by_load_factor": {str(k): min(v[0]) * 1e3 for k, v in ldsb.items()}},
         "plain": plain_out,
         "speedup_vs_best_plain": qps[best_path] / best_plain["queries_per_s"] if best_plain else None,
